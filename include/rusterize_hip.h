@@ -1,0 +1,261 @@
+/*
+ * rusterize_hip.h -- C ABI of the MI355X (gfx950) rasteriser.
+ *
+ * This is the drop-in boundary for rusterize's rasterisation path.  Every
+ * entry point below names the reference interface it replaces (paths are
+ * relative to the rusterize source tree).  All calls are plain C: pointers,
+ * sizes and enums, no C++ or torch types, no exceptions across the boundary.
+ *
+ * Ownership: input buffers are borrowed for the duration of a call (the
+ * reference borrows `&[Geometry]` and `ArrayView1`, rust/src/rasterize.rs:54-62);
+ * outputs are caller-allocated.  Errors are returned as rz_status and the
+ * message of the last failing call on the calling thread is available from
+ * rz_last_error(); messages reuse the reference's wording
+ * (rust/src/error.rs:3-12, rust/src/geo/raster.rs:50-156,
+ * rust/src/rasterize.rs:207-229, rust/src/rasterization/pixel_functions.rs:18-33).
+ *
+ * Threading: calls on distinct rz_context objects may run concurrently;
+ * one context serialises its work on its own HIP stream.
+ */
+#ifndef RUSTERIZE_HIP_H
+#define RUSTERIZE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- enums -- */
+
+typedef enum rz_status {
+  RZ_OK = 0,
+  RZ_VALUE_ERROR = 1,   /* RusterizeError::ValueError  (rust/src/error.rs:8) */
+  RZ_RUNTIME_ERROR = 2, /* RusterizeError::RuntimeError (rust/src/error.rs:6) */
+  RZ_TYPE_ERROR = 3,    /* unsupported dtype / encoding (python/src/rusterize.rs:28) */
+  RZ_DEVICE_ERROR = 4,  /* HIP runtime failure or no usable gfx950 device */
+  RZ_PARSE_ERROR = 5,   /* WKT / WKB decoding failure (python/src/geo/parse_geometry.rs:77-88) */
+  RZ_CAPACITY_ERROR = 6 /* request exceeds a stated limit (memory budget, index width) */
+} rz_status;
+
+/* Output element type; order follows python/src/rusterize.rs:171-182. */
+typedef enum rz_dtype {
+  RZ_U8 = 0, RZ_U16 = 1, RZ_U32 = 2, RZ_U64 = 3,
+  RZ_I8 = 4, RZ_I16 = 5, RZ_I32 = 6, RZ_I64 = 7,
+  RZ_F32 = 8, RZ_F64 = 9
+} rz_dtype;
+
+/* Pixel function; rust/src/rasterization/pixel_functions.rs:8-16. */
+typedef enum rz_fun {
+  RZ_SUM = 0, RZ_FIRST = 1, RZ_LAST = 2, RZ_MIN = 3, RZ_MAX = 4, RZ_COUNT = 5, RZ_ANY = 6
+} rz_fun;
+
+/* How `sum` is reduced on the device.  ORDERED folds every pixel in
+ * geometry order and is bit-identical to the reference.  UNORDERED lets the
+ * tile lists keep arbitrary geometry order (no stable sort); float sums then
+ * differ from the reference by rounding only (bound stated in the tests). */
+typedef enum rz_sum_mode { RZ_SUM_ORDERED = 0, RZ_SUM_UNORDERED = 1 } rz_sum_mode;
+
+/* Item kinds of the flattened geometry model (see rz_geometry). */
+typedef enum rz_item_kind {
+  RZ_ITEM_POLYGONS = 0, /* all rings of a Polygon / MultiPolygon: one even-odd edge list */
+  RZ_ITEM_LINES = 1,    /* all lines of a LineString / MultiLineString / Line           */
+  RZ_ITEM_POINTS = 2    /* a Point or the points of a MultiPoint                        */
+} rz_item_kind;
+
+/* ------------------------------------------------------------ raster info -- */
+
+/* rust/src/geo/raster.rs:9-20 (RasterInfo).  epsg < 0 means "None". */
+typedef struct rz_raster_info {
+  uint64_t ncols;
+  uint64_t nrows;
+  double xmin, xmax, ymin, ymax;
+  double xres, yres;
+  int32_t epsg;
+} rz_raster_info;
+
+/* --------------------------------------------------------- geometry model -- */
+
+/*
+ * Flattened, GeoArrow-like geometry set.  Geometry g (one field value, one
+ * `by` key) owns items [geom_item_offsets[g], geom_item_offsets[g+1]); item i
+ * has kind item_kind[i] and rings [item_ring_offsets[i], item_ring_offsets[i+1]);
+ * ring r has vertices [ring_vertex_offsets[r], ring_vertex_offsets[r+1]) in
+ * xy (interleaved x, y world coordinates).
+ *
+ * The mapping from geo-types follows the burn dispatch of
+ * rust/src/rasterization/burn_geometry.rs:24-210:
+ *   Point            -> one POINTS item with one vertex            (31-36)
+ *   MultiPoint       -> one POINTS item, one ring of n vertices    (37-44)
+ *   Polygon          -> one POLYGONS item: exterior + holes        (76-118)
+ *   MultiPolygon     -> one POLYGONS item: every ring of every part (120-166)
+ *   LineString/Line  -> one LINES item with one ring               (56-59, 168-187)
+ *   MultiLineString  -> one LINES item: one ring per line          (189-210)
+ *   GeometryCollection -> its members' items, in order, recursively (64-74)
+ * Polygon rings are closed (first vertex repeated) as geo-types'
+ * Polygon::new does; line rings are kept as given.
+ */
+typedef struct rz_geometry {
+  int64_t n_geoms;
+  int64_t n_items;
+  int64_t n_rings;
+  int64_t n_vertices;
+  const int64_t* geom_item_offsets;   /* [n_geoms + 1] */
+  const int32_t* item_kind;           /* [n_items]     */
+  const int64_t* item_ring_offsets;   /* [n_items + 1] */
+  const int64_t* ring_vertex_offsets; /* [n_rings + 1] */
+  const double* xy;                   /* [2 * n_vertices] */
+  const uint8_t* ring_flags;          /* [n_rings] or NULL; bit 0 = polygon interior ring.
+                                         Only extent inference reads it: geo's BoundingRect
+                                         of a Polygon uses the exterior ring alone. */
+} rz_geometry;
+
+/* Owned geometry buffers produced by the parsers / generators below. */
+typedef struct rz_geometry_buffers rz_geometry_buffers;
+
+/* ------------------------------------------------------------------ errors -- */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* rz_last_error(void);
+
+/* Library version string. */
+const char* rz_version(void);
+
+/* ------------------------------------------------------------ host helpers -- */
+
+/* Replaces RasterInfoBuilder::{build, build_with, finalize}
+ * (rust/src/geo/raster.rs:50-156) as driven by RawRasterInfo::build
+ * (python/src/geo/raster.rs:16-42).  When has_extent is 0 the extent is
+ * inferred from `geoms` (bounding rectangle of every vertex). */
+rz_status rz_raster_info_build(const rz_geometry* geoms, int has_shape, uint64_t nrows,
+                               uint64_t ncols, int has_extent, const double* extent4,
+                               int has_res, double xres, double yres, int tap, int32_t epsg,
+                               rz_raster_info* out);
+
+/* Replaces `impl FromStr for PixelFunction`
+ * (rust/src/rasterization/pixel_functions.rs:18-33). */
+rz_status rz_pixel_function_parse(const char* name, rz_fun* out);
+
+/* Replaces the dtype match of python/src/rusterize.rs:166-183. */
+rz_status rz_dtype_parse(const char* name, rz_dtype* out);
+size_t rz_dtype_size(rz_dtype dtype);
+
+/* Geometry ingestion; replaces parse_sequence_wkt / parse_sequence_wkb and
+ * try_parse_*_to_geometry (python/src/geo/parse_geometry.rs:77-133).
+ * `texts` / `blobs` are n pointers with explicit byte lengths. WKB geometries
+ * that geo cannot represent (empty points) are skipped like the reference's
+ * filter at parse_geometry.rs:113-116. */
+rz_status rz_geometry_from_wkt(const char* const* texts, const int64_t* lengths, int64_t n,
+                               rz_geometry_buffers** out);
+rz_status rz_geometry_from_wkb(const uint8_t* const* blobs, const int64_t* lengths, int64_t n,
+                               rz_geometry_buffers** out);
+/* Build from caller-supplied flat arrays (copied). */
+rz_status rz_geometry_from_arrays(const rz_geometry* src, rz_geometry_buffers** out);
+void rz_geometry_view(const rz_geometry_buffers* bufs, rz_geometry* view);
+void rz_geometry_free(rz_geometry_buffers* bufs);
+
+/* Seeded synthetic workloads (SplitMix64) used by tests and benchmarks.
+ * kind: 1 = star polygons, 2 = convex polygons with parts/holes,
+ *       3 = random-walk lines + points + collections, 4 = mixed job.
+ * The four integer knobs are interpreted per kind (documented in
+ * rusterize_amd/csrc/rz_synth.cpp). */
+rz_status rz_synth_geometry(int kind, uint64_t seed, int64_t count, double grid_w, double grid_h,
+                            double p0, double p1, double p2, double p3,
+                            rz_geometry_buffers** out);
+/* Fill n values of `dtype` from a seeded distribution:
+ * dist 0 = uniform [lo, hi) (ints: integer uniform, cast with numpy wrap),
+ * dist 1 = standard normal rounded to the dtype. */
+rz_status rz_synth_field(rz_dtype dtype, int dist, uint64_t seed, int64_t n, double lo, double hi,
+                         void* out);
+
+/* --------------------------------------------------------- rasterisation -- */
+
+/*
+ * Host-in / host-out rasterisation: replaces
+ *   `geoms.rasterize::<DenseArray<N>>(ctx)` (rust/src/rasterize.rs:54-116) and
+ *   `rusterize_py_impl` for dense encodings (python/src/rusterize.rs:45-125).
+ * field: n_geoms values of `dtype` (FieldSource::Array) or NULL, in which case
+ *   *scalar is used for every geometry (FieldSource::Scalar).
+ * layer_of_geom: NULL for one band, else the band index of every geometry
+ *   (the BTreeMap grouping of rust/src/rasterize.rs:198-205 done by the caller:
+ *   bands in byte-lexicographic key order); n_layers bands are produced.
+ * out: n_layers * nrows * ncols elements of `dtype`, C order (band, row, col).
+ */
+rz_status rz_rasterize(const rz_geometry* geoms, const rz_raster_info* info, rz_dtype dtype,
+                       rz_fun fun, const void* field, const void* scalar,
+                       const uint32_t* layer_of_geom, uint32_t n_layers, const void* background,
+                       int all_touched, rz_sum_mode sum_mode, void* out);
+
+/* ------------------------------------------------- device-resident runtime -- */
+
+typedef struct rz_context rz_context;
+typedef struct rz_device_geometry rz_device_geometry;
+
+/* A context owns one HIP stream (default priority) and a grow-only workspace
+ * on `device`. */
+rz_status rz_context_create(int device, rz_context** out);
+rz_status rz_context_destroy(rz_context* ctx);
+rz_status rz_context_synchronize(rz_context* ctx);
+/* Workspace bytes currently held by the context. */
+size_t rz_context_workspace_bytes(const rz_context* ctx);
+
+rz_status rz_device_malloc(rz_context* ctx, size_t bytes, void** out);
+rz_status rz_device_free(rz_context* ctx, void* ptr);
+rz_status rz_memcpy_h2d(rz_context* ctx, void* dst, const void* src, size_t bytes);
+rz_status rz_memcpy_d2h(rz_context* ctx, void* dst, const void* src, size_t bytes);
+
+/* Uploads a geometry set once; it stays resident until freed and can be
+ * rasterised any number of times with different fields and grids. */
+rz_status rz_device_geometry_upload(rz_context* ctx, const rz_geometry* host,
+                                    rz_device_geometry** out);
+rz_status rz_device_geometry_free(rz_device_geometry* dg);
+int64_t rz_device_geometry_count(const rz_device_geometry* dg);
+
+/* Same semantics as rz_rasterize with every array on the device:
+ * field_dev (or NULL + scalar_host), layer_dev (or NULL), out_dev.
+ * Asynchronous on the context's stream. */
+rz_status rz_rasterize_device(rz_context* ctx, const rz_device_geometry* dg,
+                              const rz_raster_info* info, rz_dtype dtype, rz_fun fun,
+                              const void* field_dev, const void* scalar_host,
+                              const uint32_t* layer_dev, uint32_t n_layers,
+                              const void* background_host, int all_touched,
+                              rz_sum_mode sum_mode, void* out_dev);
+
+/* Stage timings of the last rz_rasterize_device call on ctx, measured with
+ * HIP events on the context's stream (requires rz_context_set_profiling(ctx,1)).
+ * Writes up to n values (milliseconds) in the order given by
+ * rz_stage_name(i); returns the number of stages. */
+rz_status rz_context_set_profiling(rz_context* ctx, int enabled);
+int rz_context_stage_times(rz_context* ctx, double* ms, int n);
+const char* rz_stage_name(int i);
+
+/* Counters of the last call: pixel-burns (writes reaching the dense writer,
+ * rust/src/encoding/writers.rs:68-71), tile/item pairs and tiles. Requires
+ * rz_context_set_profiling(ctx, 1) for burns. */
+rz_status rz_context_last_counters(rz_context* ctx, uint64_t* burns, uint64_t* pairs,
+                                   uint64_t* tiles);
+
+/* ------------------------------------------------------------ test hooks -- */
+
+/* No reference counterpart: host evaluations of the closed forms the tile
+ * kernel uses, so CPU tests can check them against the literal reference
+ * loops by brute force.
+ * rz_debug_walk_cells: the cells Standard::burn_line
+ * (rust/src/rasterization/burners.rs:53-84) visits between integer cells
+ * (ix0,iy0) -> (ix1,iy1), end cell excluded, restricted to [X0,X1) x [Y0,Y1);
+ * returns the count and stores up to cap cells.
+ * rz_debug_row_coverage: which of the columns [C0, C0+width) the spans of
+ * burn_polygon (burners.rs:302-316) cover for the crossings xs[0..n) of one
+ * row; width is 64 or 128. */
+int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, int64_t iy1, int64_t X0,
+                            int64_t X1, int64_t Y0, int64_t Y1, int64_t* xs, int64_t* ys,
+                            int64_t cap);
+int rz_debug_row_coverage(const double* xs, int64_t n, int64_t ncols, int32_t C0, int32_t width,
+                          uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RUSTERIZE_HIP_H */

@@ -1,0 +1,241 @@
+// rz_kernels.h -- gfx950 kernels of the rasteriser.
+//
+// Pipeline of one rasterise call (all on one HIP stream):
+//   k_prep        one thread per vertex: world->pixel transform, polygon edge
+//                 records (rust/src/geo/edges.rs:27-46, 90-110), line / point
+//                 pixel coordinates (edges.rs:79-88, 112-134) and a per-item
+//                 pixel bounding box (segmented wave reduction + atomics).
+//   k_bin_count / scan / k_bin_emit / stable radix sort / k_tile_offsets
+//                 sort-middle binning of items into 64-row tiles, keeping
+//                 geometry order inside every tile.
+//   k_tile        one workgroup per output tile: the background fill of
+//                 RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and
+//                 every burn of every item of the tile, folded in geometry
+//                 order in registers, then written to HBM exactly once.
+//
+// Exactness: all f64 arithmetic is compiled with -ffp-contract=off, divisions
+// and floor/ceil are IEEE exact, so every crossing and every pixel-centre
+// rounding equals the reference's CPU value bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace rzk {
+
+constexpr int TILE_H = 64;       // rows per tile (4 waves x 16 rows)
+constexpr int WAVE_ROWS = 16;    // rows per wave strip
+constexpr int NUM_XCD = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// Tile width in pixels for an element size; rows of 128-256 bytes.
+__host__ __device__ constexpr int tile_w(int esz) { return esz <= 2 ? 128 : 64; }
+
+// Edge record of one polygon segment (PolyEdge, rust/src/geo/edges.rs:17-24).
+// ystart / yend are the reference's usize values clamped to nrows, which is
+// exact for the row test `ystart <= r < yend` with r < nrows.  Invalid
+// (skipped) segments have ystart == yend == 0.
+struct __align__(32) PolyEdge {
+  double x0, y0, dxdy;
+  int32_t ystart, yend;
+};
+
+enum : uint8_t { VF_HAS_NEXT = 1, VF_CLOSED = 2 };
+
+// Per-item pixel bounding box: rows [r0, r1), cols [c0, c1).
+struct __align__(16) ItemBox {
+  int32_t r0, r1, c0, c1;
+};
+
+struct PrepArgs {
+  const double* xy;
+  const int32_t* vert_ring;
+  const int64_t* ring_voff;
+  const int32_t* ring_item;
+  const int32_t* item_kind;
+  int64_t n_vertices;
+  double xmin, ymax, xres, yres;
+  int32_t nrows, ncols;
+  PolyEdge* edges;        // [n_vertices]
+  double2* px;            // [n_vertices] pixel coordinates (lines / points)
+  uint8_t* vflags;        // [n_vertices]
+  ItemBox* item_box;      // [n_items]
+  unsigned long long* item_last;  // [n_items] last kept line segment + 1 (0: none)
+};
+
+struct BinArgs {
+  const ItemBox* item_box;
+  const int32_t* item_geom;
+  const uint32_t* layer_of_geom;  // may be null
+  int64_t n_items;
+  int32_t ntx, nty, tile_w;
+  uint32_t tiles_per_layer;
+  uint32_t* counts;        // [n_items]
+  const uint64_t* offsets; // [n_items + 1] exclusive prefix of counts
+  uint32_t* keys;          // [n_pairs]
+  uint32_t* vals;          // [n_pairs]
+};
+
+struct TileArgs {
+  // grid
+  int32_t nrows, ncols, ntx, nty;
+  uint32_t tiles_per_layer, n_tiles;
+  // binned mode: pairs sorted by tile, tile_off[t]..tile_off[t+1]
+  const uint32_t* tile_off;
+  const uint32_t* pair_item;
+  // scan mode (pair_item == nullptr): every item is tested against the tile
+  int64_t n_items;
+  // items
+  const int32_t* item_kind;
+  const ItemBox* item_box;
+  const int32_t* item_geom;
+  const int64_t* item_vbeg;  // [n_items + 1] vertex range per item (items are contiguous)
+  const unsigned long long* item_last;
+  const uint32_t* layer_of_geom;  // may be null
+  const PolyEdge* edges;
+  const double2* px;
+  const uint8_t* vflags;
+  // values
+  const void* field;       // [n_geoms] of T or null
+  uint64_t scalar_bits, bg_bits;
+  int dedup_lines;         // xres != yres (burn_geometry.rs:179, 202)
+  // output
+  void* out;               // [layers][nrows][ncols] of T
+  unsigned long long* burns;  // optional counter
+  uint32_t* status;        // error flags
+};
+
+// ------------------------------------------------ polygon row coverage --
+//
+// Pixel c of a row is inside a reference span [a_2k, a_2k+1) (the sorted,
+// paired crossings of rust/src/rasterization/burners.rs:302-316) iff the
+// number m of crossings whose span column a_i <= c is odd and c < max_i a_i:
+// sorted columns are non-decreasing, so m odd puts c in [a_{m-1}, a_m), and
+// m == n (c >= max a) is the dropped odd tail of chunks_exact(2).  Crossings
+// that f64::total_cmp sorts after +inf (positive NaN) have column 0 but sit at
+// the end of the sorted list, where they only pair into empty spans; they are
+// therefore ignored.  This needs no sort: per crossing one toggle bit at its
+// column, then an inclusive prefix-XOR along the row.
+
+// Span column of a crossing: `(x + 0.5).floor().clamp(0.0, ncols) as usize`
+// (burners.rs:310-311); NaN -> 0.
+__host__ __device__ inline int32_t span_col(double x, double ncols_f) {
+  double f = floor(x + 0.5);
+  if (f < 0.0) f = 0.0;
+  if (f > ncols_f) f = ncols_f;
+  return (f > 0.0) ? (int32_t)f : 0;
+}
+
+__host__ __device__ inline bool is_pos_nan(double x) { return x != x && !signbit(x); }
+
+// Adds one crossing column to the toggle state of a TW-wide row window
+// starting at C0: `lp` is the parity of crossings left of the window.
+__host__ __device__ inline void toggle_crossing(int32_t col, int32_t C0, int32_t TW,
+                                                uint64_t* tog, uint32_t& lp, int32_t& amax) {
+  amax = col > amax ? col : amax;
+  const int32_t rel = col - C0;
+  if (rel <= 0) lp ^= 1u;
+  else if (rel < TW) tog[rel >> 6] ^= 1ull << (rel & 63);
+}
+
+// Coverage bits of the PPT pixels [C0+seg_lo, C0+seg_lo+PPT) from the
+// toggles of the whole window (NWORD words).
+template <int PPT, int NWORD>
+__host__ __device__ inline uint64_t segment_coverage(const uint64_t* tog, uint32_t lp,
+                                                     int32_t amax, int32_t C0, int32_t seg_lo) {
+  uint32_t carry = lp;
+  for (int w = 0; w < NWORD; ++w) {
+    const int lo = w * 64;
+    if (lo + 64 <= seg_lo) carry ^= (uint32_t)__builtin_popcountll(tog[w]) & 1u;
+    else if (lo < seg_lo)
+      carry ^= (uint32_t)__builtin_popcountll(tog[w] & ((1ull << (seg_lo - lo)) - 1ull)) & 1u;
+  }
+  constexpr uint64_t FULL = PPT >= 64 ? ~0ull : ((1ull << PPT) - 1ull);
+  uint64_t seg = (tog[seg_lo >> 6] >> (seg_lo & 63)) & FULL;
+  seg ^= seg << 1;
+  seg ^= seg << 2;
+  seg ^= seg << 4;
+  seg ^= seg << 8;
+  seg ^= seg << 16;
+  seg ^= seg << 32;
+  if (carry) seg = ~seg;
+  const int32_t lim = amax - (C0 + seg_lo);
+  const uint64_t lm = lim <= 0 ? 0ull : (lim >= PPT ? FULL : ((1ull << lim) - 1ull));
+  return seg & lm;
+}
+
+// Cells of Standard::burn_line for one segment between integer cells
+// (rust/src/rasterization/burners.rs:53-84), excluding the end cell, that
+// fall in the window [X0,X1) x [Y0,Y1), in loop order.  Closed form of the
+// loop: with dx = |ix1-ix0|, D = |iy1-iy0|, an x-major segment (dx >= D)
+// visits on iteration k in [0, dx)
+//     (ix0 + sx*k, iy0 + sy*floor((2*D*k + dx) / (2*dx)))
+// and a y-major one (D > dx) visits on iteration k in [0, D)
+//     (ix0 + sx*floor((2*dx*k + D) / (2*D)), iy0 + sy*k).
+// (err after i x-steps and j y-steps is dx*(1+j) - D*(1+i); the x-major loop
+// steps x every iteration and y exactly when dx*(2j+1) <= 2*D*(k+1), which is
+// the recurrence of that floor; symmetrically for y-major.)  The window
+// bounds the k range, so a long segment costs only its cells in the tile.
+// I must hold 2*max(dx,D)^2: int64_t for |coords| < 2^30, __int128 beyond.
+template <typename I, typename F>
+__host__ __device__ inline void walk_window(int64_t ix0, int64_t iy0, int64_t ix1, int64_t iy1,
+                                            int64_t X0, int64_t X1, int64_t Y0, int64_t Y1,
+                                            F&& emit) {
+  const int64_t dxs = ix1 - ix0, dys = iy1 - iy0;
+  const I dx = (I)(dxs < 0 ? -dxs : dxs), D = (I)(dys < 0 ? -dys : dys);
+  const int64_t sx = ix0 < ix1 ? 1 : -1, sy = iy0 < iy1 ? 1 : -1;
+  const bool xmaj = dx >= D;
+  const I n = xmaj ? dx : D;  // iterations (major steps)
+  const I m = xmaj ? D : dx;  // minor steps in total
+  if (n == 0) return;
+  const int64_t u0 = xmaj ? ix0 : iy0, w0 = xmaj ? iy0 : ix0;
+  const int64_t su = xmaj ? sx : sy, sw = xmaj ? sy : sx;
+  const int64_t U0 = xmaj ? X0 : Y0, U1 = xmaj ? X1 : Y1;
+  const int64_t W0 = xmaj ? Y0 : X0, W1 = xmaj ? Y1 : X1;
+  I klo, khi;
+  {
+    int64_t lo, hi;  // k in [lo, hi] from the major-axis window
+    if (su > 0) { lo = U0 - u0; hi = U1 - 1 - u0; }
+    else { lo = u0 - (U1 - 1); hi = u0 - U0; }
+    if (hi < 0 || (I)lo >= n || lo > hi) return;
+    klo = (I)(lo < 0 ? 0 : lo);
+    khi = (I)hi >= n ? n - 1 : (I)hi;
+  }
+  int64_t jlo, jhi;  // minor index window
+  if (sw > 0) { jlo = W0 - w0; jhi = W1 - 1 - w0; }
+  else { jlo = w0 - (W1 - 1); jhi = w0 - W0; }
+  if (jhi < 0 || (I)jlo > m || jlo > jhi) return;
+  const I two_n = (I)2 * n, two_m = (I)2 * m;
+  if (m == 0) {
+    if (jlo > 0) return;
+  } else {
+    if (jlo > 0) {  // j(k) >= jlo  <=>  k >= ceil((2n*jlo - n) / 2m)
+      const I num = two_n * (I)jlo - n;
+      const I kmin = (num + two_m - 1) / two_m;
+      if (kmin > klo) klo = kmin;
+    }
+    if ((I)jhi < m) {  // j(k) <= jhi  <=>  k <= ceil((2n*(jhi+1) - n) / 2m) - 1
+      const I num = two_n * (I)(jhi + 1) - n;
+      const I kmax = (num + two_m - 1) / two_m - 1;
+      if (kmax < khi) khi = kmax;
+    }
+  }
+  if (klo > khi) return;
+  // j and its remainder at klo, then stepped: each iteration adds 2m to the
+  // numerator and at most one 2n (m <= n).
+  I numer = two_m * klo + n;
+  I j = numer / two_n;
+  I rem = numer - j * two_n;
+  for (I k = klo; k <= khi; ++k) {
+    const int64_t u = u0 + su * (int64_t)k, w = w0 + sw * (int64_t)j;
+    if (xmaj) emit(u, w);
+    else emit(w, u);
+    rem += two_m;
+    if (rem >= two_n) {
+      rem -= two_n;
+      ++j;
+    }
+  }
+}
+
+}  // namespace rzk

@@ -1,0 +1,634 @@
+// rz_runtime.hip -- device runtime of librusterize_hip: contexts, resident
+// geometry, workspace and the launch sequence of one rasterise call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <rocprim/rocprim.hpp>
+#include <string>
+#include <vector>
+
+#include "rz_internal.h"
+#include "rz_kernels.h"
+
+// The kernels are compiled in this translation unit so every k_tile<T, FUN>
+// instantiation used by the dispatch below is emitted here.
+#include "rz_kernels.hip"
+
+using namespace rzk;
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      return rz::fail(RZ_DEVICE_ERROR, std::string("HIP error '") + hipGetErrorString(_e) + \
+                                           "' at " #expr);                                 \
+  } while (0)
+
+namespace {
+
+constexpr int kStages = 4;
+const char* kStageNames[kStages] = {"prep", "bin", "sort", "tile"};
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+hipError_t ensure(Buf& b, size_t bytes) {
+  if (bytes <= b.cap && b.p) return hipSuccess;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    if (e != hipSuccess) return e;
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t cap = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, cap);
+  if (e == hipSuccess) b.cap = cap;
+  return e;
+}
+
+inline unsigned blocks_for(int64_t n, int bs = 256) {
+  return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs);
+}
+
+}  // namespace
+
+struct rz_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // workspace
+  Buf edges, px, vflags, item_box, item_last, counts, offsets, keys_a, vals_a, keys_b, vals_b,
+      tile_off, temp, misc;
+  uint64_t* host_misc = nullptr;  // pinned: [0] total pairs, [1] status, [2] burns
+  bool profiling = false;
+  hipEvent_t ev[kStages + 1] = {};
+  bool have_times = false;
+  uint64_t last_pairs = 0, last_tiles = 0;
+  bool burns_pending = false;
+  std::vector<Buf*> all() {
+    return {&edges, &px,     &vflags,   &item_box, &item_last, &counts, &offsets, &keys_a,
+            &vals_a, &keys_b, &vals_b, &tile_off, &temp,     &misc};
+  }
+};
+
+struct rz_device_geometry {
+  rz_context* ctx = nullptr;
+  int64_t G = 0, I = 0, R = 0, V = 0;
+  int64_t* geom_item_off = nullptr;
+  int32_t* item_kind = nullptr;
+  int64_t* item_ring_off = nullptr;
+  int64_t* ring_voff = nullptr;
+  double* xy = nullptr;
+  int32_t* item_geom = nullptr;
+  int32_t* ring_item = nullptr;
+  int32_t* vert_ring = nullptr;
+  int64_t* item_vbeg = nullptr;
+  bool has_poly = false, has_line = false, has_point = false;
+};
+
+// ------------------------------------------------------------- contexts --
+
+static rz_status set_device(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return rz::fail(RZ_DEVICE_ERROR,
+                    "no HIP device available (rusterize_amd needs an MI355X / gfx950 GPU)");
+  if (device < 0 || device >= n) return rz::fail(RZ_DEVICE_ERROR, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return rz::fail(RZ_DEVICE_ERROR, std::string("device is ") + prop.gcnArchName +
+                                         "; librusterize_hip is built for gfx950 only");
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_context_create(int device, rz_context** out) {
+  rz_status st = set_device(device);
+  if (st) return st;
+  auto* c = new (std::nothrow) rz_context();
+  if (!c) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
+  c->device = device;
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipHostMalloc((void**)&c->host_misc, 4 * sizeof(uint64_t), hipHostMallocDefault));
+  *out = c;
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_context_destroy(rz_context* c) {
+  if (!c) return RZ_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (Buf* b : c->all())
+    if (b->p) (void)hipFree(b->p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->host_misc) (void)hipHostFree(c->host_misc);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return RZ_OK;
+}
+
+extern "C" size_t rz_context_workspace_bytes(const rz_context* c) {
+  size_t s = 0;
+  for (Buf* b : const_cast<rz_context*>(c)->all()) s += b->cap;
+  return s;
+}
+
+static rz_status check_status_word(rz_context* c) {
+  uint32_t* st = (uint32_t*)((char*)c->misc.p);
+  HIP_TRY(hipMemcpy(&c->host_misc[1], st, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  uint32_t flags = (uint32_t)c->host_misc[1];
+  if (flags & 1u)
+    return rz::fail(RZ_CAPACITY_ERROR,
+                    "line coordinates beyond +-2^60 pixels are outside the supported range");
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_context_synchronize(rz_context* c) {
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->misc.p) return check_status_word(c);
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_context_set_profiling(rz_context* c, int enabled) {
+  c->profiling = enabled != 0;
+  return RZ_OK;
+}
+
+extern "C" const char* rz_stage_name(int i) { return (i >= 0 && i < kStages) ? kStageNames[i] : ""; }
+
+extern "C" int rz_context_stage_times(rz_context* c, double* ms, int n) {
+  if (!c->have_times) return 0;
+  (void)hipSetDevice(c->device);
+  if (hipEventSynchronize(c->ev[kStages]) != hipSuccess) return 0;
+  for (int i = 0; i < kStages && i < n; ++i) {
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]);
+    ms[i] = (double)t;
+  }
+  return kStages;
+}
+
+extern "C" rz_status rz_context_last_counters(rz_context* c, uint64_t* burns, uint64_t* pairs,
+                                              uint64_t* tiles) {
+  HIP_TRY(hipSetDevice(c->device));
+  if (burns) {
+    *burns = 0;
+    if (c->burns_pending && c->misc.p) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipMemcpy(burns, (char*)c->misc.p + 8, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    }
+  }
+  if (pairs) *pairs = c->last_pairs;
+  if (tiles) *tiles = c->last_tiles;
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_device_malloc(rz_context* c, size_t bytes, void** out) {
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMalloc(out, std::max<size_t>(bytes, 1)));
+  return RZ_OK;
+}
+extern "C" rz_status rz_device_free(rz_context* c, void* p) {
+  HIP_TRY(hipSetDevice(c->device));
+  if (p) HIP_TRY(hipFree(p));
+  return RZ_OK;
+}
+extern "C" rz_status rz_memcpy_h2d(rz_context* c, void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipSetDevice(c->device));
+  if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RZ_OK;
+}
+extern "C" rz_status rz_memcpy_d2h(rz_context* c, void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipSetDevice(c->device));
+  if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RZ_OK;
+}
+
+// ----------------------------------------------------- resident geometry --
+
+template <typename T>
+static hipError_t upload(rz_context* c, T** dst, const T* src, int64_t n) {
+  hipError_t e = hipMalloc((void**)dst, std::max<size_t>(1, (size_t)n * sizeof(T)));
+  if (e != hipSuccess) return e;
+  if (n > 0) e = hipMemcpyAsync(*dst, src, (size_t)n * sizeof(T), hipMemcpyHostToDevice, c->stream);
+  return e;
+}
+
+extern "C" rz_status rz_device_geometry_free(rz_device_geometry* dg) {
+  if (!dg) return RZ_OK;
+  (void)hipSetDevice(dg->ctx->device);
+  (void)hipStreamSynchronize(dg->ctx->stream);
+  void* ptrs[] = {dg->geom_item_off, dg->item_kind, dg->item_ring_off, dg->ring_voff, dg->xy,
+                  dg->item_geom,     dg->ring_item, dg->vert_ring,     dg->item_vbeg};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete dg;
+  return RZ_OK;
+}
+
+extern "C" int64_t rz_device_geometry_count(const rz_device_geometry* dg) { return dg ? dg->G : 0; }
+
+extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry* g,
+                                               rz_device_geometry** out) {
+  rz_status st = rz::check_geometry(g);
+  if (st) return st;
+  if (g->n_items >= INT32_MAX || g->n_rings >= INT32_MAX || g->n_geoms >= INT32_MAX ||
+      g->n_vertices >= ((int64_t)1 << 40))
+    return rz::fail(RZ_CAPACITY_ERROR, "geometry set exceeds 2^31 items / rings / geometries");
+  HIP_TRY(hipSetDevice(c->device));
+  auto* dg = new (std::nothrow) rz_device_geometry();
+  if (!dg) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
+  dg->ctx = c;
+  dg->G = g->n_geoms;
+  dg->I = g->n_items;
+  dg->R = g->n_rings;
+  dg->V = g->n_vertices;
+  for (int64_t i = 0; i < g->n_items; ++i) {
+    int k = g->item_kind[i];
+    if (g->item_ring_offsets[i + 1] == g->item_ring_offsets[i]) continue;
+    dg->has_poly |= k == RZ_ITEM_POLYGONS;
+    dg->has_line |= k == RZ_ITEM_LINES;
+    dg->has_point |= k == RZ_ITEM_POINTS;
+  }
+  std::vector<int64_t> zero1(1, 0);
+  hipError_t e = upload(c, &dg->geom_item_off, g->geom_item_offsets, g->n_geoms + 1);
+  if (e == hipSuccess) e = upload(c, &dg->item_kind, g->item_kind, g->n_items);
+  if (e == hipSuccess)
+    e = upload(c, &dg->item_ring_off, g->n_items ? g->item_ring_offsets : zero1.data(),
+               g->n_items + 1);
+  if (e == hipSuccess)
+    e = upload(c, &dg->ring_voff, g->n_rings ? g->ring_vertex_offsets : zero1.data(),
+               g->n_rings + 1);
+  if (e == hipSuccess) e = upload(c, &dg->xy, g->xy, 2 * g->n_vertices);
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->item_geom, std::max<size_t>(1, 4 * g->n_items));
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->ring_item, std::max<size_t>(1, 4 * g->n_rings));
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&dg->vert_ring, std::max<size_t>(1, 4 * g->n_vertices));
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&dg->item_vbeg, std::max<size_t>(8, 8 * (g->n_items + 1)));
+  if (e != hipSuccess) {
+    rz_device_geometry_free(dg);
+    return rz::fail(RZ_DEVICE_ERROR, std::string("geometry upload failed: ") + hipGetErrorString(e));
+  }
+  if (dg->G)
+    hipLaunchKernelGGL(k_item_geom, dim3(blocks_for(dg->G)), dim3(256), 0, c->stream,
+                       dg->geom_item_off, dg->G, dg->item_geom);
+  if (dg->I)
+    hipLaunchKernelGGL(k_ring_item, dim3(blocks_for(dg->I)), dim3(256), 0, c->stream,
+                       dg->item_ring_off, dg->ring_voff, dg->I, dg->ring_item, dg->item_vbeg);
+  else
+    (void)hipMemsetAsync(dg->item_vbeg, 0, 8, c->stream);
+  if (dg->R)
+    hipLaunchKernelGGL(k_vert_ring, dim3(blocks_for(dg->R)), dim3(256), 0, c->stream,
+                       dg->ring_voff, dg->R, dg->vert_ring);
+  e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) {
+    rz_device_geometry_free(dg);
+    return rz::fail(RZ_DEVICE_ERROR, std::string("geometry upload failed: ") + hipGetErrorString(e));
+  }
+  *out = dg;
+  return RZ_OK;
+}
+
+// ------------------------------------------------------------ rasterise --
+
+namespace {
+
+template <typename T, int FUN>
+void launch_tile(const TileArgs& a, uint32_t n_tiles, hipStream_t s) {
+  hipLaunchKernelGGL((k_tile<T, FUN>), dim3(n_tiles), dim3(256), 0, s, a);
+}
+
+template <typename T>
+void launch_tile_fun(int fun, const TileArgs& a, uint32_t n_tiles, hipStream_t s) {
+  switch (fun) {
+    case 0: launch_tile<T, 0>(a, n_tiles, s); break;
+    case 1: launch_tile<T, 1>(a, n_tiles, s); break;
+    case 2: launch_tile<T, 2>(a, n_tiles, s); break;
+    case 3: launch_tile<T, 3>(a, n_tiles, s); break;
+    case 4: launch_tile<T, 4>(a, n_tiles, s); break;
+    case 5: launch_tile<T, 5>(a, n_tiles, s); break;
+    default: launch_tile<T, 6>(a, n_tiles, s); break;
+  }
+}
+
+void launch_tile_dispatch(rz_dtype dt, int fun, const TileArgs& a, uint32_t n_tiles,
+                          hipStream_t s) {
+  switch (dt) {
+    case RZ_U8: launch_tile_fun<uint8_t>(fun, a, n_tiles, s); break;
+    case RZ_U16: launch_tile_fun<uint16_t>(fun, a, n_tiles, s); break;
+    case RZ_U32: launch_tile_fun<uint32_t>(fun, a, n_tiles, s); break;
+    case RZ_U64: launch_tile_fun<uint64_t>(fun, a, n_tiles, s); break;
+    case RZ_I8: launch_tile_fun<int8_t>(fun, a, n_tiles, s); break;
+    case RZ_I16: launch_tile_fun<int16_t>(fun, a, n_tiles, s); break;
+    case RZ_I32: launch_tile_fun<int32_t>(fun, a, n_tiles, s); break;
+    case RZ_I64: launch_tile_fun<int64_t>(fun, a, n_tiles, s); break;
+    case RZ_F32: launch_tile_fun<float>(fun, a, n_tiles, s); break;
+    default: launch_tile_fun<double>(fun, a, n_tiles, s); break;
+  }
+}
+
+}  // namespace
+
+extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry* dg,
+                                         const rz_raster_info* info, rz_dtype dtype, rz_fun fun,
+                                         const void* field_dev, const void* scalar_host,
+                                         const uint32_t* layer_dev, uint32_t n_layers,
+                                         const void* bg_host, int all_touched, rz_sum_mode sum_mode,
+                                         void* out_dev) {
+  if (!c || !dg || !info || !bg_host || !out_dev)
+    return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_rasterize_device");
+  if ((unsigned)dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
+  if ((unsigned)fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
+  if (!field_dev && !scalar_host) return rz::fail(RZ_VALUE_ERROR, "no field and no scalar value");
+  if (all_touched)
+    return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not implemented on the device path yet");
+  (void)sum_mode;
+  if (!layer_dev) n_layers = 1;
+  if (n_layers == 0) return rz::fail(RZ_VALUE_ERROR, "n_layers must be >= 1");
+  if (info->nrows == 0 || info->ncols == 0) return RZ_OK;  // empty raster: nothing to write
+  if (info->nrows >= (uint64_t)INT32_MAX - TILE_H || info->ncols >= (uint64_t)INT32_MAX - 256)
+    return rz::fail(RZ_CAPACITY_ERROR, "raster dimensions exceed 2^31 rows or columns");
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t esz = rz_dtype_size(dtype);
+  const int TW = tile_w((int)esz);
+  const int32_t nrows = (int32_t)info->nrows, ncols = (int32_t)info->ncols;
+  const int32_t ntx = (ncols + TW - 1) / TW, nty = (nrows + TILE_H - 1) / TILE_H;
+  const uint64_t tpl = (uint64_t)ntx * (uint64_t)nty;
+  const uint64_t n_tiles64 = tpl * n_layers;
+  if (n_tiles64 >= (uint64_t)UINT32_MAX)
+    return rz::fail(RZ_CAPACITY_ERROR, "too many output tiles (> 2^32)");
+  const uint32_t n_tiles = (uint32_t)n_tiles64;
+  const int64_t I = dg->I, V = dg->V;
+  hipStream_t s = c->stream;
+
+  // workspace
+  HIP_TRY(ensure(c->misc, 64));
+  uint32_t* d_status = (uint32_t*)c->misc.p;
+  unsigned long long* d_burns = (unsigned long long*)((char*)c->misc.p + 8);
+  HIP_TRY(hipMemsetAsync(c->misc.p, 0, 16, s));
+  if (dg->has_poly) HIP_TRY(ensure(c->edges, (size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)));
+  if (dg->has_line || dg->has_point) HIP_TRY(ensure(c->px, (size_t)std::max<int64_t>(V, 1) * 16));
+  if (dg->has_line || dg->has_point) HIP_TRY(ensure(c->vflags, (size_t)std::max<int64_t>(V, 1)));
+  HIP_TRY(ensure(c->item_box, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox)));
+  HIP_TRY(ensure(c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
+
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], s));
+  // 1. edges / pixel coordinates / item boxes
+  if (I > 0) {
+    hipLaunchKernelGGL(k_init_items, dim3(blocks_for(I)), dim3(256), 0, s, (ItemBox*)c->item_box.p,
+                       (unsigned long long*)c->item_last.p, I);
+  }
+  if (V > 0) {
+    PrepArgs pa;
+    pa.xy = dg->xy;
+    pa.vert_ring = dg->vert_ring;
+    pa.ring_voff = dg->ring_voff;
+    pa.ring_item = dg->ring_item;
+    pa.item_kind = dg->item_kind;
+    pa.n_vertices = V;
+    pa.xmin = info->xmin;
+    pa.ymax = info->ymax;
+    pa.xres = info->xres;
+    pa.yres = info->yres;
+    pa.nrows = nrows;
+    pa.ncols = ncols;
+    pa.edges = (PolyEdge*)c->edges.p;
+    pa.px = (double2*)c->px.p;
+    pa.vflags = (uint8_t*)c->vflags.p;
+    pa.item_box = (ItemBox*)c->item_box.p;
+    pa.item_last = (unsigned long long*)c->item_last.p;
+    hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
+  }
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
+
+  // 2. binning: scan mode for few items, stable sort-middle bins otherwise
+  TileArgs ta;
+  std::memset(&ta, 0, sizeof(ta));
+  // RUSTERIZE_BINNING=always|never overrides the size heuristic (tests use it
+  // to run both item-list forms on the same inputs).
+  bool scan_mode = I <= 256 || (uint64_t)I * (uint64_t)n_tiles <= ((uint64_t)1 << 24);
+  if (const char* bm = std::getenv("RUSTERIZE_BINNING")) {
+    if (std::strcmp(bm, "always") == 0) scan_mode = I == 0;
+    else if (std::strcmp(bm, "never") == 0) scan_mode = true;
+  }
+  uint64_t n_pairs = 0;
+  if (!scan_mode) {
+    HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
+    BinArgs ba;
+    ba.item_box = (const ItemBox*)c->item_box.p;
+    ba.item_geom = dg->item_geom;
+    ba.layer_of_geom = layer_dev;
+    ba.n_items = I;
+    ba.ntx = ntx;
+    ba.nty = nty;
+    ba.tile_w = TW;
+    ba.tiles_per_layer = (uint32_t)tpl;
+    ba.counts = (uint32_t*)c->counts.p;
+    ba.offsets = (const uint64_t*)c->offsets.p;
+    HIP_TRY(hipMemsetAsync((uint32_t*)c->counts.p + I, 0, 4, s));
+    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(I)), dim3(256), 0, s, ba);
+    size_t tb = 0;
+    HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->counts.p,
+                                    (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
+                                    rocprim::plus<uint64_t>(), s));
+    HIP_TRY(ensure(c->temp, tb));
+    HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->counts.p,
+                                    (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
+                                    rocprim::plus<uint64_t>(), s));
+    HIP_TRY(hipMemcpyAsync(&c->host_misc[0], (uint64_t*)c->offsets.p + I, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    n_pairs = c->host_misc[0];
+    if (n_pairs >= (uint64_t)UINT32_MAX)
+      return rz::fail(RZ_CAPACITY_ERROR, "more than 2^32 tile/item pairs in one call");
+    HIP_TRY(ensure(c->keys_a, std::max<uint64_t>(n_pairs, 1) * 4));
+    HIP_TRY(ensure(c->vals_a, std::max<uint64_t>(n_pairs, 1) * 4));
+    HIP_TRY(ensure(c->keys_b, std::max<uint64_t>(n_pairs, 1) * 4));
+    HIP_TRY(ensure(c->vals_b, std::max<uint64_t>(n_pairs, 1) * 4));
+    HIP_TRY(ensure(c->tile_off, ((size_t)n_tiles + 1) * 4));
+    ba.keys = (uint32_t*)c->keys_a.p;
+    ba.vals = (uint32_t*)c->vals_a.p;
+    hipLaunchKernelGGL(k_bin_emit, dim3(blocks_for(I)), dim3(256), 0, s, ba);
+    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], s));
+    // stable LSD radix sort by tile key keeps geometry order inside each tile
+    unsigned end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)n_tiles) ++end_bit;
+    if (n_pairs > 0) {
+      tb = 0;
+      HIP_TRY(rocprim::radix_sort_pairs(nullptr, tb, (const uint32_t*)c->keys_a.p,
+                                        (uint32_t*)c->keys_b.p, (const uint32_t*)c->vals_a.p,
+                                        (uint32_t*)c->vals_b.p, (size_t)n_pairs, 0, end_bit, s));
+      HIP_TRY(ensure(c->temp, tb));
+      HIP_TRY(rocprim::radix_sort_pairs(c->temp.p, tb, (const uint32_t*)c->keys_a.p,
+                                        (uint32_t*)c->keys_b.p, (const uint32_t*)c->vals_a.p,
+                                        (uint32_t*)c->vals_b.p, (size_t)n_pairs, 0, end_bit, s));
+    }
+    hipLaunchKernelGGL(k_tile_offsets, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
+                       (const uint32_t*)c->keys_b.p, (uint32_t)n_pairs, n_tiles,
+                       (uint32_t*)c->tile_off.p);
+    ta.tile_off = (const uint32_t*)c->tile_off.p;
+    ta.pair_item = (const uint32_t*)c->vals_b.p;
+  } else if (c->profiling) {
+    HIP_TRY(hipEventRecord(c->ev[2], s));
+  }
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
+
+  // 3. tiles
+  ta.nrows = nrows;
+  ta.ncols = ncols;
+  ta.ntx = ntx;
+  ta.nty = nty;
+  ta.tiles_per_layer = (uint32_t)tpl;
+  ta.n_tiles = n_tiles;
+  ta.n_items = I;
+  ta.item_kind = dg->item_kind;
+  ta.item_box = (const ItemBox*)c->item_box.p;
+  ta.item_geom = dg->item_geom;
+  ta.item_vbeg = dg->item_vbeg;
+  ta.item_last = (const unsigned long long*)c->item_last.p;
+  ta.layer_of_geom = layer_dev;
+  ta.edges = (const PolyEdge*)c->edges.p;
+  ta.px = (const double2*)c->px.p;
+  ta.vflags = (const uint8_t*)c->vflags.p;
+  ta.field = field_dev;
+  ta.scalar_bits = 0;
+  ta.bg_bits = 0;
+  if (scalar_host) std::memcpy(&ta.scalar_bits, scalar_host, esz);
+  std::memcpy(&ta.bg_bits, bg_host, esz);
+  ta.dedup_lines = info->xres != info->yres;
+  ta.out = out_dev;
+  ta.burns = c->profiling ? d_burns : nullptr;
+  ta.status = d_status;
+  launch_tile_dispatch(dtype, (int)fun, ta, n_tiles, s);
+  HIP_TRY(hipGetLastError());
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
+  c->have_times = c->profiling;
+  c->burns_pending = c->profiling;
+  c->last_pairs = n_pairs;
+  c->last_tiles = n_tiles;
+  return RZ_OK;
+}
+
+// ------------------------------------------------- host-in / host-out ----
+
+namespace {
+std::mutex g_default_mu;
+rz_context* g_default_ctx = nullptr;
+
+rz_status default_context(rz_context** out) {
+  if (!g_default_ctx) {
+    int dev = 0;
+    if (const char* e = std::getenv("RUSTERIZE_DEVICE")) dev = std::atoi(e);
+    rz_status st = rz_context_create(dev, &g_default_ctx);
+    if (st) {
+      g_default_ctx = nullptr;
+      return st;
+    }
+  }
+  *out = g_default_ctx;
+  return RZ_OK;
+}
+}  // namespace
+
+extern "C" rz_status rz_rasterize(const rz_geometry* geoms, const rz_raster_info* info,
+                                  rz_dtype dtype, rz_fun fun, const void* field,
+                                  const void* scalar, const uint32_t* layer_of_geom,
+                                  uint32_t n_layers, const void* background, int all_touched,
+                                  rz_sum_mode sum_mode, void* out) {
+  if (!geoms || !info || !background || !out)
+    return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_rasterize");
+  if ((unsigned)dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
+  if (!layer_of_geom) n_layers = 1;
+  for (int64_t i = 0; layer_of_geom && i < geoms->n_geoms; ++i)
+    if (layer_of_geom[i] >= n_layers) return rz::fail(RZ_VALUE_ERROR, "layer index out of range");
+  std::lock_guard<std::mutex> lock(g_default_mu);
+  rz_context* c = nullptr;
+  rz_status st = default_context(&c);
+  if (st) return st;
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t esz = rz_dtype_size(dtype);
+  const size_t out_bytes = (size_t)n_layers * info->nrows * info->ncols * esz;
+  rz_device_geometry* dg = nullptr;
+  st = rz_device_geometry_upload(c, geoms, &dg);
+  if (st) return st;
+  void *d_field = nullptr, *d_layer = nullptr, *d_out = nullptr;
+  hipError_t e = hipSuccess;
+  if (field) e = upload(c, (uint8_t**)&d_field, (const uint8_t*)field, (int64_t)(geoms->n_geoms * esz));
+  if (e == hipSuccess && layer_of_geom) e = upload(c, (uint32_t**)&d_layer, layer_of_geom, geoms->n_geoms);
+  if (e == hipSuccess) e = hipMalloc(&d_out, std::max<size_t>(out_bytes, 1));
+  if (e == hipSuccess) {
+    st = rz_rasterize_device(c, dg, info, dtype, fun, d_field, field ? nullptr : scalar,
+                             (const uint32_t*)d_layer, n_layers, background, all_touched, sum_mode,
+                             d_out);
+    if (st == RZ_OK) {
+      e = hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess) st = check_status_word(c);
+    }
+  }
+  if (d_field) (void)hipFree(d_field);
+  if (d_layer) (void)hipFree(d_layer);
+  if (d_out) (void)hipFree(d_out);
+  rz_device_geometry_free(dg);
+  if (e != hipSuccess) return rz::fail(RZ_DEVICE_ERROR, std::string("HIP error: ") + hipGetErrorString(e));
+  return st;
+}
+
+// ------------------------------------------------------------ test hooks --
+
+// Host evaluation of the Bresenham closed form used by the tile kernel
+// (walk_window in rz_kernels.h), for brute-force checks on the CPU.
+extern "C" int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, int64_t iy1,
+                                       int64_t X0, int64_t X1, int64_t Y0, int64_t Y1,
+                                       int64_t* xs, int64_t* ys, int64_t cap) {
+  int64_t n = 0;
+  auto emit = [&](int64_t x, int64_t y) {
+    if (n < cap) {
+      xs[n] = x;
+      ys[n] = y;
+    }
+    ++n;
+  };
+  const int64_t lim = (int64_t)1 << 30;
+  if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
+      iy1 > -lim && iy1 < lim)
+    walk_window<int64_t>(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, emit);
+  else
+    walk_window<__int128>(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, emit);
+  return n;
+}
+
+// Host evaluation of the polygon row coverage used by the tile kernel: the
+// covered columns of [C0, C0+width) for crossings xs[0..n) of one row, as a
+// 0/1 byte per column.  width must be 64 or 128.
+extern "C" int rz_debug_row_coverage(const double* xs, int64_t n, int64_t ncols, int32_t C0,
+                                     int32_t width, uint8_t* out) {
+  if (width != 64 && width != 128) return -1;
+  uint64_t tog[2] = {0, 0};
+  uint32_t lp = 0;
+  int32_t amax = -1;
+  for (int64_t i = 0; i < n; ++i)
+    if (!is_pos_nan(xs[i])) toggle_crossing(span_col(xs[i], (double)ncols), C0, width, tog, lp, amax);
+  const int ppt = width / 4;
+  for (int q = 0; q < 4; ++q) {
+    uint64_t cov = width == 64 ? segment_coverage<16, 1>(tog, lp, amax, C0, q * ppt)
+                               : segment_coverage<32, 2>(tog, lp, amax, C0, q * ppt);
+    for (int j = 0; j < ppt; ++j) out[q * ppt + j] = (uint8_t)((cov >> j) & 1u);
+  }
+  return 0;
+}

@@ -1,0 +1,213 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the
+reference's golden rasters.  Bit-exact for every function, dtype and input;
+NaN payloads are compared as NaN (a generated NaN's payload is not defined
+by the reference's source)."""
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from rusterize_amd import _lib, rusterize
+from rusterize_amd._lib import Geometry
+from tests.cases import EXPLODED_VALUES, EXPLODED_WKT, FIELD_RANGE, GEOMS, VALUES, sq
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def assert_same(gpu: np.ndarray, ref: np.ndarray, what: str = ""):
+    assert gpu.shape == ref.shape, (gpu.shape, ref.shape)
+    assert gpu.dtype == ref.dtype
+    if gpu.dtype.kind == "f":
+        gn, rn = np.isnan(gpu), np.isnan(ref)
+        bad = (gn != rn) | (~gn & (gpu.view(f"u{gpu.itemsize}") != ref.view(f"u{ref.itemsize}")))
+    else:
+        bad = gpu != ref
+    n = int(bad.sum())
+    if n:
+        idx = np.argwhere(bad)[:5]
+        raise AssertionError(f"{what}: {n} pixels differ, e.g. {[(tuple(i), gpu[tuple(i)], ref[tuple(i)]) for i in idx]}")
+
+
+def field_for(dtype, n, seed):
+    lo, hi = FIELD_RANGE[dtype]
+    rng = np.random.default_rng(seed)
+    if dtype.startswith("float"):
+        v = rng.integers(lo, hi, n).astype(dtype) + rng.random(n).astype(dtype)
+    else:
+        v = rng.integers(lo, hi, n).astype(dtype)
+    return v
+
+
+def run_both(geom, info, dtype, fun, field=None, scalar=1, layers=None, n_layers=1, bg=0):
+    got = _lib.rasterize(geom, info, dtype, fun, field, scalar, layers, n_layers, bg)
+    ref = oracle.rasterize(geom, oracle.copy_info(info), dtype, fun, field=field, scalar=scalar,
+                           layer_of_geom=layers, n_layers=n_layers, background=bg)
+    return got, ref
+
+
+# ------------------------------------------------------------ golden rasters
+
+def test_golden_standard_sum():
+    # python/test/test_many.py:228-235 through the drop-in API
+    r = rusterize(GEOMS, res=(1, 1), dtype="uint8", burn=VALUES, fun="sum", encoding="numpy").squeeze()
+    assert_same(r, np.load(os.path.join(GOLDEN, "standard_output_sum.npy")), "standard_output_sum")
+
+
+def test_golden_custom_shape():
+    # python/test/test_many.py:329-346 (non-square pixels: per-line dedup)
+    r = rusterize(EXPLODED_WKT, out_shape=(47, 319), dtype="uint8", burn=EXPLODED_VALUES, fun="sum",
+                  encoding="numpy").squeeze()
+    assert_same(r, np.load(os.path.join(GOLDEN, "standard_output_sum_custom_shape.npy")), "custom_shape")
+
+
+# ---------------------------------------------------------------- KATs (R)
+
+def test_r_known_answers():
+    # R/rusterize/tests/testthat/test-geometry.R and test-boundary.R on a 4x4 grid
+    kw = dict(out_shape=(4, 4), extent=(0, 0, 4, 4), encoding="numpy", background=0)
+    m = rusterize(["LINESTRING (0 0, 4 4)"], burn=9, fun="last", **kw)[0]
+    want = np.zeros((4, 4))
+    for r, c in [(3, 1), (2, 2), (1, 3)]:
+        want[r, c] = 9
+    assert_same(m, want)
+    mp = "MULTIPOLYGON (((0 0, 2 0, 2 2, 0 2, 0 0)), ((2 2, 4 2, 4 4, 2 4, 2 2)))"
+    m = rusterize([mp], burn=9, fun="last", **kw)[0]
+    want = np.zeros((4, 4))
+    want[2:4, 0:2] = 9
+    want[0:2, 2:4] = 9
+    assert_same(m, want)
+    two = [sq(0, 0, 4, 4), sq(0, 0, 4, 4)]
+    for fun, val in [("sum", 12), ("min", 5), ("max", 7), ("first", 5), ("last", 7), ("any", 1)]:
+        m = rusterize(two, burn=np.array([5.0, 7.0]), fun=fun, **kw)[0]
+        assert_same(m, np.full((4, 4), float(val)), fun)
+    m = rusterize(two, burn=np.array([1.0, 1.0]), fun="count", **kw)[0]
+    assert_same(m, np.full((4, 4), 2.0))
+    m = rusterize(["POINT (1.5 1.5)"], burn=9, fun="last", **kw)[0]
+    assert m.sum() == 9 and (m > 0).sum() == 1
+    assert rusterize([sq(0, 0, 4, 4)], out_shape=(4, 4), encoding="numpy").shape == (1, 4, 4)
+    assert rusterize([sq(0, 0, 4, 4)], res=(1, 1), encoding="numpy").shape == (1, 5, 5)
+
+
+# ------------------------------------------------- all functions x dtypes
+
+MIXED_CACHE = {}
+
+
+def mixed_geometry():
+    if "g" not in MIXED_CACHE:
+        stars = Geometry.synth(1, 11, 60, 300.0, 200.0, 3.0, 40.0)
+        convex = Geometry.synth(2, 12, 80, 300.0, 200.0, 2.0, 60.0, 0.3, 0.4)
+        lines = Geometry.synth(3, 13, 0, 300.0, 200.0, 40, 60, 20)
+        wkts = [sq(-10, -10, 30, 20), "LINESTRING (-50 -50, 400 300)", "POINT (299.5 199.5)",
+                "MULTIPOINT ((10.5 10.5), (10.5 10.5), (11.5 12.5))",
+                "LINESTRING (10 10, 20 10, 20 20, 10 10)", "POLYGON EMPTY",
+                "POLYGON ((100 50, 100 50.0000000000001, 150 50, 100 50))"]
+        extra = Geometry.from_wkt(wkts)
+        MIXED_CACHE["g"] = concat([stars, convex, lines, extra])
+    return MIXED_CACHE["g"]
+
+
+def concat(parts):
+    gio, kinds, iro, rvo, xy, fl = [0], [], [0], [0], [], []
+    for p in parts:
+        base_i, base_r, base_v = len(kinds), len(rvo) - 1, rvo[-1]
+        gio.extend((p.geom_item_offsets[1:] + base_i).tolist())
+        kinds.extend(p.item_kind.tolist())
+        iro.extend((p.item_ring_offsets[1:] + base_r).tolist())
+        rvo.extend((p.ring_vertex_offsets[1:] + base_v).tolist())
+        xy.append(p.xy)
+        fl.extend(p.ring_flags.tolist())
+    return Geometry(gio, kinds, iro, rvo, np.concatenate(xy), fl)
+
+
+@pytest.mark.parametrize("dtype", _lib.DTYPES)
+@pytest.mark.parametrize("fun", _lib.FUNS)
+def test_mixed_all_functions(dtype, fun):
+    g = mixed_geometry()
+    info = _lib.raster_info(None, shape=(200, 300), extent=(0, 0, 300, 200))
+    field = field_for(dtype, g.n_geoms, seed=zlib.crc32(f"{dtype}{fun}".encode()))
+    # make some values equal to the background to exercise the `== bg` quirk
+    field[::17] = 0
+    for bg in ([0, 7] if not dtype.startswith("float") else [0, float("nan")]):
+        got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+        assert_same(got, ref, f"{dtype} {fun} bg={bg}")
+
+
+@pytest.mark.parametrize("binning", ["always", "never"])
+def test_binning_modes_agree(binning, monkeypatch):
+    monkeypatch.setenv("RUSTERIZE_BINNING", binning)
+    g = mixed_geometry()
+    info = _lib.raster_info(None, shape=(200, 300), extent=(0, 0, 300, 200))
+    for dtype, fun in [("float64", "sum"), ("uint8", "count"), ("int32", "first"), ("float32", "min")]:
+        field = field_for(dtype, g.n_geoms, seed=5)
+        got, ref = run_both(g, info, dtype, fun, field=field, bg=0)
+        assert_same(got, ref, f"{binning} {dtype} {fun}")
+
+
+def test_ragged_grids_and_offsets():
+    g = mixed_geometry()
+    for shape, ext in [((1, 1), (0, 0, 300, 200)), ((333, 517), (-20.5, -13.25, 310.0, 222.0)),
+                       ((65, 129), (0, 0, 300, 200)), ((7, 3000), (0, 0, 300, 200))]:
+        info = _lib.raster_info(None, shape=shape, extent=ext)
+        for dtype, fun in [("float64", "sum"), ("uint8", "last"), ("int16", "max")]:
+            field = field_for(dtype, g.n_geoms, seed=3)
+            got, ref = run_both(g, info, dtype, fun, field=field, bg=0)
+            assert_same(got, ref, f"{shape} {dtype} {fun}")
+
+
+def test_non_square_pixels_dedup():
+    g = mixed_geometry()
+    info = _lib.raster_info(None, shape=(70, 300), extent=(0, 0, 300, 200))  # yres != xres
+    for fun in ["sum", "count", "last"]:
+        field = field_for("float64", g.n_geoms, seed=9)
+        got, ref = run_both(g, info, "float64", fun, field=field, bg=float("nan"))
+        assert_same(got, ref, fun)
+
+
+def test_multiband_by():
+    g = Geometry.synth(2, 21, 3000, 1000.0, 700.0, 2.0, 120.0, 0.1, 0.2)
+    info = _lib.raster_info(None, shape=(700, 1000), extent=(0, 0, 1000, 700))
+    layers = np.random.default_rng(1).integers(0, 5, g.n_geoms).astype(np.uint32)
+    for dtype, fun in [("uint8", "first"), ("int32", "last"), ("float32", "max"), ("float64", "sum")]:
+        field = field_for(dtype, g.n_geoms, seed=2)
+        got, ref = run_both(g, info, dtype, fun, field=field, layers=layers, n_layers=5, bg=0)
+        assert_same(got, ref, f"by {dtype} {fun}")
+
+
+# ------------------------------------------------------ workload configs
+
+def test_config1_star_polygons_f64_sum():
+    g = Geometry.synth(1, 1, 1000, 2048.0, 2048.0, 5.0, 50.0)
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 2048, 2048))
+    field = np.empty(g.n_geoms)
+    _lib.lib().rz_synth_field(9, 2, 101, g.n_geoms, 0.0, 1.0, field.ctypes.data)
+    got, ref = run_both(g, info, "float64", "sum", field=field, bg=float("nan"))
+    assert_same(got, ref, "config 1")
+
+
+@pytest.mark.parametrize("dtype", ["uint8", "int32", "float32"])
+def test_config2_reduced(dtype):
+    # the convex-polygon workload at 1/100 of the polygons on a 4000^2 grid
+    g = Geometry.synth(2, 2, 10000, 4000.0, 4000.0, 2.0, 200.0, 0.1, 0.2)
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 4000, 4000))
+    f32 = np.empty(g.n_geoms, dtype=np.int32)
+    _lib.lib().rz_synth_field(6, 0, 102, g.n_geoms, 0.0, 1e6, f32.ctypes.data)
+    field = f32.astype(dtype)  # numpy cast: wraps for uint8
+    layers = np.random.default_rng(2).integers(0, 8, g.n_geoms).astype(np.uint32)
+    for fun in ["count", "any", "first", "last", "min", "max"]:
+        got, ref = run_both(g, info, dtype, fun, field=field, layers=layers, n_layers=8, bg=0)
+        assert_same(got, ref, f"config2 {dtype} {fun}")
+
+
+def test_config3_reduced():
+    g = Geometry.synth(3, 3, 0, 2000.0, 2000.0, 5000, 20000, 500)
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 2000, 2000))
+    field = np.empty(g.n_geoms, dtype=np.float32)
+    _lib.lib().rz_synth_field(8, 1, 103, g.n_geoms, 0.0, 1.0, field.ctypes.data)
+    for fun in ["max", "sum"]:
+        got, ref = run_both(g, info, "float32", fun, field=field, bg=float("nan"))
+        assert_same(got, ref, f"config3 {fun}")

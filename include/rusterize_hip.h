@@ -222,17 +222,20 @@ rz_status rz_rasterize_device(rz_context* ctx, const rz_device_geometry* dg,
                               const void* background_host, int all_touched,
                               rz_sum_mode sum_mode, void* out_dev);
 
-/* Stage timings of the last rz_rasterize_device call on ctx, measured with
- * HIP events on the context's stream (requires rz_context_set_profiling(ctx,1)).
- * Writes up to n values (milliseconds) in the order given by
+/* Profiling level: 0 off; 1 records HIP events around every stage of every
+ * rz_rasterize_device call on the context's stream (no added
+ * synchronisation); 2 also counts exact pixel-burns.
+ * rz_context_stage_times waits for the recorded calls and writes up to n
+ * per-stage sums (milliseconds since the last reset) in the order given by
  * rz_stage_name(i); returns the number of stages. */
-rz_status rz_context_set_profiling(rz_context* ctx, int enabled);
+rz_status rz_context_set_profiling(rz_context* ctx, int level);
 int rz_context_stage_times(rz_context* ctx, double* ms, int n);
+void rz_context_reset_stage_times(rz_context* ctx);
 const char* rz_stage_name(int i);
 
 /* Counters of the last call: pixel-burns (writes reaching the dense writer,
- * rust/src/encoding/writers.rs:68-71), tile/item pairs and tiles. Requires
- * rz_context_set_profiling(ctx, 1) for burns. */
+ * rust/src/encoding/writers.rs:68-71), tile/item pairs and tiles.  Burns are
+ * counted at profiling level 2 only. */
 rz_status rz_context_last_counters(rz_context* ctx, uint64_t* burns, uint64_t* pairs,
                                    uint64_t* tiles);
 

@@ -654,7 +654,16 @@ int oracle_rasterize(const rz_geometry* g, const rz_raster_info* ri, int dtype, 
   if (row0 > row1) row0 = row1;
   size_t esz = dtype_sizes[dtype];
   uint64_t band_elems = (row1 - row0) * ri->ncols;
-  for (uint64_t i = 0; i < band_elems * n_layers; ++i) memcpy((char*)out + i * esz, bg, esz);
+  /* Array3::from_elem: fill by doubling copies of the first element */
+  uint64_t total_bytes = band_elems * n_layers * esz;
+  if (total_bytes) {
+    memcpy(out, bg, esz);
+    for (uint64_t done = esz; done < total_bytes;) {
+      uint64_t n = done < total_bytes - done ? done : total_bytes - done;
+      memcpy((char*)out + done, out, n);
+      done += n;
+    }
+  }
 
   burn_ctx c = {g, ri, all_touched, all_touched && (fun == RZ_SUM || fun == RZ_COUNT)};
   /* group_keys, rasterize.rs:198-205 (the caller supplies sorted band ids) */

@@ -84,6 +84,7 @@ _SIGNATURES = [
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_context_set_profiling", c_int, [c_void_p, c_int]),
     ("rz_context_stage_times", c_int, [c_void_p, POINTER(c_double), c_int]),
+    ("rz_context_reset_stage_times", None, [c_void_p]),
     ("rz_stage_name", c_char_p, [c_int]),
     ("rz_context_last_counters", c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     ("rz_debug_walk_cells", c_int64, [c_int64] * 8 + [POINTER(c_int64), POINTER(c_int64), c_int64]),

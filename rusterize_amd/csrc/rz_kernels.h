@@ -63,21 +63,47 @@ struct PrepArgs {
   unsigned long long* item_last;  // [n_items] last kept line segment + 1 (0: none)
 };
 
-struct BinArgs {
+// Per-call item descriptor (64 B, one scalar load): everything the tile
+// kernel needs about an item, with the owning geometry's field value and
+// output band already resolved.
+struct __align__(64) ItemDesc {
+  int32_t r0, r1, c0, c1;   // pixel box: rows [r0, r1), cols [c0, c1)
+  int64_t vbeg;             // first vertex (edge / segment / point index)
+  int32_t nv;               // vertex count
+  uint32_t kind_layer;      // item kind | output band << 2
+  uint64_t value;           // field value bits (dtype of the call)
+  uint64_t last;            // lines: last kept segment + 1 (0: none)
+  uint64_t pad[2];
+};
+
+struct DescArgs {
   const ItemBox* item_box;
+  const int32_t* item_kind;
   const int32_t* item_geom;
+  const int64_t* item_vbeg;
+  const unsigned long long* item_last;
   const uint32_t* layer_of_geom;  // may be null
+  const void* field;              // [n_geoms] of the call's dtype, or null
+  uint64_t scalar_bits;
+  int32_t esz;
+  int64_t n_items;
+  ItemDesc* desc;
+  // binning (counts == nullptr in scan mode)
+  int32_t ntx, nty, tile_w;
+  uint32_t* counts;  // [n_items]
+};
+
+struct BinArgs {
+  const ItemDesc* desc;
   int64_t n_items;
   int32_t ntx, nty, tile_w;
   uint32_t tiles_per_layer;
-  uint32_t* counts;        // [n_items]
-  const uint64_t* offsets; // [n_items + 1] exclusive prefix of counts
-  uint32_t* keys;          // [n_pairs]
-  uint32_t* vals;          // [n_pairs]
+  const uint64_t* offsets;  // [n_items + 1] exclusive prefix of the tile counts
+  uint32_t* keys;           // [n_pairs]
+  uint32_t* vals;           // [n_pairs]
 };
 
 struct TileArgs {
-  // grid
   int32_t nrows, ncols, ntx, nty;
   uint32_t tiles_per_layer, n_tiles;
   // binned mode: pairs sorted by tile, tile_off[t]..tile_off[t+1]
@@ -85,25 +111,18 @@ struct TileArgs {
   const uint32_t* pair_item;
   // scan mode (pair_item == nullptr): every item is tested against the tile
   int64_t n_items;
-  // items
-  const int32_t* item_kind;
-  const ItemBox* item_box;
-  const int32_t* item_geom;
-  const int64_t* item_vbeg;  // [n_items + 1] vertex range per item (items are contiguous)
-  const unsigned long long* item_last;
-  const uint32_t* layer_of_geom;  // may be null
+  const ItemDesc* desc;
   const PolyEdge* edges;
   const double2* px;
   const uint8_t* vflags;
-  // values
-  const void* field;       // [n_geoms] of T or null
-  uint64_t scalar_bits, bg_bits;
-  int dedup_lines;         // xres != yres (burn_geometry.rs:179, 202)
-  // output
-  void* out;               // [layers][nrows][ncols] of T
-  unsigned long long* burns;  // optional counter
-  uint32_t* status;        // error flags
+  uint64_t bg_bits;
+  int dedup_lines;          // xres != yres (burn_geometry.rs:179, 202)
+  void* out;                // [layers][nrows][ncols] of T
+  unsigned long long* burns;  // [kBurnSlots] counters, or null
+  uint32_t* status;         // error flags
 };
+
+constexpr int kBurnSlots = 1024;
 
 // ------------------------------------------------ polygon row coverage --
 //
@@ -138,30 +157,32 @@ __host__ __device__ inline void toggle_crossing(int32_t col, int32_t C0, int32_t
   else if (rel < TW) tog[rel >> 6] ^= 1ull << (rel & 63);
 }
 
-// Coverage bits of the PPT pixels [C0+seg_lo, C0+seg_lo+PPT) from the
-// toggles of the whole window (NWORD words).
-template <int PPT, int NWORD>
-__host__ __device__ inline uint64_t segment_coverage(const uint64_t* tog, uint32_t lp,
-                                                     int32_t amax, int32_t C0, int32_t seg_lo) {
+// Inclusive prefix-XOR of the bits of one 64-bit word.
+__host__ __device__ inline uint64_t prefix_xor64(uint64_t x) {
+  x ^= x << 1;
+  x ^= x << 2;
+  x ^= x << 4;
+  x ^= x << 8;
+  x ^= x << 16;
+  x ^= x << 32;
+  return x;
+}
+
+// Coverage bits of columns [C0 + 64*w, C0 + 64*w + 64) of the window from
+// the toggles of the whole window (NWORD words): bit = odd prefix parity and
+// column < amax.
+template <int NWORD>
+__host__ __device__ inline uint64_t row_coverage_word(const uint64_t* tog, uint32_t lp,
+                                                      int32_t amax, int32_t C0, int w) {
   uint32_t carry = lp;
-  for (int w = 0; w < NWORD; ++w) {
-    const int lo = w * 64;
-    if (lo + 64 <= seg_lo) carry ^= (uint32_t)__builtin_popcountll(tog[w]) & 1u;
-    else if (lo < seg_lo)
-      carry ^= (uint32_t)__builtin_popcountll(tog[w] & ((1ull << (seg_lo - lo)) - 1ull)) & 1u;
+  uint64_t word = 0;
+  for (int k = 0; k < NWORD; ++k) {
+    const uint64_t x = prefix_xor64(tog[k]) ^ (carry ? ~0ull : 0ull);
+    if (k == w) word = x;
+    carry ^= (uint32_t)__builtin_popcountll(tog[k]) & 1u;
   }
-  constexpr uint64_t FULL = PPT >= 64 ? ~0ull : ((1ull << PPT) - 1ull);
-  uint64_t seg = (tog[seg_lo >> 6] >> (seg_lo & 63)) & FULL;
-  seg ^= seg << 1;
-  seg ^= seg << 2;
-  seg ^= seg << 4;
-  seg ^= seg << 8;
-  seg ^= seg << 16;
-  seg ^= seg << 32;
-  if (carry) seg = ~seg;
-  const int32_t lim = amax - (C0 + seg_lo);
-  const uint64_t lm = lim <= 0 ? 0ull : (lim >= PPT ? FULL : ((1ull << lim) - 1ull));
-  return seg & lm;
+  const int32_t lim = amax - (C0 + w * 64);
+  return word & (lim <= 0 ? 0ull : (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull)));
 }
 
 // Cells of Standard::burn_line for one segment between integer cells

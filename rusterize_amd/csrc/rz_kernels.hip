@@ -184,36 +184,66 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   reduce_box(item, box, a.item_box, active);
 }
 
-// --------------------------------------------------------------- binning --
+// ------------------------------------------------------- item descriptors --
 
-__device__ inline bool tile_rect(const ItemBox& b, int32_t tw, int32_t& tx0, int32_t& tx1,
-                                 int32_t& ty0, int32_t& ty1) {
-  if (!(b.r0 < b.r1 && b.c0 < b.c1)) return false;
-  tx0 = b.c0 / tw;
-  tx1 = (b.c1 - 1) / tw;
-  ty0 = b.r0 / TILE_H;
-  ty1 = (b.r1 - 1) / TILE_H;
+__device__ inline bool tile_rect(int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t tw,
+                                 int32_t& tx0, int32_t& tx1, int32_t& ty0, int32_t& ty1) {
+  if (!(r0 < r1 && c0 < c1)) return false;
+  tx0 = c0 / tw;
+  tx1 = (c1 - 1) / tw;
+  ty0 = r0 / TILE_H;
+  ty1 = (r1 - 1) / TILE_H;
   return true;
 }
 
-__global__ void k_bin_count(BinArgs a) {
+// One thread per item: resolve the field value and band of the owning
+// geometry (rust/src/rasterize.rs:169-179) into the item's descriptor and,
+// for the binned path, count the tiles its box covers.
+__global__ void k_item_desc(DescArgs a) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_items) return;
-  int32_t tx0, tx1, ty0, ty1;
-  uint32_t c = 0;
-  if (tile_rect(a.item_box[i], a.tile_w, tx0, tx1, ty0, ty1))
-    c = (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1);
-  a.counts[i] = c;
+  const ItemBox b = a.item_box[i];
+  const int32_t g = a.item_geom[i];
+  ItemDesc d;
+  d.r0 = b.r0;
+  d.r1 = b.r1;
+  d.c0 = b.c0;
+  d.c1 = b.c1;
+  d.vbeg = a.item_vbeg[i];
+  d.nv = (int32_t)(a.item_vbeg[i + 1] - d.vbeg);
+  const uint32_t layer = a.layer_of_geom ? a.layer_of_geom[g] : 0u;
+  d.kind_layer = (uint32_t)a.item_kind[i] | (layer << 2);
+  uint64_t v = a.scalar_bits;
+  if (a.field) {
+    const uint8_t* f = (const uint8_t*)a.field + (size_t)g * (size_t)a.esz;
+    v = 0;
+    for (int k = 0; k < a.esz; ++k) v |= (uint64_t)f[k] << (8 * k);
+  }
+  d.value = v;
+  d.last = a.item_last[i];
+  d.pad[0] = d.pad[1] = 0;
+  a.desc[i] = d;
+  if (a.counts) {
+    int32_t tx0, tx1, ty0, ty1;
+    uint32_t c = 0;
+    if (tile_rect(d.r0, d.r1, d.c0, d.c1, a.tile_w, tx0, tx1, ty0, ty1))
+      c = (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1);
+    a.counts[i] = c;
+  }
 }
 
+// --------------------------------------------------------------- binning --
+
+// Pairs (tile key, item) in item order; a stable sort by key then keeps
+// geometry order inside every tile.
 __global__ void k_bin_emit(BinArgs a) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_items) return;
+  const ItemDesc d = a.desc[i];
   int32_t tx0, tx1, ty0, ty1;
-  if (!tile_rect(a.item_box[i], a.tile_w, tx0, tx1, ty0, ty1)) return;
-  uint32_t layer = a.layer_of_geom ? a.layer_of_geom[a.item_geom[i]] : 0u;
+  if (!tile_rect(d.r0, d.r1, d.c0, d.c1, a.tile_w, tx0, tx1, ty0, ty1)) return;
   uint64_t o = a.offsets[i];
-  uint32_t base = layer * a.tiles_per_layer;
+  const uint32_t base = (d.kind_layer >> 2) * a.tiles_per_layer;
   for (int32_t ty = ty0; ty <= ty1; ++ty)
     for (int32_t tx = tx0; tx <= tx1; ++tx) {
       a.keys[o] = base + (uint32_t)ty * (uint32_t)a.ntx + (uint32_t)tx;
@@ -285,219 +315,4 @@ template <typename T> __device__ inline T from_bits(uint64_t b) {
   __builtin_memcpy(&v, &b, sizeof(T));
   return v;
 }
-
-// ------------------------------------------------------------- tile kernel --
-
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
-  // Workgroup b runs on XCD b % 8; give each XCD a contiguous run of tiles so
-  // neighbouring tiles (shared cache lines, shared items) meet in one L2.
-  uint32_t q = n / NUM_XCD, r = n % NUM_XCD;
-  uint32_t x = b % NUM_XCD, i = b / NUM_XCD;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-template <typename T, int FUN>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a) {
-  constexpr int TW = tile_w(sizeof(T));
-  constexpr int PPT = TW / 4;      // pixels per lane (one quarter of a tile row)
-  constexpr int NWORD = TW / 64;   // u64 words of a row mask
-  constexpr int CNT_WORDS = WAVE_ROWS * TW / 2;  // u16 line counters, 2 per word
-  __shared__ uint32_t s_cnt[4][CNT_WORDS];
-
-  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t layer = tile / a.tiles_per_layer;
-  const uint32_t tl = tile - layer * a.tiles_per_layer;
-  const int32_t ty = (int32_t)(tl / (uint32_t)a.ntx), tx = (int32_t)(tl % (uint32_t)a.ntx);
-  const int32_t R0 = ty * TILE_H, C0 = tx * TW;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int rl = lane >> 2, q = lane & 3;          // row in the wave strip, quarter
-  const int32_t Rw = R0 + wave * WAVE_ROWS;         // first row of this wave's strip
-  const int32_t r = Rw + rl;                        // this lane's row
-  const int32_t seg_lo = q * PPT;                   // first local column of this lane
-  const double ncols_f = (double)a.ncols;
-
-  const T bg = from_bits<T>(a.bg_bits);
-  const T scalar = from_bits<T>(a.scalar_bits);
-  const T* field = (const T*)a.field;
-
-  T acc[PPT];
-#pragma unroll
-  for (int j = 0; j < PPT; ++j) acc[j] = bg;
-  unsigned long long burns = 0;
-
-  uint32_t* cnt = s_cnt[wave];
-  // counters start zeroed; every apply pass zeroes what it read
-  for (int w = lane; w < CNT_WORDS; w += 64) cnt[w] = 0u;
-  wave_sync();
-
-  const int32_t strip_r1 = min(Rw + WAVE_ROWS, a.nrows);
-  const int32_t tile_c1 = min(C0 + TW, a.ncols);
-
-  auto process_item = [&](uint32_t item) {
-    const ItemBox bx = a.item_box[item];
-    // wave-uniform skip: item rows do not meet this wave's strip
-    if (bx.r1 <= Rw || bx.r0 >= strip_r1 || bx.c1 <= C0 || bx.c0 >= tile_c1) return;
-    const int32_t kind = a.item_kind[item];
-    const int32_t g = a.item_geom[item];
-    const T v = field ? field[g] : scalar;
-    const int64_t vb = a.item_vbeg[item], ve = a.item_vbeg[item + 1];
-    if (kind == 0) {
-      // ---- polygon: even-odd coverage of this row from crossing counts
-      // (segment_coverage in rz_kernels.h states why this equals the spans).
-      uint64_t tog[NWORD];
-#pragma unroll
-      for (int w = 0; w < NWORD; ++w) tog[w] = 0ull;
-      uint32_t lp = 0;
-      int32_t amax = -1;
-      if (r < strip_r1 && r >= bx.r0 && r < bx.r1) {
-        for (int64_t e = vb + q; e < ve; e += 4) {
-          const PolyEdge ed = a.edges[e];
-          if (ed.ystart <= r && r < ed.yend) {
-            const double x = crossing(ed, r);
-            if (!is_pos_nan(x)) toggle_crossing(span_col(x, ncols_f), C0, TW, tog, lp, amax);
-          }
-        }
-      }
-      // combine the four lanes of this row (lanes 4*rl .. 4*rl+3)
-#pragma unroll
-      for (int msk = 1; msk <= 2; msk <<= 1) {
-        lp ^= (uint32_t)__shfl_xor((int)lp, msk, 64);
-        amax = max(amax, __shfl_xor(amax, msk, 64));
-#pragma unroll
-        for (int w = 0; w < NWORD; ++w)
-          tog[w] ^= (uint64_t)__shfl_xor((unsigned long long)tog[w], msk, 64);
-      }
-      const uint64_t cov = segment_coverage<PPT, NWORD>(tog, lp, amax, C0, seg_lo);
-      if (cov) {
-        burns += (unsigned long long)__popcll(cov);
-#pragma unroll
-        for (int j = 0; j < PPT; ++j)
-          if ((cov >> j) & 1ull) fold<T, FUN>(acc[j], v, bg, 1u);
-      }
-      return;
-    }
-    // ---- lines and points: multiplicities in wave-private u16 counters
-    auto emit = [&](int64_t cx, int64_t cy) {
-      if (cx < C0 || cx >= tile_c1 || cy < Rw || cy >= strip_r1) return;
-      const uint32_t idx = (uint32_t)(cy - Rw) * TW + (uint32_t)(cx - C0);
-      atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
-    };
-    const bool dedup = kind == 1 && a.dedup_lines;
-    const unsigned long long last = kind == 1 ? a.item_last[item] : 0ull;
-    uint32_t seen = 0u;  // PixelCache bits of this lane's pixels (PPT <= 32)
-    for (int64_t cs = vb; cs < ve; cs += 65535) {  // <= 65535 writes per pixel per chunk
-      const int64_t ce = min(ve, cs + 65535);
-      for (int64_t s = cs + lane; s < ce; s += 64) {
-        const double2 p0 = a.px[s];
-        if (kind == 2) {  // point: extract_point + burn_point
-          if (p0.x >= 0.0 && p0.x < ncols_f && p0.y >= 0.0 && p0.y < (double)a.nrows)
-            emit((int64_t)p0.x, (int64_t)p0.y);
-          continue;
-        }
-        const uint8_t f = a.vflags[s];
-        if (!(f & VF_HAS_NEXT)) continue;
-        const double2 p1 = a.px[s + 1];
-        if (!(fmin(p0.x, p1.x) < ncols_f && fmax(p0.x, p1.x) >= 0.0 &&
-              fmin(p0.y, p1.y) < (double)a.nrows && fmax(p0.y, p1.y) >= 0.0))
-          continue;  // extract_line bbox filter, edges.rs:130
-        const int64_t ix0 = sat_isize(floor(p0.x)), iy0 = sat_isize(floor(p0.y));
-        const int64_t ix1 = sat_isize(floor(p1.x)), iy1 = sat_isize(floor(p1.y));
-        const int64_t lim = (int64_t)1 << 30;
-        if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
-            iy1 > -lim && iy1 < lim) {
-          walk_window<int64_t>(ix0, iy0, ix1, iy1, C0, tile_c1, Rw, strip_r1, emit);
-        } else if (ix0 > -(lim << 30) && ix0 < (lim << 30) && ix1 > -(lim << 30) &&
-                   ix1 < (lim << 30) && iy0 > -(lim << 30) && iy0 < (lim << 30) &&
-                   iy1 > -(lim << 30) && iy1 < (lim << 30)) {
-          walk_window<__int128>(ix0, iy0, ix1, iy1, C0, tile_c1, Rw, strip_r1, emit);
-        } else {
-          atomicOr(a.status, 1u);  // pixel coordinates beyond +-2^60: unsupported
-        }
-        // end cell of the last kept edge of an open line (burners.rs:86-89)
-        if ((unsigned long long)(s + 1) == last && !(f & VF_CLOSED)) emit(ix1, iy1);
-      }
-      wave_sync();
-      // apply the multiplicities of this lane's pixels, then clear them
-      uint32_t* crow = cnt + (rl * TW + seg_lo) / 2;
-#pragma unroll
-      for (int j2 = 0; j2 < PPT / 2; ++j2) {
-        const uint32_t wv = crow[j2];
-        if (wv) {
-          crow[j2] = 0u;
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            uint32_t k = (wv >> (16 * h)) & 0xFFFFu;
-            const int j = 2 * j2 + h;
-            if (k && dedup) {  // LineWriter + PixelCache: one write per pixel per item
-              const uint32_t bit = 1u << j;
-              if (seen & bit) k = 0;
-              else { seen |= bit; k = 1; }
-            }
-            if (k) {
-              burns += k;
-              fold<T, FUN>(acc[j], v, bg, k);
-            }
-          }
-        }
-      }
-      wave_sync();
-    }
-  };
-
-  if (a.pair_item) {
-    const uint32_t b = a.tile_off[tile], e = a.tile_off[tile + 1];
-    for (uint32_t i = b; i < e; ++i) process_item(a.pair_item[i]);
-  } else {
-    // scan mode: test every item's box in order (small item counts)
-    const int32_t tile_r1 = min(R0 + TILE_H, a.nrows);
-    for (int64_t base = 0; base < a.n_items; base += 64) {
-      const int64_t it = base + lane;
-      bool hit = false;
-      if (it < a.n_items) {
-        const ItemBox bx = a.item_box[it];
-        const uint32_t il = a.layer_of_geom ? a.layer_of_geom[a.item_geom[it]] : 0u;
-        hit = il == layer && bx.r0 < tile_r1 && bx.r1 > R0 && bx.c0 < tile_c1 && bx.c1 > C0 &&
-              bx.r0 < bx.r1 && bx.c0 < bx.c1;
-      }
-      uint64_t m = __ballot(hit);
-      while (m) {
-        const int bit = __builtin_ctzll(m);
-        m &= m - 1;
-        process_item((uint32_t)(base + bit));
-      }
-    }
-  }
-
-  // ---- write the tile once (background + burns), 16-byte stores when aligned
-  if (r < a.nrows) {
-    T* orow = (T*)a.out + ((uint64_t)layer * (uint64_t)a.nrows + (uint64_t)r) * (uint64_t)a.ncols;
-    const int32_t c = C0 + seg_lo;
-    constexpr int BYTES = PPT * (int)sizeof(T);
-    if (c + PPT <= a.ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
-      u32x4 vec[BYTES / 16];
-      __builtin_memcpy(vec, acc, BYTES);
-      u32x4* dst = (u32x4*)(orow + c);
-#pragma unroll
-      for (int k = 0; k < BYTES / 16; ++k) __builtin_nontemporal_store(vec[k], dst + k);
-    } else {
-#pragma unroll
-      for (int j = 0; j < PPT; ++j)
-        if (c + j < a.ncols) orow[c + j] = acc[j];
-    }
-  }
-  if (a.burns) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) burns += __shfl_xor(burns, off, 64);
-    if (lane == 0 && burns) atomicAdd(a.burns, burns);
-  }
-}
-
 }  // namespace rzk

@@ -18,6 +18,7 @@
 // The kernels are compiled in this translation unit so every k_tile<T, FUN>
 // instantiation used by the dispatch below is emitted here.
 #include "rz_kernels.hip"
+#include "rz_tile.hip"
 
 using namespace rzk;
 
@@ -59,21 +60,56 @@ inline unsigned blocks_for(int64_t n, int bs = 256) {
 
 }  // namespace
 
+struct EventSet {
+  hipEvent_t ev[kStages + 1] = {};
+};
+
 struct rz_context {
   int device = 0;
   hipStream_t stream = nullptr;
   // workspace
-  Buf edges, px, vflags, item_box, item_last, counts, offsets, keys_a, vals_a, keys_b, vals_b,
-      tile_off, temp, misc;
-  uint64_t* host_misc = nullptr;  // pinned: [0] total pairs, [1] status, [2] burns
-  bool profiling = false;
-  hipEvent_t ev[kStages + 1] = {};
-  bool have_times = false;
+  Buf edges, px, vflags, item_box, item_last, desc, counts, offsets, keys_a, vals_a, keys_b,
+      vals_b, tile_off, temp, misc;
+  uint64_t* host_misc = nullptr;  // pinned: [0] total pairs, [1] status
+  // profiling: 0 off, 1 stage events, 2 stage events + exact burn counts
+  int profiling = 0;
+  hipEvent_t ev[kStages + 1] = {};  // the current call's events (taken from the pool)
+  std::vector<EventSet> ev_pool, ev_pending;
+  EventSet cur;
+  double acc_ms[kStages] = {};
   uint64_t last_pairs = 0, last_tiles = 0;
   bool burns_pending = false;
   std::vector<Buf*> all() {
-    return {&edges, &px,     &vflags,   &item_box, &item_last, &counts, &offsets, &keys_a,
-            &vals_a, &keys_b, &vals_b, &tile_off, &temp,     &misc};
+    return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts, &offsets,
+            &keys_a, &vals_a, &keys_b, &vals_b,   &tile_off,  &temp, &misc};
+  }
+  // Takes an event set for the next call (events are recorded on the stream
+  // and resolved later, so timing adds no synchronisation to the call).
+  hipError_t take_events() {
+    if (ev_pool.empty()) {
+      EventSet s;
+      for (auto& e : s.ev) {
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+      }
+      ev_pool.push_back(s);
+    }
+    cur = ev_pool.back();
+    ev_pool.pop_back();
+    for (int i = 0; i <= kStages; ++i) ev[i] = cur.ev[i];
+    return hipSuccess;
+  }
+  void accumulate_times() { ev_pending.push_back(cur); }
+  void resolve_times() {
+    for (auto& s : ev_pending) {
+      if (hipEventSynchronize(s.ev[kStages]) == hipSuccess)
+        for (int i = 0; i < kStages; ++i) {
+          float t = 0.f;
+          if (hipEventElapsedTime(&t, s.ev[i], s.ev[i + 1]) == hipSuccess) acc_ms[i] += (double)t;
+        }
+      ev_pool.push_back(s);
+    }
+    ev_pending.clear();
   }
 };
 
@@ -117,7 +153,6 @@ extern "C" rz_status rz_context_create(int device, rz_context** out) {
   if (!c) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
   c->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipHostMalloc((void**)&c->host_misc, 4 * sizeof(uint64_t), hipHostMallocDefault));
   *out = c;
   return RZ_OK;
@@ -129,8 +164,10 @@ extern "C" rz_status rz_context_destroy(rz_context* c) {
   (void)hipStreamSynchronize(c->stream);
   for (Buf* b : c->all())
     if (b->p) (void)hipFree(b->p);
-  for (auto& e : c->ev)
-    if (e) (void)hipEventDestroy(e);
+  c->resolve_times();
+  for (auto& s : c->ev_pool)
+    for (auto& e : s.ev)
+      if (e) (void)hipEventDestroy(e);
   if (c->host_misc) (void)hipHostFree(c->host_misc);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -160,23 +197,23 @@ extern "C" rz_status rz_context_synchronize(rz_context* c) {
   return RZ_OK;
 }
 
-extern "C" rz_status rz_context_set_profiling(rz_context* c, int enabled) {
-  c->profiling = enabled != 0;
+extern "C" rz_status rz_context_set_profiling(rz_context* c, int level) {
+  c->profiling = level < 0 ? 0 : (level > 2 ? 2 : level);
   return RZ_OK;
 }
 
 extern "C" const char* rz_stage_name(int i) { return (i >= 0 && i < kStages) ? kStageNames[i] : ""; }
 
 extern "C" int rz_context_stage_times(rz_context* c, double* ms, int n) {
-  if (!c->have_times) return 0;
   (void)hipSetDevice(c->device);
-  if (hipEventSynchronize(c->ev[kStages]) != hipSuccess) return 0;
-  for (int i = 0; i < kStages && i < n; ++i) {
-    float t = 0.f;
-    (void)hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]);
-    ms[i] = (double)t;
-  }
+  c->resolve_times();
+  for (int i = 0; i < kStages && i < n; ++i) ms[i] = c->acc_ms[i];
   return kStages;
+}
+
+extern "C" void rz_context_reset_stage_times(rz_context* c) {
+  c->resolve_times();
+  for (double& m : c->acc_ms) m = 0.0;
 }
 
 extern "C" rz_status rz_context_last_counters(rz_context* c, uint64_t* burns, uint64_t* pairs,
@@ -185,8 +222,10 @@ extern "C" rz_status rz_context_last_counters(rz_context* c, uint64_t* burns, ui
   if (burns) {
     *burns = 0;
     if (c->burns_pending && c->misc.p) {
+      unsigned long long slots[kBurnSlots];
       HIP_TRY(hipStreamSynchronize(c->stream));
-      HIP_TRY(hipMemcpy(burns, (char*)c->misc.p + 8, sizeof(uint64_t), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(slots, (char*)c->misc.p + 64, sizeof(slots), hipMemcpyDeviceToHost));
+      for (unsigned long long v : slots) *burns += v;
     }
   }
   if (pairs) *pairs = c->last_pairs;
@@ -308,37 +347,61 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
 
 namespace {
 
-template <typename T, int FUN>
-void launch_tile(const TileArgs& a, uint32_t n_tiles, hipStream_t s) {
-  hipLaunchKernelGGL((k_tile<T, FUN>), dim3(n_tiles), dim3(256), 0, s, a);
+// Persistent grid for one k_tile instantiation: enough workgroups to fill
+// every CU at the kernel's occupancy, a multiple of the 8 XCDs.
+template <typename T, int FUN, bool LINES>
+uint32_t tile_grid(uint32_t n_tiles) {
+  static uint32_t cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 256, per_cu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile<T, FUN, LINES>, 256, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    cached = (uint32_t)(cus * per_cu);
+  }
+  uint32_t g = std::min<uint32_t>(cached, std::max<uint32_t>(n_tiles, 1));
+  return std::max<uint32_t>(NUM_XCD, (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
 }
 
-template <typename T>
-void launch_tile_fun(int fun, const TileArgs& a, uint32_t n_tiles, hipStream_t s) {
-  switch (fun) {
-    case 0: launch_tile<T, 0>(a, n_tiles, s); break;
-    case 1: launch_tile<T, 1>(a, n_tiles, s); break;
-    case 2: launch_tile<T, 2>(a, n_tiles, s); break;
-    case 3: launch_tile<T, 3>(a, n_tiles, s); break;
-    case 4: launch_tile<T, 4>(a, n_tiles, s); break;
-    case 5: launch_tile<T, 5>(a, n_tiles, s); break;
-    default: launch_tile<T, 6>(a, n_tiles, s); break;
+template <typename T, int FUN>
+void launch_tile(const TileArgs& a, bool lines, hipStream_t s) {
+  if (lines) {
+    hipLaunchKernelGGL((k_tile<T, FUN, true>), dim3(tile_grid<T, FUN, true>(a.n_tiles)), dim3(256),
+                       0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_tile<T, FUN, false>), dim3(tile_grid<T, FUN, false>(a.n_tiles)),
+                       dim3(256), 0, s, a);
   }
 }
 
-void launch_tile_dispatch(rz_dtype dt, int fun, const TileArgs& a, uint32_t n_tiles,
-                          hipStream_t s) {
+template <typename T>
+void launch_tile_fun(int fun, const TileArgs& a, bool lines, hipStream_t s) {
+  switch (fun) {
+    case 0: launch_tile<T, 0>(a, lines, s); break;
+    case 1: launch_tile<T, 1>(a, lines, s); break;
+    case 2: launch_tile<T, 2>(a, lines, s); break;
+    case 3: launch_tile<T, 3>(a, lines, s); break;
+    case 4: launch_tile<T, 4>(a, lines, s); break;
+    case 5: launch_tile<T, 5>(a, lines, s); break;
+    default: launch_tile<T, 6>(a, lines, s); break;
+  }
+}
+
+void launch_tile_dispatch(rz_dtype dt, int fun, const TileArgs& a, bool lines, hipStream_t s) {
   switch (dt) {
-    case RZ_U8: launch_tile_fun<uint8_t>(fun, a, n_tiles, s); break;
-    case RZ_U16: launch_tile_fun<uint16_t>(fun, a, n_tiles, s); break;
-    case RZ_U32: launch_tile_fun<uint32_t>(fun, a, n_tiles, s); break;
-    case RZ_U64: launch_tile_fun<uint64_t>(fun, a, n_tiles, s); break;
-    case RZ_I8: launch_tile_fun<int8_t>(fun, a, n_tiles, s); break;
-    case RZ_I16: launch_tile_fun<int16_t>(fun, a, n_tiles, s); break;
-    case RZ_I32: launch_tile_fun<int32_t>(fun, a, n_tiles, s); break;
-    case RZ_I64: launch_tile_fun<int64_t>(fun, a, n_tiles, s); break;
-    case RZ_F32: launch_tile_fun<float>(fun, a, n_tiles, s); break;
-    default: launch_tile_fun<double>(fun, a, n_tiles, s); break;
+    case RZ_U8: launch_tile_fun<uint8_t>(fun, a, lines, s); break;
+    case RZ_U16: launch_tile_fun<uint16_t>(fun, a, lines, s); break;
+    case RZ_U32: launch_tile_fun<uint32_t>(fun, a, lines, s); break;
+    case RZ_U64: launch_tile_fun<uint64_t>(fun, a, lines, s); break;
+    case RZ_I8: launch_tile_fun<int8_t>(fun, a, lines, s); break;
+    case RZ_I16: launch_tile_fun<int16_t>(fun, a, lines, s); break;
+    case RZ_I32: launch_tile_fun<int32_t>(fun, a, lines, s); break;
+    case RZ_I64: launch_tile_fun<int64_t>(fun, a, lines, s); break;
+    case RZ_F32: launch_tile_fun<float>(fun, a, lines, s); break;
+    default: launch_tile_fun<double>(fun, a, lines, s); break;
   }
 }
 
@@ -359,7 +422,7 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
     return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not implemented on the device path yet");
   (void)sum_mode;
   if (!layer_dev) n_layers = 1;
-  if (n_layers == 0) return rz::fail(RZ_VALUE_ERROR, "n_layers must be >= 1");
+  if (n_layers == 0 || n_layers >= (1u << 29)) return rz::fail(RZ_VALUE_ERROR, "n_layers must be in [1, 2^29)");
   if (info->nrows == 0 || info->ncols == 0) return RZ_OK;  // empty raster: nothing to write
   if (info->nrows >= (uint64_t)INT32_MAX - TILE_H || info->ncols >= (uint64_t)INT32_MAX - 256)
     return rz::fail(RZ_CAPACITY_ERROR, "raster dimensions exceed 2^31 rows or columns");
@@ -377,18 +440,31 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
   hipStream_t s = c->stream;
 
   // workspace
-  HIP_TRY(ensure(c->misc, 64));
+  const size_t misc_bytes = 64 + kBurnSlots * sizeof(unsigned long long);
+  HIP_TRY(ensure(c->misc, misc_bytes));
   uint32_t* d_status = (uint32_t*)c->misc.p;
-  unsigned long long* d_burns = (unsigned long long*)((char*)c->misc.p + 8);
-  HIP_TRY(hipMemsetAsync(c->misc.p, 0, 16, s));
+  unsigned long long* d_burns = (unsigned long long*)((char*)c->misc.p + 64);
+  HIP_TRY(hipMemsetAsync(c->misc.p, 0, c->profiling > 1 ? misc_bytes : 64, s));
   if (dg->has_poly) HIP_TRY(ensure(c->edges, (size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)));
   if (dg->has_line || dg->has_point) HIP_TRY(ensure(c->px, (size_t)std::max<int64_t>(V, 1) * 16));
   if (dg->has_line || dg->has_point) HIP_TRY(ensure(c->vflags, (size_t)std::max<int64_t>(V, 1)));
   HIP_TRY(ensure(c->item_box, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox)));
   HIP_TRY(ensure(c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
+  HIP_TRY(ensure(c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
 
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], s));
-  // 1. edges / pixel coordinates / item boxes
+  // RUSTERIZE_BINNING=always|never overrides the size heuristic (tests use it
+  // to run both item-list forms on the same inputs).
+  bool scan_mode = I <= 256 || (uint64_t)I * (uint64_t)n_tiles <= ((uint64_t)1 << 24);
+  if (const char* bm = std::getenv("RUSTERIZE_BINNING")) {
+    if (std::strcmp(bm, "always") == 0) scan_mode = I == 0;
+    else if (std::strcmp(bm, "never") == 0) scan_mode = true;
+  }
+
+  if (c->profiling) {
+    HIP_TRY(c->take_events());
+    HIP_TRY(hipEventRecord(c->ev[0], s));
+  }
+  // 1. edges / pixel coordinates / item boxes, then per-call item descriptors
   if (I > 0) {
     hipLaunchKernelGGL(k_init_items, dim3(blocks_for(I)), dim3(256), 0, s, (ItemBox*)c->item_box.p,
                        (unsigned long long*)c->item_last.p, I);
@@ -414,35 +490,46 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
     pa.item_last = (unsigned long long*)c->item_last.p;
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
   }
+  if (!scan_mode) {
+    HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
+    HIP_TRY(hipMemsetAsync((uint32_t*)c->counts.p + I, 0, 4, s));
+  }
+  if (I > 0) {
+    DescArgs da;
+    da.item_box = (const ItemBox*)c->item_box.p;
+    da.item_kind = dg->item_kind;
+    da.item_geom = dg->item_geom;
+    da.item_vbeg = dg->item_vbeg;
+    da.item_last = (const unsigned long long*)c->item_last.p;
+    da.layer_of_geom = layer_dev;
+    da.field = field_dev;
+    da.scalar_bits = 0;
+    if (!field_dev) std::memcpy(&da.scalar_bits, scalar_host, esz);
+    da.esz = (int32_t)esz;
+    da.n_items = I;
+    da.desc = (ItemDesc*)c->desc.p;
+    da.ntx = ntx;
+    da.nty = nty;
+    da.tile_w = TW;
+    da.counts = scan_mode ? nullptr : (uint32_t*)c->counts.p;
+    hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
+  }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
 
   // 2. binning: scan mode for few items, stable sort-middle bins otherwise
   TileArgs ta;
   std::memset(&ta, 0, sizeof(ta));
-  // RUSTERIZE_BINNING=always|never overrides the size heuristic (tests use it
-  // to run both item-list forms on the same inputs).
-  bool scan_mode = I <= 256 || (uint64_t)I * (uint64_t)n_tiles <= ((uint64_t)1 << 24);
-  if (const char* bm = std::getenv("RUSTERIZE_BINNING")) {
-    if (std::strcmp(bm, "always") == 0) scan_mode = I == 0;
-    else if (std::strcmp(bm, "never") == 0) scan_mode = true;
-  }
   uint64_t n_pairs = 0;
   if (!scan_mode) {
-    HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
-    HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
     BinArgs ba;
-    ba.item_box = (const ItemBox*)c->item_box.p;
-    ba.item_geom = dg->item_geom;
-    ba.layer_of_geom = layer_dev;
+    ba.desc = (const ItemDesc*)c->desc.p;
     ba.n_items = I;
     ba.ntx = ntx;
     ba.nty = nty;
     ba.tile_w = TW;
     ba.tiles_per_layer = (uint32_t)tpl;
-    ba.counts = (uint32_t*)c->counts.p;
     ba.offsets = (const uint64_t*)c->offsets.p;
-    HIP_TRY(hipMemsetAsync((uint32_t*)c->counts.p + I, 0, 4, s));
-    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(I)), dim3(256), 0, s, ba);
     size_t tb = 0;
     HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->counts.p,
                                     (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
@@ -496,29 +583,21 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
   ta.tiles_per_layer = (uint32_t)tpl;
   ta.n_tiles = n_tiles;
   ta.n_items = I;
-  ta.item_kind = dg->item_kind;
-  ta.item_box = (const ItemBox*)c->item_box.p;
-  ta.item_geom = dg->item_geom;
-  ta.item_vbeg = dg->item_vbeg;
-  ta.item_last = (const unsigned long long*)c->item_last.p;
-  ta.layer_of_geom = layer_dev;
+  ta.desc = (const ItemDesc*)c->desc.p;
   ta.edges = (const PolyEdge*)c->edges.p;
   ta.px = (const double2*)c->px.p;
   ta.vflags = (const uint8_t*)c->vflags.p;
-  ta.field = field_dev;
-  ta.scalar_bits = 0;
   ta.bg_bits = 0;
-  if (scalar_host) std::memcpy(&ta.scalar_bits, scalar_host, esz);
   std::memcpy(&ta.bg_bits, bg_host, esz);
   ta.dedup_lines = info->xres != info->yres;
   ta.out = out_dev;
-  ta.burns = c->profiling ? d_burns : nullptr;
+  ta.burns = c->profiling > 1 ? d_burns : nullptr;
   ta.status = d_status;
-  launch_tile_dispatch(dtype, (int)fun, ta, n_tiles, s);
+  launch_tile_dispatch(dtype, (int)fun, ta, dg->has_line || dg->has_point, s);
   HIP_TRY(hipGetLastError());
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
-  c->have_times = c->profiling;
-  c->burns_pending = c->profiling;
+  if (c->profiling) c->accumulate_times();
+  c->burns_pending = c->profiling > 1;
   c->last_pairs = n_pairs;
   c->last_tiles = n_tiles;
   return RZ_OK;
@@ -624,11 +703,10 @@ extern "C" int rz_debug_row_coverage(const double* xs, int64_t n, int64_t ncols,
   int32_t amax = -1;
   for (int64_t i = 0; i < n; ++i)
     if (!is_pos_nan(xs[i])) toggle_crossing(span_col(xs[i], (double)ncols), C0, width, tog, lp, amax);
-  const int ppt = width / 4;
-  for (int q = 0; q < 4; ++q) {
-    uint64_t cov = width == 64 ? segment_coverage<16, 1>(tog, lp, amax, C0, q * ppt)
-                               : segment_coverage<32, 2>(tog, lp, amax, C0, q * ppt);
-    for (int j = 0; j < ppt; ++j) out[q * ppt + j] = (uint8_t)((cov >> j) & 1u);
+  for (int w = 0; w < width / 64; ++w) {
+    uint64_t cov = width == 64 ? row_coverage_word<1>(tog, lp, amax, C0, w)
+                               : row_coverage_word<2>(tog, lp, amax, C0, w);
+    for (int j = 0; j < 64; ++j) out[w * 64 + j] = (uint8_t)((cov >> j) & 1u);
   }
   return 0;
 }

@@ -98,6 +98,25 @@ def cpu_baseline(g, f32, lay, seconds_cap: float):
     }
 
 
+def copy_calibration(nbytes: int) -> int:
+    """One device-to-device copy of nbytes (after a warm-up copy): reading
+    and writing nbytes each is what a traffic counter should report for it."""
+    L = _lib.lib()
+    ctx = c_void_p()
+    check(L.rz_context_create(0, byref(ctx)))
+    src, dst = c_void_p(), c_void_p()
+    check(L.rz_device_malloc(ctx, nbytes, byref(src)))
+    check(L.rz_device_malloc(ctx, nbytes, byref(dst)))
+    for _ in range(2):
+        check(L.rz_memcpy_d2d(ctx, dst, src, nbytes))
+        check(L.rz_context_synchronize(ctx))
+    L.rz_device_free(ctx, src)
+    L.rz_device_free(ctx, dst)
+    L.rz_context_destroy(ctx)
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +124,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--polygons", type=int, default=N_POLY)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", default=None,
+                    help="DTYPE:FUN -- one warm-up and one timed call of that combination only "
+                         "(for counter collection)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run only a device-to-device copy of this many bytes (the plain copy "
+                         "kernel that calibrates HBM traffic counters)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        return copy_calibration(args.copy_bytes)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the rasteriser shards by output rows with no exchange "
               "(see DESIGN.md); multi-GPU runs are not part of this build", file=sys.stderr)
@@ -146,6 +173,20 @@ def main() -> int:
         for dt in DTYPES:
             for fun in FUNS:
                 call(dt, fun)
+
+    if args.only:
+        dt, fun = args.only.split(":")
+        call(dt, fun)  # warm-up
+        check(L.rz_context_synchronize(ctx))
+        check(L.rz_context_set_profiling(ctx, 1))
+        L.rz_context_reset_stage_times(ctx)
+        call(dt, fun)
+        check(L.rz_context_synchronize(ctx))
+        times = (c_double * 8)()
+        n = L.rz_context_stage_times(ctx, times, 8)
+        print(json.dumps({"only": args.only, "stages_ms": {L.rz_stage_name(i).decode(): times[i] for i in range(n)},
+                          "b_alg_tile": b_alg_tile(g, np.dtype(dt).itemsize)}), flush=True)
+        return 0
 
     # HIP events around every stage of every call, recorded on the context's
     # stream and resolved after the timed region (no added synchronisation)

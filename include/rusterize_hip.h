@@ -204,6 +204,9 @@ rz_status rz_device_malloc(rz_context* ctx, size_t bytes, void** out);
 rz_status rz_device_free(rz_context* ctx, void* ptr);
 rz_status rz_memcpy_h2d(rz_context* ctx, void* dst, const void* src, size_t bytes);
 rz_status rz_memcpy_d2h(rz_context* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy on the context's stream (asynchronous); also the
+ * plain copy kernel used to calibrate HBM traffic counters. */
+rz_status rz_memcpy_d2d(rz_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* Uploads a geometry set once; it stays resident until freed and can be
  * rasterised any number of times with different fields and grids. */
@@ -250,7 +253,7 @@ rz_status rz_context_last_counters(rz_context* ctx, uint64_t* burns, uint64_t* p
  * returns the count and stores up to cap cells.
  * rz_debug_row_coverage: which of the columns [C0, C0+width) the spans of
  * burn_polygon (burners.rs:302-316) cover for the crossings xs[0..n) of one
- * row; width is 64 or 128. */
+ * row; width is 64, 128 or 256. */
 int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, int64_t iy1, int64_t X0,
                             int64_t X1, int64_t Y0, int64_t Y1, int64_t* xs, int64_t* ys,
                             int64_t cap);

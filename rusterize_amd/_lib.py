@@ -14,7 +14,7 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "librusterize_hip.so"
+LIB_PATH = Path(os.environ.get("RUSTERIZE_LIBRARY", Path(__file__).resolve().parent / "librusterize_hip.so"))
 
 DTYPES = ("uint8", "uint16", "uint32", "uint64", "int8", "int16", "int32", "int64", "float32", "float64")
 FUNS = ("sum", "first", "last", "min", "max", "count", "any")
@@ -77,6 +77,7 @@ _SIGNATURES = [
     ("rz_device_free", c_int, [c_void_p, c_void_p]),
     ("rz_memcpy_h2d", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     ("rz_memcpy_d2h", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    ("rz_memcpy_d2d", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     ("rz_device_geometry_upload", c_int, [c_void_p, POINTER(RzGeometry), POINTER(c_void_p)]),
     ("rz_device_geometry_free", c_int, [c_void_p]),
     ("rz_device_geometry_count", c_int64, [c_void_p]),

@@ -28,8 +28,9 @@ constexpr int TILE_H = 64;       // rows per tile (4 waves x 16 rows)
 constexpr int WAVE_ROWS = 16;    // rows per wave strip
 constexpr int NUM_XCD = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 
-// Tile width in pixels for an element size; rows of 128-256 bytes.
-__host__ __device__ constexpr int tile_w(int esz) { return esz <= 2 ? 128 : 64; }
+// Tile width in pixels for an element size: rows of 256 bytes (512 for
+// 16-bit and 64-bit elements), 16 KB or more per tile.
+__host__ __device__ constexpr int tile_w(int esz) { return esz <= 2 ? 256 : 64; }
 
 // Edge record of one polygon segment (PolyEdge, rust/src/geo/edges.rs:17-24).
 // ystart / yend are the reference's usize values clamped to nrows, which is
@@ -108,7 +109,7 @@ struct TileArgs {
   uint32_t tiles_per_layer, n_tiles;
   // binned mode: pairs sorted by tile, tile_off[t]..tile_off[t+1]
   const uint32_t* tile_off;
-  const uint32_t* pair_item;
+  const ItemDesc* pair_desc;  // descriptors in pair order (binned mode)
   // scan mode (pair_item == nullptr): every item is tested against the tile
   int64_t n_items;
   const ItemDesc* desc;
@@ -155,6 +156,19 @@ __host__ __device__ inline void toggle_crossing(int32_t col, int32_t C0, int32_t
   const int32_t rel = col - C0;
   if (rel <= 0) lp ^= 1u;
   else if (rel < TW) tog[rel >> 6] ^= 1ull << (rel & 63);
+}
+
+// Branch-free form of toggle_crossing for NWORD-word windows, applied only
+// where `on` (inactive edges of a predicated loop contribute nothing).
+template <int NWORD>
+__host__ __device__ inline void toggle_crossing_if(bool on, int32_t col, int32_t C0,
+                                                   uint64_t* tog, uint32_t& lp, int32_t& amax) {
+  amax = (on && col > amax) ? col : amax;
+  const int32_t rel = col - C0;
+  lp ^= (on && rel <= 0) ? 1u : 0u;
+  const bool in = on && rel > 0 && rel < NWORD * 64;
+  const uint64_t bit = 1ull << (rel & 63);
+  for (int w = 0; w < NWORD; ++w) tog[w] ^= (in && (rel >> 6) == w) ? bit : 0ull;
 }
 
 // Inclusive prefix-XOR of the bits of one 64-bit word.

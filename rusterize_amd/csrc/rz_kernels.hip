@@ -252,6 +252,14 @@ __global__ void k_bin_emit(BinArgs a) {
     }
 }
 
+// Descriptors in pair order, so a tile's items are one contiguous run.
+__global__ void k_gather_desc(const ItemDesc* desc, const uint32_t* pair_item, uint32_t n_pairs,
+                              ItemDesc* pair_desc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pairs) return;
+  pair_desc[i] = desc[pair_item[i]];
+}
+
 // tile_off[t] = first pair index whose key >= t, for t in [0, n_tiles].
 __global__ void k_tile_offsets(const uint32_t* keys, uint32_t n_pairs, uint32_t n_tiles,
                                uint32_t* tile_off) {
@@ -308,6 +316,21 @@ __device__ inline void fold(T& p, T v, T bg, uint32_t k) {
   } else {  // any
     p = (T)1;
   }
+}
+
+// One write of value v, applied where `hit`: the same transitions as fold
+// with k = 1, written as selects so the compiler emits no branch per pixel.
+template <typename T, int FUN>
+__device__ inline T fold_select(T p, T v, T bg, bool hit) {
+  T n;
+  if (FUN == 0) n = (p == bg || isnan_t(p) || isnan_t(v)) ? v : add_wrap(p, v);
+  else if (FUN == 1) n = (p == bg || isnan_t(p)) ? v : p;
+  else if (FUN == 2) n = v;
+  else if (FUN == 3) n = (p == bg || isnan_t(p) || p > v) ? v : p;
+  else if (FUN == 4) n = (p == bg || isnan_t(p) || p < v) ? v : p;
+  else if (FUN == 5) n = (p == bg || isnan_t(p)) ? (T)1 : add_wrap(p, (T)1);
+  else n = (T)1;
+  return hit ? n : p;
 }
 
 template <typename T> __device__ inline T from_bits(uint64_t b) {

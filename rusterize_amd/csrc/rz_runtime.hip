@@ -68,7 +68,7 @@ struct rz_context {
   int device = 0;
   hipStream_t stream = nullptr;
   // workspace
-  Buf edges, px, vflags, item_box, item_last, desc, counts, offsets, keys_a, vals_a, keys_b,
+  Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
       vals_b, tile_off, temp, misc;
   uint64_t* host_misc = nullptr;  // pinned: [0] total pairs, [1] status
   // profiling: 0 off, 1 stage events, 2 stage events + exact burn counts
@@ -81,7 +81,7 @@ struct rz_context {
   bool burns_pending = false;
   std::vector<Buf*> all() {
     return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts, &offsets,
-            &keys_a, &vals_a, &keys_b, &vals_b,   &tile_off,  &temp, &misc};
+            &keys_a, &vals_a, &keys_b, &vals_b,   &tile_off,  &temp, &misc, &pair_desc};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -247,6 +247,11 @@ extern "C" rz_status rz_memcpy_h2d(rz_context* c, void* dst, const void* src, si
   HIP_TRY(hipSetDevice(c->device));
   if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return RZ_OK;
+}
+extern "C" rz_status rz_memcpy_d2d(rz_context* c, void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipSetDevice(c->device));
+  if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
   return RZ_OK;
 }
 extern "C" rz_status rz_memcpy_d2h(rz_context* c, void* dst, const void* src, size_t bytes) {
@@ -569,7 +574,15 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
                        (const uint32_t*)c->keys_b.p, (uint32_t)n_pairs, n_tiles,
                        (uint32_t*)c->tile_off.p);
     ta.tile_off = (const uint32_t*)c->tile_off.p;
-    ta.pair_item = (const uint32_t*)c->vals_b.p;
+    if (n_pairs > 0) {
+      HIP_TRY(ensure(c->pair_desc, (size_t)n_pairs * sizeof(ItemDesc)));
+      hipLaunchKernelGGL(k_gather_desc, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
+                         (const ItemDesc*)c->desc.p, (const uint32_t*)c->vals_b.p,
+                         (uint32_t)n_pairs, (ItemDesc*)c->pair_desc.p);
+    } else {
+      HIP_TRY(ensure(c->pair_desc, sizeof(ItemDesc)));
+    }
+    ta.pair_desc = (const ItemDesc*)c->pair_desc.p;
   } else if (c->profiling) {
     HIP_TRY(hipEventRecord(c->ev[2], s));
   }
@@ -694,18 +707,19 @@ extern "C" int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, in
 
 // Host evaluation of the polygon row coverage used by the tile kernel: the
 // covered columns of [C0, C0+width) for crossings xs[0..n) of one row, as a
-// 0/1 byte per column.  width must be 64 or 128.
+// 0/1 byte per column.  width must be 64, 128 or 256.
 extern "C" int rz_debug_row_coverage(const double* xs, int64_t n, int64_t ncols, int32_t C0,
                                      int32_t width, uint8_t* out) {
-  if (width != 64 && width != 128) return -1;
-  uint64_t tog[2] = {0, 0};
+  if (width != 64 && width != 128 && width != 256) return -1;
+  uint64_t tog[4] = {0, 0, 0, 0};
   uint32_t lp = 0;
   int32_t amax = -1;
   for (int64_t i = 0; i < n; ++i)
     if (!is_pos_nan(xs[i])) toggle_crossing(span_col(xs[i], (double)ncols), C0, width, tog, lp, amax);
   for (int w = 0; w < width / 64; ++w) {
-    uint64_t cov = width == 64 ? row_coverage_word<1>(tog, lp, amax, C0, w)
-                               : row_coverage_word<2>(tog, lp, amax, C0, w);
+    uint64_t cov = width == 64    ? row_coverage_word<1>(tog, lp, amax, C0, w)
+                   : width == 128 ? row_coverage_word<2>(tog, lp, amax, C0, w)
+                                  : row_coverage_word<4>(tog, lp, amax, C0, w);
     for (int j = 0; j < 64; ++j) out[w * 64 + j] = (uint8_t)((cov >> j) & 1u);
   }
   return 0;

@@ -14,17 +14,35 @@ __device__ inline void wave_sync() {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// XOR-combine a value across the four lanes of a quad with DPP quad
+// permutations ([1,0,3,2] then [2,3,0,1]): VALU data movement, no LDS.
+__device__ inline uint32_t quad_xor_all(uint32_t x) {
+  x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+  x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+  return x;
+}
+__device__ inline uint64_t quad_xor_all64(uint64_t x) {
+  return (uint64_t)quad_xor_all((uint32_t)x) | ((uint64_t)quad_xor_all((uint32_t)(x >> 32)) << 32);
+}
+__device__ inline int32_t quad_max_all(int32_t x) {
+  x = max(x, __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false));
+  return x;
+}
+
 // Persistent tile kernel.
 //
 // Scheduling: workgroup b runs on XCD b % 8; each XCD owns a contiguous
 // eighth of the tiles (neighbouring tiles share cache lines and items in
 // that XCD's L2) and its workgroups walk that range with a stride.  The next
 // tile's item range is fetched with scalar loads before the stores go out.
+// The four waves of a workgroup work on the four 16-row strips of a tile
+// independently (no workgroup barrier).
 //
-// Per tile (64 rows x TW columns, one 16-row strip per wave): the background
-// fill of RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and every
-// write of every item meeting the tile, folded in geometry order in
-// registers, then written to HBM once.
+// Per tile (64 rows x TW columns): the background fill of
+// RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and every write of
+// every item meeting the tile, folded in geometry order in registers, then
+// written to HBM once.
 //
 // Two lane layouts per wave:
 //  * coverage: lane (rl = lane/4, q = lane%4) takes every 4th active edge of
@@ -33,6 +51,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 //  * fold / store: lane l owns the 16-byte chunk l % LPR of rows
 //    l / LPR + k * RSTEP, so every store instruction writes whole tile rows
 //    (full 128-byte lines, contiguous across lanes).
+// The polygon path is branch-free per lane (predicated crossings and folds).
 template <typename T, int FUN, bool LINES>
 __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
   constexpr int TW = tile_w(sizeof(T));
@@ -43,7 +62,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
   constexpr int RPL = WAVE_ROWS / RSTEP;            // rows per lane
   constexpr int NPIX = RPL * PPC;                   // pixels per lane
   constexpr int CNT_WORDS = LINES ? WAVE_ROWS * TW / 2 : 1;  // u16 counters, 2 per word
-  static_assert(NPIX <= 32, "dedup bits assume <= 32 pixels per lane");
+  static_assert(NPIX <= 64, "dedup bits assume <= 64 pixels per lane");
   __shared__ PolyEdge s_edge[4][64];
   __shared__ uint64_t s_mask[4][2][WAVE_ROWS][NWORD];
   __shared__ uint32_t s_cnt[4][CNT_WORDS];
@@ -68,10 +87,11 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
   const uint32_t slots = gridDim.x / NUM_XCD;
   const uint32_t per = a.n_tiles / NUM_XCD, rem = a.n_tiles % NUM_XCD;
   const uint32_t lo = xcd * per + min(xcd, rem), hi = lo + per + (xcd < rem ? 1u : 0u);
+  const bool binned = a.pair_desc != nullptr;
 
   uint32_t t = lo + slot;
   uint32_t pb = 0, pe = 0;
-  if (t < hi && a.pair_item) {
+  if (t < hi && binned) {
     pb = a.tile_off[t];
     pe = a.tile_off[t + 1];
   }
@@ -83,14 +103,15 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     const int32_t Rw = R0 + wave * WAVE_ROWS;  // first row of this wave's strip
     const int32_t r = Rw + rl;                 // coverage-layout row of this lane
     const int32_t strip_r1 = min(Rw + WAVE_ROWS, a.nrows);
+    const int32_t tile_r1 = min(R0 + TILE_H, a.nrows);
     const int32_t tile_c1 = min(C0 + TW, a.ncols);
+    const bool row_ok = r < strip_r1;
 
     T acc[NPIX];  // acc[k * PPC + j]: row srow + k*RSTEP, column ccol + j
 #pragma unroll
     for (int j = 0; j < NPIX; ++j) acc[j] = bg;
 
-    auto process_item = [&](uint32_t item) {
-      const ItemDesc d = a.desc[item];
+    auto process_item = [&](const ItemDesc& d) {
       // wave-uniform skip: the item box misses this wave's strip or the tile
       if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) return;
       const int32_t kind = (int32_t)(d.kind_layer & 3u);
@@ -98,7 +119,6 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
       const int64_t vb = d.vbeg, ve = d.vbeg + d.nv;
       if (kind == 0) {
         // ---- polygon: even-odd coverage of every row from crossing counts
-        // (the note above row_coverage_word in rz_kernels.h states why this equals the spans).
         uint64_t tog[NWORD];
 #pragma unroll
         for (int w = 0; w < NWORD; ++w) tog[w] = 0ull;
@@ -117,39 +137,34 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
           if (act) stage[__popcll(m & ((1ull << lane) - 1ull))] = ed;
           wave_sync();
           const int n = __popcll(m);
-          if (r < strip_r1) {
-            for (int k = q; k < n; k += 4) {
-              const PolyEdge ee = stage[k];
-              if (ee.ystart <= r && r < ee.yend) {
-                const double x = crossing(ee, r);
-                if (!is_pos_nan(x)) toggle_crossing(span_col(x, ncols_f), C0, TW, tog, lp, amax);
-              }
-            }
+          for (int k0 = 0; k0 < n; k0 += 4) {  // uniform trip count, predicated lanes
+            const int k = k0 + q;
+            const PolyEdge ee = stage[k < n ? k : n - 1];
+            const bool on = k < n && row_ok && ee.ystart <= r && r < ee.yend;
+            const double x = crossing(ee, r);
+            toggle_crossing_if<NWORD>(on && !is_pos_nan(x), span_col(x, ncols_f), C0, tog, lp,
+                                      amax);
           }
           wave_sync();
         }
-        // combine the four lanes of this row (lanes 4*rl .. 4*rl+3)
+        // combine the four lanes of this row, then publish the row's coverage
+        // mask (the note above row_coverage_word in rz_kernels.h states why it
+        // equals the reference's spans) for the store layout
+        lp = quad_xor_all(lp);
+        amax = quad_max_all(amax);
 #pragma unroll
-        for (int msk = 1; msk <= 2; msk <<= 1) {
-          lp ^= (uint32_t)__shfl_xor((int)lp, msk, 64);
-          amax = max(amax, __shfl_xor(amax, msk, 64));
-#pragma unroll
-          for (int w = 0; w < NWORD; ++w)
-            tog[w] ^= (uint64_t)__shfl_xor((unsigned long long)tog[w], msk, 64);
-        }
-        // row coverage mask: inclusive prefix parity, limited to c < max column
+        for (int w = 0; w < NWORD; ++w) tog[w] = quad_xor_all64(tog[w]);
         if (q < NWORD) s_mask[wave][mbuf][rl][q] = row_coverage_word<NWORD>(tog, lp, amax, C0, q);
         wave_sync();
-        // fold in the store layout
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
           const uint64_t row_word = s_mask[wave][mbuf][srow + k * RSTEP][ccol >> 6];
           const uint32_t bits = (uint32_t)(row_word >> (ccol & 63)) & ((1u << PPC) - 1u);
-          if (bits) {
+          if (__any(bits != 0u)) {  // wave-uniform skip of uncovered row groups
             burns += (unsigned long long)__popc(bits);
 #pragma unroll
             for (int j = 0; j < PPC; ++j)
-              if ((bits >> j) & 1u) fold<T, FUN>(acc[k * PPC + j], v, bg, 1u);
+              acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
           }
         }
         mbuf ^= 1;  // the next polygon writes the other buffer
@@ -163,7 +178,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
       };
       const bool dedup = kind == 1 && a.dedup_lines;
-      uint32_t seen = 0u;  // PixelCache bits of this lane's pixels
+      uint64_t seen = 0u;  // PixelCache bits of this lane's pixels
       for (int64_t cs = vb; cs < ve; cs += 65535) {  // <= 65535 writes per pixel per chunk
         const int64_t ce = min(ve, cs + 65535);
         for (int64_t s = cs + lane; s < ce; s += 64) {
@@ -209,7 +224,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
                 uint32_t kk = (wv >> (16 * h)) & 0xFFFFu;
                 const int j = k * PPC + 2 * j2 + h;
                 if (kk && dedup) {  // LineWriter + PixelCache: one write per pixel per item
-                  const uint32_t bit = 1u << j;
+                  const uint64_t bit = 1ull << j;
                   if (seen & bit) kk = 0;
                   else {
                     seen |= bit;
@@ -228,11 +243,10 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
       }
     };
 
-    if (a.pair_item) {
-      for (uint32_t i = pb; i < pe; ++i) process_item(a.pair_item[i]);
+    if (binned) {
+      for (uint32_t i = pb; i < pe; ++i) process_item(a.pair_desc[i]);
     } else {
       // scan mode: test every item's box in order (small item counts)
-      const int32_t tile_r1 = min(R0 + TILE_H, a.nrows);
       for (int64_t base = 0; base < a.n_items; base += 64) {
         const int64_t it = base + lane;
         bool hit = false;
@@ -245,14 +259,14 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         while (m) {
           const int bit = __builtin_ctzll(m);
           m &= m - 1;
-          process_item((uint32_t)(base + bit));
+          process_item(a.desc[base + bit]);
         }
       }
     }
 
     // fetch the next tile's item range before this tile's stores
     const uint32_t tn = t + slots;
-    if (tn < hi && a.pair_item) {
+    if (tn < hi && binned) {
       pb = a.tile_off[tn];
       pe = a.tile_off[tn + 1];
     }

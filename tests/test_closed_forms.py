@@ -115,7 +115,7 @@ NEG_NAN = struct.unpack("<d", struct.pack("<Q", 0xFFF8000000000000))[0]
 POS_NAN = struct.unpack("<d", struct.pack("<Q", 0x7FF8000000000000))[0]
 
 
-@pytest.mark.parametrize("width", [64, 128])
+@pytest.mark.parametrize("width", [64, 128, 256])
 @pytest.mark.parametrize("seed", range(4))
 def test_row_coverage_matches_sorted_spans(width, seed):
     rng = np.random.default_rng(seed)

@@ -1,21 +1,29 @@
 """Benchmark of the rasterisation hot path on one MI355X.
 
-Workload (the large single-raster case): 1,000,000 random convex polygons
-with 4-12 vertices and diameters log-uniform in [2, 200] px, 10 % of them
-MultiPolygons of 2-4 parts and 20 % with 1-3 holes at half scale, on a
-40,000 x 40,000 grid (res 1), an int32 field uniform on [0, 1e6) and a
-`by` column of 8 classes (8 output bands).  One step rasterises it with
-each of count, any, first, last, min and max into uint8, int32 and float32
-outputs (18 calls), with geometry, fields and output resident in HBM.
+Default workload (the large single-raster case, the headline line):
+1,000,000 random convex polygons with 4-12 vertices and diameters
+log-uniform in [2, 200] px, 10 % of them MultiPolygons of 2-4 parts and 20 %
+with 1-3 holes at half scale, on a 40,000 x 40,000 grid (res 1), an int32
+field uniform on [0, 1e6) and a `by` column of 8 classes (8 output bands).
+One step rasterises it with each of count, any, first, last, min and max into
+uint8, int32 and float32 outputs (18 calls), with geometry, fields and output
+resident in HBM.
 
-Prints one JSON line (see the keys below).  Timing: W untimed warm-up steps,
-then exactly K steps bracketed by device synchronisation; the tile kernel's
-time comes from HIP events recorded on the context's own stream.
+--workload stars: 1,000 star polygons (8-64 vertices, R in [5, 50] px) on
+2,048 x 2,048, float64 field U[0, 1), fun=sum, background NaN; one call per
+step.
+--workload lines: 500,000 random-walk linestrings, 2,000,000 points and
+50,000 collections on 20,000 x 20,000, float32 field N(0, 1), fun=max and
+fun=sum (two calls per step), background NaN.
+
+Prints one JSON line.  Timing: W untimed warm-up steps, then exactly K steps
+bracketed by device synchronisation; per-stage times (the tile kernel's
+among them) come from HIP events recorded on the context's own stream over
+the timed region.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes
 import json
 import os
 import platform
@@ -33,11 +41,7 @@ from rusterize_amd import _lib  # noqa: E402
 from rusterize_amd._lib import Geometry, check  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-FUNS = ["count", "any", "first", "last", "min", "max"]
-DTYPES = ["uint8", "int32", "float32"]
-W = H = 40000
-N_POLY = 1_000_000
-N_BANDS = 8
+METRIC = "pixel-burns/s + HBM GB/s vs roofline; batched/served vs plain p50/p99"
 
 
 def log(*a):
@@ -54,49 +58,89 @@ def cpu_model() -> str:
     return platform.processor()
 
 
-def make_inputs(n_poly: int):
-    g = Geometry.synth(2, 2, n_poly, float(W), float(H), 2.0, 200.0, 0.1, 0.2)
-    f32 = np.empty(g.n_geoms, np.int32)
-    check(_lib.lib().rz_synth_field(6, 0, 202, g.n_geoms, 0.0, 1e6, f32.ctypes.data))
-    lay = np.empty(g.n_geoms, np.uint32)
-    check(_lib.lib().rz_synth_field(2, 0, 203, g.n_geoms, 0.0, float(N_BANDS), lay.ctypes.data))
-    return g, f32, lay
+# ----------------------------------------------------------------- workloads
+
+class Workload:
+    """Inputs of one benchmark configuration and the calls of one step."""
+
+    def __init__(self, name, geom, W, H, bands, fields, layers, calls, bg, description):
+        self.name, self.geom, self.W, self.H, self.bands = name, geom, W, H, bands
+        self.fields, self.layers, self.calls, self.bg = fields, layers, calls, bg
+        self.description = description
+
+    def b_alg(self, esz: int) -> int:
+        """Algorithmic bytes of one call: 16 B per input vertex, the field and
+        band index per geometry, and the raster written once."""
+        g = self.geom
+        return 16 * g.n_vertices + esz * g.n_geoms + (4 * g.n_geoms if self.layers is not None else 0) + \
+            esz * self.bands * self.W * self.H
+
+    def b_alg_tile(self, esz: int) -> int:
+        """Algorithmic bytes of the tile kernel: the raster written once, every
+        vertex record (polygon edges 32 B, line / point coordinates 16 B) and
+        field value read once."""
+        g = self.geom
+        return esz * self.bands * self.W * self.H + 32 * g.n_vertices + esz * g.n_geoms
 
 
-def b_alg(g: Geometry, esz: int) -> int:
-    """Algorithmic bytes of one call: 16 B per input vertex, the field and
-    band index per geometry, and the raster written once."""
-    return 16 * g.n_vertices + esz * g.n_geoms + 4 * g.n_geoms + esz * N_BANDS * W * H
+def make_workload(name: str, n_poly: int) -> Workload:
+    L = _lib.lib()
+    if name == "polygons":
+        W = H = 40000
+        g = Geometry.synth(2, 2, n_poly, float(W), float(H), 2.0, 200.0, 0.1, 0.2)
+        f32 = np.empty(g.n_geoms, np.int32)
+        check(L.rz_synth_field(6, 0, 202, g.n_geoms, 0.0, 1e6, f32.ctypes.data))
+        lay = np.empty(g.n_geoms, np.uint32)
+        check(L.rz_synth_field(2, 0, 203, g.n_geoms, 0.0, 8.0, lay.ctypes.data))
+        fields = {dt: f32.astype(dt) for dt in ("uint8", "int32", "float32")}  # numpy cast: wraps
+        calls = [(dt, fun) for dt in ("uint8", "int32", "float32")
+                 for fun in ("count", "any", "first", "last", "min", "max")]
+        return Workload(name, g, W, H, 8, fields, lay, calls, 0.0,
+                        f"{g.n_geoms} convex polygons (10% multi, 20% holed) on {W}x{H}, by=8 bands, "
+                        f"funs count/any/first/last/min/max x dtypes uint8/int32/float32 per step")
+    if name == "stars":
+        W = H = 2048
+        g = Geometry.synth(1, 1, 1000, float(W), float(H), 5.0, 50.0)
+        f = np.empty(g.n_geoms, np.float64)
+        check(L.rz_synth_field(9, 2, 101, g.n_geoms, 0.0, 1.0, f.ctypes.data))
+        return Workload(name, g, W, H, 1, {"float64": f}, None, [("float64", "sum")], float("nan"),
+                        f"{g.n_geoms} star polygons on {W}x{H}, float64 sum, background NaN")
+    if name == "lines":
+        W = H = 20000
+        g = Geometry.synth(3, 3, 0, float(W), float(H), 500000, 2000000, 50000)
+        f = np.empty(g.n_geoms, np.float32)
+        check(L.rz_synth_field(8, 1, 103, g.n_geoms, 0.0, 1.0, f.ctypes.data))
+        return Workload(name, g, W, H, 1, {"float32": f}, None, [("float32", "max"), ("float32", "sum")],
+                        float("nan"), f"500000 random-walk lines + 2000000 points + 50000 collections on "
+                                      f"{W}x{H}, float32 max and sum, background NaN")
+    raise SystemExit(f"unknown workload {name}")
 
 
-def b_alg_tile(g: Geometry, esz: int) -> int:
-    """Algorithmic bytes of the tile kernel: the raster written once, every
-    polygon edge record (32 B) and field value read once."""
-    return esz * N_BANDS * W * H + 32 * g.n_vertices + esz * g.n_geoms
-
-
-def cpu_baseline(g, f32, lay, seconds_cap: float):
-    """The CPU restatement timed on a bounded sample: the top 5,000-row band
-    (one eighth of the grid, exactly the same computation for those rows)
-    for fun=last into int32, one thread per output band like the reference's
-    rayon loop."""
+def cpu_baseline(wl: Workload):
+    """The CPU restatement (oracle/rz_oracle.c) timed on a bounded sample of
+    the same workload, one thread per output band like the reference's rayon
+    loop over bands."""
     from oracle import oracle
 
-    info = oracle.make_info(W, 5000, 0.0, float(W), float(H - 5000), float(H), 1.0, 1.0)
-    threads = min(N_BANDS, os.cpu_count() or 1)
+    if wl.name == "polygons":
+        # the top 5,000-row band: the same computation for those rows
+        info = oracle.make_info(wl.W, 5000, 0.0, float(wl.W), float(wl.H - 5000), float(wl.H), 1.0, 1.0)
+        dt, fun, sample = "int32", "last", "rows 0-5000 of 40000 (8 bands x 5000 x 40000 int32), fun=last, all polygons"
+    else:
+        info = oracle.make_info(wl.W, wl.H, 0.0, float(wl.W), 0.0, float(wl.H), 1.0, 1.0)
+        dt, fun = wl.calls[0]
+        sample = f"the whole call ({dt}, fun={fun})"
+    threads = min(wl.bands, os.cpu_count() or 1)
     t0 = time.perf_counter()
-    _, burns = oracle.rasterize(g, info, "int32", "last", field=f32, layer_of_geom=lay, n_layers=N_BANDS,
-                                background=0, threads=threads, return_burns=True)
-    dt = time.perf_counter() - t0
-    return {
-        "value": burns / dt,
-        "unit": "pixel-burns/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"rows 0-5000 of 40000 (8 bands x 5000 x 40000 int32), fun=last, all 1M polygons; "
-                  f"{burns} burns in {dt:.2f} s incl. background fill; CPU {cpu_model()}, nproc={os.cpu_count()}",
-    }
+    _, burns = oracle.rasterize(wl.geom, info, dt, fun, field=wl.fields[dt], layer_of_geom=wl.layers,
+                                n_layers=wl.bands, background=wl.bg, threads=threads, return_burns=True)
+    secs = time.perf_counter() - t0
+    return {"value": burns / secs, "unit": "pixel-burns/s", "cores": threads, "kind": "port",
+            "sample": f"{sample}; {burns} burns in {secs:.2f} s incl. background fill; "
+                      f"CPU {cpu_model()}, nproc={os.cpu_count()}"}
 
+
+# --------------------------------------------------------------------- runs
 
 def copy_calibration(nbytes: int) -> int:
     """One device-to-device copy of nbytes (after a warm-up copy): reading
@@ -120,9 +164,10 @@ def copy_calibration(nbytes: int) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--polygons", type=int, default=N_POLY)
+    ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines"])
+    ap.add_argument("--polygons", type=int, default=1_000_000, help="polygon count of the default workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None,
                     help="DTYPE:FUN -- one warm-up and one timed call of that combination only "
@@ -137,11 +182,13 @@ def main() -> int:
         print("bench.py measures one GPU: the rasteriser shards by output rows with no exchange "
               "(see DESIGN.md); multi-GPU runs are not part of this build", file=sys.stderr)
         return 2
+    steps = args.steps if args.steps is not None else {"polygons": 3, "stars": 200, "lines": 5}[args.workload]
 
     L = _lib.lib()
     t0 = time.perf_counter()
-    g, f32, lay = make_inputs(args.polygons)
-    log(f"inputs: {g.n_geoms} geometries, {g.n_rings} rings, {g.n_vertices} vertices "
+    wl = make_workload(args.workload, args.polygons)
+    g = wl.geom
+    log(f"inputs: {g.n_geoms} geometries, {g.n_items} items, {g.n_rings} rings, {g.n_vertices} vertices "
         f"({time.perf_counter() - t0:.1f} s)")
 
     ctx = c_void_p()
@@ -150,29 +197,36 @@ def main() -> int:
     gs = g.as_struct()
     check(L.rz_device_geometry_upload(ctx, byref(gs), byref(dg)))
     d_field = {}
-    for dt in DTYPES:
-        host = f32.astype(dt)  # numpy cast: uint8 wraps
+    for dt, host in wl.fields.items():
         p = c_void_p()
         check(L.rz_device_malloc(ctx, host.nbytes, byref(p)))
         check(L.rz_memcpy_h2d(ctx, p, host.ctypes.data, host.nbytes))
         d_field[dt] = p
-    d_lay = c_void_p()
-    check(L.rz_device_malloc(ctx, lay.nbytes, byref(d_lay)))
-    check(L.rz_memcpy_h2d(ctx, d_lay, lay.ctypes.data, lay.nbytes))
-    out_bytes = 4 * N_BANDS * W * H
+    d_lay = None
+    if wl.layers is not None:
+        d_lay = c_void_p()
+        check(L.rz_device_malloc(ctx, wl.layers.nbytes, byref(d_lay)))
+        check(L.rz_memcpy_h2d(ctx, d_lay, wl.layers.ctypes.data, wl.layers.nbytes))
+    max_esz = max(np.dtype(dt).itemsize for dt, _ in wl.calls)
     d_out = c_void_p()
-    check(L.rz_device_malloc(ctx, out_bytes, byref(d_out)))
-    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, W, H))
-    bg = {dt: np.zeros(1, dtype=dt) for dt in DTYPES}
+    check(L.rz_device_malloc(ctx, max_esz * wl.bands * wl.W * wl.H, byref(d_out)))
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, wl.W, wl.H))
+    bg = {dt: np.array([wl.bg], dtype=dt) for dt in wl.fields}
 
     def call(dt, fun):
         check(L.rz_rasterize_device(ctx, dg, byref(info), _lib.dtype_code(dt), _lib.fun_code(fun), d_field[dt],
-                                    None, d_lay, N_BANDS, bg[dt].ctypes.data, 0, 0, d_out))
+                                    None, d_lay, wl.bands if d_lay is not None else 1, bg[dt].ctypes.data, 0, 0,
+                                    d_out))
 
     def step():
-        for dt in DTYPES:
-            for fun in FUNS:
-                call(dt, fun)
+        for dt, fun in wl.calls:
+            call(dt, fun)
+
+    times = (c_double * 8)()
+
+    def stage_times(div):
+        n = L.rz_context_stage_times(ctx, times, 8)
+        return {L.rz_stage_name(i).decode(): times[i] / div for i in range(n)}
 
     if args.only:
         dt, fun = args.only.split(":")
@@ -182,10 +236,8 @@ def main() -> int:
         L.rz_context_reset_stage_times(ctx)
         call(dt, fun)
         check(L.rz_context_synchronize(ctx))
-        times = (c_double * 8)()
-        n = L.rz_context_stage_times(ctx, times, 8)
-        print(json.dumps({"only": args.only, "stages_ms": {L.rz_stage_name(i).decode(): times[i] for i in range(n)},
-                          "b_alg_tile": b_alg_tile(g, np.dtype(dt).itemsize)}), flush=True)
+        print(json.dumps({"only": args.only, "stages_ms": stage_times(1),
+                          "b_alg_tile": wl.b_alg_tile(np.dtype(dt).itemsize)}), flush=True)
         return 0
 
     # HIP events around every stage of every call, recorded on the context's
@@ -196,76 +248,72 @@ def main() -> int:
     check(L.rz_context_synchronize(ctx))
     L.rz_context_reset_stage_times(ctx)
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         step()
-        log(f"step {k + 1}/{args.steps} queued")
+        if steps <= 10 or (k + 1) % max(1, steps // 5) == 0:
+            log(f"step {k + 1}/{steps} queued")
     check(L.rz_context_synchronize(ctx))
     elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    times = (c_double * 8)()
-    n = L.rz_context_stage_times(ctx, times, 8)
-    stages = {L.rz_stage_name(i).decode(): times[i] / args.steps for i in range(n)}  # ms per step
-    tile_ms = stages["tile"]  # summed over the 18 tile launches of a step
+    ms_per_step = 1e3 * elapsed / steps
+    stages = stage_times(steps)  # ms per step, summed over the step's calls
+    tile_ms = stages["tile"]
 
-    # per-dtype breakdown (untimed): 6 calls each
+    # per-dtype breakdown (untimed)
     by_dtype = {}
-    for dt in DTYPES:
+    for dt in sorted({dt for dt, _ in wl.calls}):
+        funs = [f for d, f in wl.calls if d == dt]
         L.rz_context_reset_stage_times(ctx)
-        for fun in FUNS:
+        for fun in funs:
             call(dt, fun)
-        n = L.rz_context_stage_times(ctx, times, 8)
-        st = {L.rz_stage_name(i).decode(): times[i] / len(FUNS) for i in range(n)}
-        byt = b_alg_tile(g, np.dtype(dt).itemsize)
+        st = stage_times(len(funs))
+        byt = wl.b_alg_tile(np.dtype(dt).itemsize)
         by_dtype[dt] = {"tile_ms": round(st["tile"], 4), "GB/s": round(byt / (st["tile"] * 1e-3) / 1e9, 1),
                         "frac": round(byt / (st["tile"] * 1e-3) / HBM_PEAK, 4),
                         "stages_ms": {k: round(v, 4) for k, v in st.items()}}
     # exact burn counts (profiling level 2 adds counters; untimed)
     check(L.rz_context_set_profiling(ctx, 2))
-    burns_call = {}
-    for dt in DTYPES:
-        call(dt, "max")
+    burns_per_step = 0
+    for dt, fun in wl.calls:
+        call(dt, fun)
         burns, pairs, tiles = c_uint64(), c_uint64(), c_uint64()
         check(L.rz_context_last_counters(ctx, byref(burns), byref(pairs), byref(tiles)))
-        burns_call[dt] = (burns.value, pairs.value, tiles.value)
-        by_dtype[dt]["pairs"] = pairs.value
-        by_dtype[dt]["tiles"] = tiles.value
+        burns_per_step += burns.value
+        by_dtype[dt]["pairs"], by_dtype[dt]["tiles"] = pairs.value, tiles.value
     check(L.rz_context_set_profiling(ctx, 0))
-    # every fun writes the same pixels, so a step burns 6x each dtype's count
-    burns_per_step = sum(len(FUNS) * burns_call[dt][0] for dt in DTYPES)
-    tile_bytes = len(FUNS) * sum(b_alg_tile(g, np.dtype(dt).itemsize) for dt in DTYPES)
-    call_bytes = len(FUNS) * sum(b_alg(g, np.dtype(dt).itemsize) for dt in DTYPES)
+    tile_bytes = sum(wl.b_alg_tile(np.dtype(dt).itemsize) for dt, _ in wl.calls)
+    call_bytes = sum(wl.b_alg(np.dtype(dt).itemsize) for dt, _ in wl.calls)
     call_ms = sum(stages.values())
     achieved = tile_bytes / (tile_ms * 1e-3)
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(g, f32, lay, 30.0)
+        cpu = cpu_baseline(wl)
 
-    for p in list(d_field.values()) + [d_lay, d_out]:
+    for p in list(d_field.values()) + ([d_lay] if d_lay is not None else []) + [d_out]:
         L.rz_device_free(ctx, p)
     L.rz_device_geometry_free(dg)
     L.rz_context_destroy(ctx)
 
+    dtypes = sorted({dt for dt, _ in wl.calls})
     result = {
-        "metric": "pixel-burns/s + HBM GB/s vs roofline; batched/served vs plain p50/p99",
+        "metric": METRIC,
         "value": burns_per_step / (ms_per_step * 1e-3),
         "unit": "pixel-burns/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 crossings; u8/i32/f32 folds",
+        "dtype": "f64 crossings; " + "/".join(dtypes) + " folds",
         "data": "synthetic (seeded SplitMix64 generator)",
         "config": {
-            "workload": f"{g.n_geoms} convex polygons (10% multi, 20% holed) on {W}x{H}, by=8 bands, "
-                        f"funs {'/'.join(FUNS)} x dtypes {'/'.join(DTYPES)} per step",
-            "polygons": g.n_geoms, "vertices": g.n_vertices, "grid": [H, W], "bands": N_BANDS,
-            "calls_per_step": len(FUNS) * len(DTYPES), "burns_per_step": burns_per_step,
-            "stages_ms_per_step": {k: round(v, 3) for k, v in stages.items()},
+            "workload": wl.description,
+            "geometries": g.n_geoms, "vertices": g.n_vertices, "grid": [wl.H, wl.W], "bands": wl.bands,
+            "calls_per_step": len(wl.calls), "burns_per_step": burns_per_step,
+            "stages_ms_per_step": {k: round(v, 4) for k, v in stages.items()},
         },
         "roofline": {
             "bound": "hbm",
@@ -274,7 +322,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
             "traffic": None,
-            "kernel": "k_tile (18 launches per step; bytes-weighted)",
+            "kernel": f"k_tile ({len(wl.calls)} launches per step; bytes-weighted)",
             "by_dtype": by_dtype,
             "call_GB/s": round(call_bytes / (call_ms * 1e-3) / 1e9, 1),
             "call_frac": round(call_bytes / (call_ms * 1e-3) / HBM_PEAK, 4),

@@ -52,9 +52,10 @@ typedef enum rz_fun {
 } rz_fun;
 
 /* How `sum` is reduced on the device.  ORDERED folds every pixel in
- * geometry order and is bit-identical to the reference.  UNORDERED lets the
- * tile lists keep arbitrary geometry order (no stable sort); float sums then
- * differ from the reference by rounding only (bound stated in the tests). */
+ * geometry order and is bit-identical to the reference.  UNORDERED permits
+ * any reduction order (float sums may then differ from the reference by
+ * rounding, within 1e-12 relative); this build serves it with the ordered
+ * fold, which costs the same here, so both modes are bit-exact. */
 typedef enum rz_sum_mode { RZ_SUM_ORDERED = 0, RZ_SUM_UNORDERED = 1 } rz_sum_mode;
 
 /* Item kinds of the flattened geometry model (see rz_geometry). */
@@ -158,14 +159,15 @@ void rz_geometry_free(rz_geometry_buffers* bufs);
 /* Seeded synthetic workloads (SplitMix64) used by tests and benchmarks.
  * kind: 1 = star polygons, 2 = convex polygons with parts/holes,
  *       3 = random-walk lines + points + collections, 4 = mixed job.
- * The four integer knobs are interpreted per kind (documented in
- * rusterize_amd/csrc/rz_synth.cpp). */
+ * The four knobs p0..p3 are interpreted per kind (documented at the
+ * generators in rusterize_amd/csrc/rz_host.cpp). */
 rz_status rz_synth_geometry(int kind, uint64_t seed, int64_t count, double grid_w, double grid_h,
                             double p0, double p1, double p2, double p3,
                             rz_geometry_buffers** out);
 /* Fill n values of `dtype` from a seeded distribution:
- * dist 0 = uniform [lo, hi) (ints: integer uniform, cast with numpy wrap),
- * dist 1 = standard normal rounded to the dtype. */
+ * dist 0 = integer-valued uniform [lo, hi) cast with numpy's wrap,
+ * dist 1 = standard normal rounded to the dtype,
+ * dist 2 = real uniform [lo, hi) rounded to the dtype. */
 rz_status rz_synth_field(rz_dtype dtype, int dist, uint64_t seed, int64_t n, double lo, double hi,
                          void* out);
 
@@ -224,6 +226,31 @@ rz_status rz_rasterize_device(rz_context* ctx, const rz_device_geometry* dg,
                               const uint32_t* layer_dev, uint32_t n_layers,
                               const void* background_host, int all_touched,
                               rz_sum_mode sum_mode, void* out_dev);
+
+/* ------------------------------------------------------------- batching -- */
+
+/* No reference counterpart (the reference rasterises one call at a time):
+ * one job of a batch, i.e. the arguments of one rz_rasterize_device call.
+ * Values are the raw little-endian bits of the job's dtype. */
+typedef struct rz_job {
+  const rz_device_geometry* geometry;
+  rz_raster_info info;
+  rz_dtype dtype;
+  rz_fun fun;
+  const void* field_dev;     /* [n_geoms] values of dtype, or NULL: scalar_bits */
+  uint64_t scalar_bits;
+  const uint32_t* layer_dev; /* [n_geoms] band of every geometry, or NULL: one band */
+  uint32_t n_layers;
+  uint64_t background_bits;
+  void* out_dev;             /* [n_layers][nrows][ncols] of dtype, disjoint per job */
+} rz_job;
+
+/* Rasterises n_jobs independent jobs (any mix of sizes, dtypes and pixel
+ * functions) as one pipeline: one edge/descriptor/binning pass over all of
+ * them and one persistent work-queue tile launch per (dtype, function)
+ * group.  Every output equals the job's own rz_rasterize_device output.
+ * Asynchronous on the context's stream. */
+rz_status rz_batch_rasterize(rz_context* ctx, const rz_job* jobs, int64_t n_jobs);
 
 /* Profiling level: 0 off; 1 records HIP events around every stage of every
  * rz_rasterize_device call on the context's stream (no added

@@ -22,8 +22,19 @@
  * f64::total_cmp, wrapping integer arithmetic (release profile), and no FP
  * contraction (build with -ffp-contract=off; see oracle/Makefile).
  *
- * Parity status of each part is stated in the tests that pin it
- * (tests/test_oracle_*.py).
+ * What pins each part (tests/test_oracle.py, tests/test_closed_forms.py):
+ *   raster info      golden rasters' grids, R test-boundary.R known answers
+ *   polygon path     golden standard_output_sum (polygon with a hole), exact
+ *                    rational point-in-polygon on random polygons, rectangle
+ *                    closed form, R known answers
+ *   standard lines   both golden rasters, brute-force Bresenham vs the
+ *                    literal loop (exhaustive small cases, random, 128-bit)
+ *   points           golden rasters, R and Rust known answers
+ *   pixel cache      golden standard_output_sum_custom_shape (xres != yres)
+ *   pixel functions  golden rasters (sum), R known answers, documented
+ *                    triplets of python/docs/python.md:106-136
+ *   burn_line_all_touched: parity unpinned (the reference tests it only
+ *                    against GDAL, which is absent; not on the device path).
  */
 #include <math.h>
 #include <pthread.h>

@@ -39,6 +39,15 @@ class RzRasterInfo(ctypes.Structure):
     ]
 
 
+class RzJob(ctypes.Structure):
+    """rz_job: one job of rz_batch_rasterize (device pointers, raw value bits)."""
+    _fields_ = [
+        ("geometry", c_void_p), ("info", RzRasterInfo), ("dtype", c_int), ("fun", c_int),
+        ("field_dev", c_void_p), ("scalar_bits", c_uint64), ("layer_dev", c_void_p), ("n_layers", c_uint32),
+        ("background_bits", c_uint64), ("out_dev", c_void_p),
+    ]
+
+
 class RusterizeNativeError(RuntimeError):
     """A failing call into librusterize_hip; `status` is the rz_status code."""
 
@@ -83,6 +92,7 @@ _SIGNATURES = [
     ("rz_device_geometry_count", c_int64, [c_void_p]),
     ("rz_rasterize_device", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
+    ("rz_batch_rasterize", c_int, [c_void_p, POINTER(RzJob), c_int64]),
     ("rz_context_set_profiling", c_int, [c_void_p, c_int]),
     ("rz_context_stage_times", c_int, [c_void_p, POINTER(c_double), c_int]),
     ("rz_context_reset_stage_times", None, [c_void_p]),

@@ -1,17 +1,20 @@
 // rz_kernels.h -- gfx950 kernels of the rasteriser.
 //
-// Pipeline of one rasterise call (all on one HIP stream):
+// Pipeline of one rasterise call or batch of calls (all on one HIP stream):
 //   k_prep        one thread per vertex: world->pixel transform, polygon edge
 //                 records (rust/src/geo/edges.rs:27-46, 90-110), line / point
 //                 pixel coordinates (edges.rs:79-88, 112-134) and a per-item
 //                 pixel bounding box (segmented wave reduction + atomics).
-//   k_bin_count / scan / k_bin_emit / stable radix sort / k_tile_offsets
+//   k_item_desc   one thread per item: its descriptor (field value, band)
+//                 and tile count.
+//   scan / k_bin_emit / stable radix sort / k_tile_offsets / k_gather_desc
 //                 sort-middle binning of items into 64-row tiles, keeping
 //                 geometry order inside every tile.
-//   k_tile        one workgroup per output tile: the background fill of
-//                 RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and
-//                 every burn of every item of the tile, folded in geometry
-//                 order in registers, then written to HBM exactly once.
+//   k_tile        persistent work-queue kernel over 16-row tile strips: the
+//                 background fill of RasterInfo::build_raster
+//                 (rust/src/geo/raster.rs:23-28) and every burn of every item
+//                 of the strip, folded in geometry order in registers, then
+//                 written to HBM exactly once.
 //
 // Exactness: all f64 arithmetic is compiled with -ffp-contract=off, divisions
 // and floor/ceil are IEEE exact, so every crossing and every pixel-centre
@@ -48,25 +51,70 @@ struct __align__(16) ItemBox {
   int32_t r0, r1, c0, c1;
 };
 
-struct PrepArgs {
+// One rasterise job of a batch (a single call is a batch of one).  Every
+// per-call array is indexed in the batch's global vertex / item / tile
+// spaces; a job owns the ranges [v_base, v_base + nv), [i_base, i_base + ni)
+// and [tile_base, tile_base + n_tiles).  The geometry pointers are the
+// job's resident geometry set, indexed locally.
+struct JobDev {
   const double* xy;
   const int32_t* vert_ring;
   const int64_t* ring_voff;
   const int32_t* ring_item;
   const int32_t* item_kind;
-  int64_t n_vertices;
+  const int32_t* item_geom;
+  const int64_t* item_vbeg;  // [ni + 1] local vertex range of every item
+  int64_t v_base, i_base, nv, ni;
   double xmin, ymax, xres, yres;
-  int32_t nrows, ncols;
-  PolyEdge* edges;        // [n_vertices]
-  double2* px;            // [n_vertices] pixel coordinates (lines / points)
-  uint8_t* vflags;        // [n_vertices]
-  ItemBox* item_box;      // [n_items]
-  unsigned long long* item_last;  // [n_items] last kept line segment + 1 (0: none)
+  int32_t nrows, ncols, ntx, nty;
+  int32_t esz, tile_w;       // element size of the output dtype, tile width in pixels
+  uint32_t tiles_per_layer, tile_base, n_tiles;
+  int32_t fun;               // pixel function (rz_fun)
+  int32_t dedup_lines;       // xres != yres (burn_geometry.rs:179, 202)
+  int32_t scan;              // 1: no bins, the tile kernel tests every item of the job
+  const void* field;         // [n_geoms] of the batch dtype, or null
+  uint64_t scalar_bits, bg_bits;
+  const uint32_t* layer_of_geom;  // may be null
+  void* out;                 // [layers][nrows][ncols]
+};
+
+// Read-only tables of a launch (job table, descriptors, tile offsets) are
+// read through the constant address space: they do not change while a
+// kernel runs, so the compiler may use scalar loads for uniform indices even
+// though the kernel also stores to global memory.
+#define RZ_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ inline const RZ_CONST T* as_const(const T* p) {
+  return (const RZ_CONST T*)p;
+}
+
+// Index of the job whose exclusive prefix end first exceeds x.
+template <typename P>
+__device__ inline int find_job(P ends, int n, int64_t x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ends[mid] > x) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+struct PrepArgs {
+  const JobDev* jobs;
+  const int64_t* job_vend;  // [n_jobs] prefix ends of the vertex ranges
+  int n_jobs;
+  int64_t n_vertices;       // batch total
+  PolyEdge* edges;          // [n_vertices]
+  double2* px;              // [n_vertices] pixel coordinates (lines / points)
+  uint8_t* vflags;          // [n_vertices]
+  ItemBox* item_box;        // [n_items]
+  unsigned long long* item_last;  // [n_items] last kept line segment + 1 (global; 0: none)
 };
 
 // Per-call item descriptor (64 B, one scalar load): everything the tile
 // kernel needs about an item, with the owning geometry's field value and
-// output band already resolved.
+// output band already resolved.  vbeg / last are global vertex indices.
 struct __align__(64) ItemDesc {
   int32_t r0, r1, c0, c1;   // pixel box: rows [r0, r1), cols [c0, c1)
   int64_t vbeg;             // first vertex (edge / segment / point index)
@@ -74,51 +122,52 @@ struct __align__(64) ItemDesc {
   uint32_t kind_layer;      // item kind | output band << 2
   uint64_t value;           // field value bits (dtype of the call)
   uint64_t last;            // lines: last kept segment + 1 (0: none)
-  uint64_t pad[2];
+  uint32_t job, pad0;
+  uint64_t pad1;
 };
 
 struct DescArgs {
-  const ItemBox* item_box;
-  const int32_t* item_kind;
-  const int32_t* item_geom;
-  const int64_t* item_vbeg;
-  const unsigned long long* item_last;
-  const uint32_t* layer_of_geom;  // may be null
-  const void* field;              // [n_geoms] of the call's dtype, or null
-  uint64_t scalar_bits;
-  int32_t esz;
+  const JobDev* jobs;
+  const int64_t* job_iend;  // [n_jobs] prefix ends of the item ranges
+  int n_jobs;
   int64_t n_items;
-  ItemDesc* desc;
-  // binning (counts == nullptr in scan mode)
-  int32_t ntx, nty, tile_w;
-  uint32_t* counts;  // [n_items]
+  const ItemBox* item_box;
+  const unsigned long long* item_last;
+  ItemDesc* desc;           // [n_items]
+  uint32_t* counts;         // [n_items] tiles per item, or null (no binned job)
 };
 
 struct BinArgs {
+  const JobDev* jobs;
   const ItemDesc* desc;
   int64_t n_items;
-  int32_t ntx, nty, tile_w;
-  uint32_t tiles_per_layer;
   const uint64_t* offsets;  // [n_items + 1] exclusive prefix of the tile counts
-  uint32_t* keys;           // [n_pairs]
-  uint32_t* vals;           // [n_pairs]
+  uint64_t cap;             // capacity of keys / vals
+  uint32_t* keys;           // [cap]
+  uint32_t* vals;           // [cap]
+  uint32_t* status;         // bit 2 set when the pairs exceed cap
 };
 
+// Work queue of the tile kernel: one counter per (XCD, strip) pair, each
+// counting the tile blocks handed out from that XCD's eighth of the launch.
+constexpr int kQueueStride = 32;  // u32 words between counters (one 128-B line each)
+constexpr int kQueueWords = NUM_XCD * 4 * kQueueStride;
+
 struct TileArgs {
-  int32_t nrows, ncols, ntx, nty;
-  uint32_t tiles_per_layer, n_tiles;
-  // binned mode: pairs sorted by tile, tile_off[t]..tile_off[t+1]
-  const uint32_t* tile_off;
-  const ItemDesc* pair_desc;  // descriptors in pair order (binned mode)
-  // scan mode (pair_item == nullptr): every item is tested against the tile
-  int64_t n_items;
+  const JobDev* jobs;
+  const int64_t* job_tend;  // [n_jobs] prefix ends of the tile ranges
+  int n_jobs;
+  uint32_t t_lo, t_hi;      // tiles of this launch (one pixel-function group)
+  uint32_t chunk;           // tiles per queue grab (>= 1)
+  uint32_t* queue;          // [kQueueWords], zero at launch
+  // binned jobs: pairs sorted by tile, tile_off[t]..tile_off[t+1]
+  const uint32_t* tile_off;  // [n_tiles + 1], or null when no job is binned
+  const ItemDesc* pair_desc;  // descriptors in pair order
+  // scan jobs: every item of the job is tested against the tile
   const ItemDesc* desc;
   const PolyEdge* edges;
   const double2* px;
   const uint8_t* vflags;
-  uint64_t bg_bits;
-  int dedup_lines;          // xres != yres (burn_geometry.rs:179, 202)
-  void* out;                // [layers][nrows][ncols] of T
   unsigned long long* burns;  // [kBurnSlots] counters, or null
   uint32_t* status;         // error flags
 };

@@ -1,5 +1,6 @@
-// rz_kernels.hip -- gfx950 kernels: device-side edge construction, binning
-// and the tile scanline / fold kernel.  See rz_kernels.h for the pipeline.
+// rz_kernels.hip -- gfx950 kernels: device-side edge construction, item
+// descriptors and binning.  The tile kernel is in rz_tile.hip; the pipeline
+// is described in rz_kernels.h.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <math.h>
@@ -24,6 +25,7 @@ __device__ inline int64_t sat_isize(double v) {
   if (v >= 9223372036854775808.0) return INT64_MAX;
   return (int64_t)v;
 }
+
 // PolyEdge::intersect_at (rust/src/geo/edges.rs:50-55): x0 + ((r+0.5) - y0) * dxdy.
 // A NaN produced by an invalid operation (0 * inf, inf - inf) is given the
 // sign x86 SSE gives its default NaN (negative), so total_cmp places it like
@@ -68,17 +70,31 @@ __global__ void k_init_items(ItemBox* box, unsigned long long* last, int64_t n_i
   last[i] = 0ull;
 }
 
+// Clears n words (a kernel rather than a memset node, so captured batches
+// replay it reliably).
+__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// Fills keys[n .. cap) with a key above every tile (graph-captured batches
+// sort a fixed capacity; the filler sorts to the end and is never read).
+__global__ void k_fill_keys(uint32_t* keys, const uint64_t* n_pairs, uint64_t cap) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap && i >= *n_pairs) keys[i] = 0xFFFFFFFFu;
+}
+
 // ------------------------------------------------------------------ prep --
 
 // Segmented (by item) min/max reduction across the wave, then one set of
 // atomics per item segment.  Items own contiguous vertex ranges, so lanes of
 // one item are contiguous.
-__device__ inline void reduce_box(int32_t item, ItemBox b, ItemBox* out, bool active) {
+__device__ inline void reduce_box(int64_t item, ItemBox b, ItemBox* out, bool active) {
   const int lane = threadIdx.x & 63;
-  int32_t it = active ? item : -1;
+  int64_t it = active ? item : -1;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    int32_t oit = __shfl_up(it, off, 64);
+    int64_t oit = __shfl_up(it, off, 64);
     int32_t r0 = __shfl_up(b.r0, off, 64), r1 = __shfl_up(b.r1, off, 64);
     int32_t c0 = __shfl_up(b.c0, off, 64), c1 = __shfl_up(b.c1, off, 64);
     if (lane >= off && oit == it) {
@@ -88,7 +104,7 @@ __device__ inline void reduce_box(int32_t item, ItemBox b, ItemBox* out, bool ac
       b.c1 = max(b.c1, c1);
     }
   }
-  int32_t nit = __shfl_down(it, 1, 64);
+  int64_t nit = __shfl_down(it, 1, 64);
   bool tail = (lane == 63) || (nit != it);
   if (active && tail && b.r0 < b.r1 && b.c0 < b.c1) {
     atomicMin(&out[item].r0, b.r0);
@@ -98,34 +114,40 @@ __device__ inline void reduce_box(int32_t item, ItemBox b, ItemBox* out, bool ac
   }
 }
 
+// One thread per vertex of the batch: world -> pixel (edges.rs:94-97),
+// polygon edge records (extract_ring + PolyEdge::new, edges.rs:27-46,
+// 90-110), line / point pixel coordinates (extract_line / extract_point,
+// edges.rs:79-88, 112-134) and each item's pixel bounding box.
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = v < a.n_vertices;
-  int32_t item = -1;
+  int64_t item = -1;
   ItemBox box{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
   if (active) {
-    const int32_t ring = a.vert_ring[v];
-    item = a.ring_item[ring];
-    const int32_t kind = a.item_kind[item];
-    const int64_t rbeg = a.ring_voff[ring], rend = a.ring_voff[ring + 1];
-    const bool has_next = v + 1 < rend;
-    const double rows = (double)a.nrows, cols = (double)a.ncols;
-    // world -> pixel, rust/src/geo/edges.rs:94-97
-    const double x0 = (a.xy[2 * v] - a.xmin) / a.xres;
-    const double y0 = (a.ymax - a.xy[2 * v + 1]) / a.yres;
+    const JobDev& J = a.jobs[find_job(a.job_vend, a.n_jobs, v)];
+    const int64_t lv = v - J.v_base;
+    const int32_t ring = J.vert_ring[lv];
+    const int32_t litem = J.ring_item[ring];
+    item = J.i_base + litem;
+    const int32_t kind = J.item_kind[litem];
+    const int64_t rbeg = J.ring_voff[ring], rend = J.ring_voff[ring + 1];
+    const bool has_next = lv + 1 < rend;
+    const double rows = (double)J.nrows, cols = (double)J.ncols;
+    const double x0 = (J.xy[2 * lv] - J.xmin) / J.xres;
+    const double y0 = (J.ymax - J.xy[2 * lv + 1]) / J.yres;
     if (kind == 0) {  // polygon rings: extract_ring + PolyEdge::new
       PolyEdge e{0.0, 0.0, 0.0, 0, 0};
       if (has_next) {
-        const double x1 = (a.xy[2 * v + 2] - a.xmin) / a.xres;
-        const double y1 = (a.ymax - a.xy[2 * v + 3]) / a.yres;
+        const double x1 = (J.xy[2 * lv + 2] - J.xmin) / J.xres;
+        const double y1 = (J.ymax - J.xy[2 * lv + 3]) / J.yres;
         if (fabs(y0 - y1) >= 2.220446049250313e-16) {  // edges.rs:100
           const double min_y = fmin(y0, y1), max_y = fmax(y0, y1);
           if (min_y < rows && max_y >= 0.0) {  // edges.rs:105
             double xt, yt, xb, yb;
             if (y0 < y1) { xt = x0; yt = y0; xb = x1; yb = y1; }
             else { xt = x1; yt = y1; xb = x0; yb = y0; }
-            e.ystart = sat_usize_cap(ceil(yt - 0.5), a.nrows);
-            e.yend = sat_usize_cap(ceil(yb - 0.5), a.nrows);
+            e.ystart = sat_usize_cap(ceil(yt - 0.5), J.nrows);
+            e.yend = sat_usize_cap(ceil(yb - 0.5), J.nrows);
             e.dxdy = (xb - xt) / (yb - yt);
             e.x0 = xt;
             e.y0 = yt;
@@ -137,7 +159,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
               const double ax = fabs(xt), bx = fabs(xb);
               if (isnan(xt) || isnan(xb) || ax > 1.0e12 || bx > 1.0e12) {
                 box.c0 = 0;
-                box.c1 = a.ncols;
+                box.c1 = J.ncols;
               } else {
                 const double lo = floor(fmin(xt, xb) + 0.5) - 1.0;
                 const double hi = floor(fmax(xt, xb) + 0.5) + 1.0;
@@ -153,12 +175,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       a.px[v] = make_double2(x0, y0);
       uint8_t f = has_next ? VF_HAS_NEXT : 0;
       // geo-types LineString::is_closed: first coordinate == last coordinate
-      if (a.xy[2 * rbeg] == a.xy[2 * (rend - 1)] && a.xy[2 * rbeg + 1] == a.xy[2 * (rend - 1) + 1])
+      if (J.xy[2 * rbeg] == J.xy[2 * (rend - 1)] && J.xy[2 * rbeg + 1] == J.xy[2 * (rend - 1) + 1])
         f |= VF_CLOSED;
       a.vflags[v] = f;
       if (has_next) {
-        const double x1 = (a.xy[2 * v + 2] - a.xmin) / a.xres;
-        const double y1 = (a.ymax - a.xy[2 * v + 3]) / a.yres;
+        const double x1 = (J.xy[2 * lv + 2] - J.xmin) / J.xres;
+        const double y1 = (J.ymax - J.xy[2 * lv + 3]) / J.yres;
         const double mnx = fmin(x0, x1), mxx = fmax(x0, x1);
         const double mny = fmin(y0, y1), mxy = fmax(y0, y1);
         if (mnx < cols && mxx >= 0.0 && mny < rows && mxy >= 0.0) {  // edges.rs:130
@@ -166,10 +188,10 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
           // Bresenham visits cells between the floored end points only.
           const int64_t fx0 = sat_isize(floor(mnx)), fx1 = sat_isize(floor(mxx));
           const int64_t fy0 = sat_isize(floor(mny)), fy1 = sat_isize(floor(mxy));
-          box.c0 = (int32_t)max((int64_t)0, min(fx0, (int64_t)a.ncols));
-          box.c1 = (int32_t)max((int64_t)0, min(fx1 + 1, (int64_t)a.ncols));
-          box.r0 = (int32_t)max((int64_t)0, min(fy0, (int64_t)a.nrows));
-          box.r1 = (int32_t)max((int64_t)0, min(fy1 + 1, (int64_t)a.nrows));
+          box.c0 = (int32_t)max((int64_t)0, min(fx0, (int64_t)J.ncols));
+          box.c1 = (int32_t)max((int64_t)0, min(fx1 + 1, (int64_t)J.ncols));
+          box.r0 = (int32_t)max((int64_t)0, min(fy0, (int64_t)J.nrows));
+          box.r1 = (int32_t)max((int64_t)0, min(fy1 + 1, (int64_t)J.nrows));
         }
       }
     } else {  // points: extract_point
@@ -198,35 +220,40 @@ __device__ inline bool tile_rect(int32_t r0, int32_t r1, int32_t c0, int32_t c1,
 
 // One thread per item: resolve the field value and band of the owning
 // geometry (rust/src/rasterize.rs:169-179) into the item's descriptor and,
-// for the binned path, count the tiles its box covers.
+// for binned jobs, count the tiles its box covers.
 __global__ void k_item_desc(DescArgs a) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_items) return;
+  const int j = find_job(a.job_iend, a.n_jobs, i);
+  const JobDev& J = a.jobs[j];
+  const int64_t li = i - J.i_base;
   const ItemBox b = a.item_box[i];
-  const int32_t g = a.item_geom[i];
+  const int32_t g = J.item_geom[li];
   ItemDesc d;
   d.r0 = b.r0;
   d.r1 = b.r1;
   d.c0 = b.c0;
   d.c1 = b.c1;
-  d.vbeg = a.item_vbeg[i];
-  d.nv = (int32_t)(a.item_vbeg[i + 1] - d.vbeg);
-  const uint32_t layer = a.layer_of_geom ? a.layer_of_geom[g] : 0u;
-  d.kind_layer = (uint32_t)a.item_kind[i] | (layer << 2);
-  uint64_t v = a.scalar_bits;
-  if (a.field) {
-    const uint8_t* f = (const uint8_t*)a.field + (size_t)g * (size_t)a.esz;
+  d.vbeg = J.v_base + J.item_vbeg[li];
+  d.nv = (int32_t)(J.item_vbeg[li + 1] - J.item_vbeg[li]);
+  const uint32_t layer = J.layer_of_geom ? J.layer_of_geom[g] : 0u;
+  d.kind_layer = (uint32_t)J.item_kind[li] | (layer << 2);
+  uint64_t v = J.scalar_bits;
+  if (J.field) {
+    const uint8_t* f = (const uint8_t*)J.field + (size_t)g * (size_t)J.esz;
     v = 0;
-    for (int k = 0; k < a.esz; ++k) v |= (uint64_t)f[k] << (8 * k);
+    for (int k = 0; k < J.esz; ++k) v |= (uint64_t)f[k] << (8 * k);
   }
   d.value = v;
   d.last = a.item_last[i];
-  d.pad[0] = d.pad[1] = 0;
+  d.job = (uint32_t)j;
+  d.pad0 = 0;
+  d.pad1 = 0;
   a.desc[i] = d;
   if (a.counts) {
     int32_t tx0, tx1, ty0, ty1;
     uint32_t c = 0;
-    if (tile_rect(d.r0, d.r1, d.c0, d.c1, a.tile_w, tx0, tx1, ty0, ty1))
+    if (!J.scan && tile_rect(d.r0, d.r1, d.c0, d.c1, J.tile_w, tx0, tx1, ty0, ty1))
       c = (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1);
     a.counts[i] = c;
   }
@@ -240,32 +267,46 @@ __global__ void k_bin_emit(BinArgs a) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_items) return;
   const ItemDesc d = a.desc[i];
+  const JobDev& J = a.jobs[d.job];
+  if (J.scan) return;
   int32_t tx0, tx1, ty0, ty1;
-  if (!tile_rect(d.r0, d.r1, d.c0, d.c1, a.tile_w, tx0, tx1, ty0, ty1)) return;
+  if (!tile_rect(d.r0, d.r1, d.c0, d.c1, J.tile_w, tx0, tx1, ty0, ty1)) return;
   uint64_t o = a.offsets[i];
-  const uint32_t base = (d.kind_layer >> 2) * a.tiles_per_layer;
+  if (a.offsets[i + 1] > a.cap) {  // capacity of a captured batch exceeded
+    atomicOr(a.status, 4u);
+    return;
+  }
+  const uint32_t base = J.tile_base + (d.kind_layer >> 2) * J.tiles_per_layer;
   for (int32_t ty = ty0; ty <= ty1; ++ty)
     for (int32_t tx = tx0; tx <= tx1; ++tx) {
-      a.keys[o] = base + (uint32_t)ty * (uint32_t)a.ntx + (uint32_t)tx;
+      a.keys[o] = base + (uint32_t)ty * (uint32_t)J.ntx + (uint32_t)tx;
       a.vals[o] = (uint32_t)i;
       ++o;
     }
 }
 
 // Descriptors in pair order, so a tile's items are one contiguous run.
-__global__ void k_gather_desc(const ItemDesc* desc, const uint32_t* pair_item, uint32_t n_pairs,
-                              ItemDesc* pair_desc) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_pairs) return;
-  pair_desc[i] = desc[pair_item[i]];
+// n_pairs is the device total; cap bounds it (a captured batch sorts a
+// fixed capacity).
+__global__ void k_gather_desc(const ItemDesc* desc, int64_t n_items, const uint32_t* pair_item,
+                              const uint64_t* n_pairs, uint64_t cap, ItemDesc* pair_desc) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= min(*n_pairs, cap)) return;
+  const uint32_t it = pair_item[i];
+  if (it < (uint64_t)n_items) pair_desc[i] = desc[it];
 }
 
-// tile_off[t] = first pair index whose key >= t, for t in [0, n_tiles].
-__global__ void k_tile_offsets(const uint32_t* keys, uint32_t n_pairs, uint32_t n_tiles,
-                               uint32_t* tile_off) {
+// tile_off[t] = first pair index whose key >= t, for t in [0, n_tiles]; all
+// lists are empty when the pairs overflowed their capacity (status bit 2).
+__global__ void k_tile_offsets(const uint32_t* keys, const uint64_t* n_pairs, uint64_t cap,
+                               const uint32_t* status, uint32_t n_tiles, uint32_t* tile_off) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t > n_tiles) return;
-  uint32_t lo = 0, hi = n_pairs;
+  if (*status & 4u) {
+    tile_off[t] = 0;
+    return;
+  }
+  uint32_t lo = 0, hi = (uint32_t)min(*n_pairs, cap);
   while (lo < hi) {
     uint32_t mid = lo + (hi - lo) / 2;
     if (keys[mid] < (uint32_t)t) lo = mid + 1;
@@ -338,4 +379,5 @@ template <typename T> __device__ inline T from_bits(uint64_t b) {
   __builtin_memcpy(&v, &b, sizeof(T));
   return v;
 }
+
 }  // namespace rzk

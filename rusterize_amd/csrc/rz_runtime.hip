@@ -69,8 +69,18 @@ struct rz_context {
   hipStream_t stream = nullptr;
   // workspace
   Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
-      vals_b, tile_off, temp, misc;
+      vals_b, tile_off, temp, misc, jobtab, queue;
   uint64_t* host_misc = nullptr;  // pinned: [0] total pairs, [1] status
+  // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
+  // the end of its last upload); tab_dev / tab_dev_bytes say which device
+  // buffer holds the staged bytes
+  std::vector<unsigned char> tab_build;
+  void* host_tab = nullptr;
+  size_t host_tab_cap = 0;
+  hipEvent_t tab_ev = nullptr;
+  bool tab_pending = false;
+  const void* tab_dev = nullptr;
+  size_t tab_dev_bytes = 0;
   // profiling: 0 off, 1 stage events, 2 stage events + exact burn counts
   int profiling = 0;
   hipEvent_t ev[kStages + 1] = {};  // the current call's events (taken from the pool)
@@ -80,8 +90,8 @@ struct rz_context {
   uint64_t last_pairs = 0, last_tiles = 0;
   bool burns_pending = false;
   std::vector<Buf*> all() {
-    return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts, &offsets,
-            &keys_a, &vals_a, &keys_b, &vals_b,   &tile_off,  &temp, &misc, &pair_desc};
+    return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
+            &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -154,6 +164,7 @@ extern "C" rz_status rz_context_create(int device, rz_context** out) {
   c->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipHostMalloc((void**)&c->host_misc, 4 * sizeof(uint64_t), hipHostMallocDefault));
+  HIP_TRY(hipEventCreateWithFlags(&c->tab_ev, hipEventDisableTiming));
   *out = c;
   return RZ_OK;
 }
@@ -169,6 +180,8 @@ extern "C" rz_status rz_context_destroy(rz_context* c) {
     for (auto& e : s.ev)
       if (e) (void)hipEventDestroy(e);
   if (c->host_misc) (void)hipHostFree(c->host_misc);
+  if (c->host_tab) (void)hipHostFree(c->host_tab);
+  if (c->tab_ev) (void)hipEventDestroy(c->tab_ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return RZ_OK;
@@ -187,6 +200,9 @@ static rz_status check_status_word(rz_context* c) {
   if (flags & 1u)
     return rz::fail(RZ_CAPACITY_ERROR,
                     "line coordinates beyond +-2^60 pixels are outside the supported range");
+  if (flags & 4u)
+    return rz::fail(RZ_CAPACITY_ERROR,
+                    "tile/item pairs exceeded the capacity of the captured batch; capture it again");
   return RZ_OK;
 }
 
@@ -353,9 +369,10 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
 namespace {
 
 // Persistent grid for one k_tile instantiation: enough workgroups to fill
-// every CU at the kernel's occupancy, a multiple of the 8 XCDs.
+// every CU at the kernel's occupancy (no more than the units need), a
+// multiple of the 8 XCDs.
 template <typename T, int FUN, bool LINES>
-uint32_t tile_grid(uint32_t n_tiles) {
+uint32_t tile_grid(uint64_t n_tiles) {
   static uint32_t cached = 0;
   if (!cached) {
     int dev = 0, cus = 256, per_cu = 1;
@@ -367,18 +384,31 @@ uint32_t tile_grid(uint32_t n_tiles) {
       per_cu = 1;
     cached = (uint32_t)(cus * per_cu);
   }
-  uint32_t g = std::min<uint32_t>(cached, std::max<uint32_t>(n_tiles, 1));
+  const uint32_t g = (uint32_t)std::min<uint64_t>(cached, std::max<uint64_t>(n_tiles, 1));
   return std::max<uint32_t>(NUM_XCD, (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
 }
 
+// Tiles per queue grab: 4 once every wave has many tiles (measured best of
+// 1, 2, 4 on the 40,000^2 workload), else 1.  RUSTERIZE_CHUNK overrides it.
+inline uint32_t queue_chunk(uint64_t n_tiles, uint32_t grid) {
+  if (const char* e = std::getenv("RUSTERIZE_CHUNK")) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 4096) return (uint32_t)v;
+  }
+  return n_tiles >= (uint64_t)grid * 16 ? 4u : 1u;
+}
+
 template <typename T, int FUN>
-void launch_tile(const TileArgs& a, bool lines, hipStream_t s) {
+void launch_tile(TileArgs a, bool lines, hipStream_t s) {
+  const uint64_t tiles = (uint64_t)(a.t_hi - a.t_lo);
   if (lines) {
-    hipLaunchKernelGGL((k_tile<T, FUN, true>), dim3(tile_grid<T, FUN, true>(a.n_tiles)), dim3(256),
-                       0, s, a);
+    const uint32_t g = tile_grid<T, FUN, true>(tiles);
+    a.chunk = queue_chunk(tiles, g);
+    hipLaunchKernelGGL((k_tile<T, FUN, true>), dim3(g), dim3(256), 0, s, a);
   } else {
-    hipLaunchKernelGGL((k_tile<T, FUN, false>), dim3(tile_grid<T, FUN, false>(a.n_tiles)),
-                       dim3(256), 0, s, a);
+    const uint32_t g = tile_grid<T, FUN, false>(tiles);
+    a.chunk = queue_chunk(tiles, g);
+    hipLaunchKernelGGL((k_tile<T, FUN, false>), dim3(g), dim3(256), 0, s, a);
   }
 }
 
@@ -410,84 +440,204 @@ void launch_tile_dispatch(rz_dtype dt, int fun, const TileArgs& a, bool lines, h
   }
 }
 
+void zero_words(void* p, uint64_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for((int64_t)n)), dim3(256), 0, s, (uint32_t*)p, n);
+}
+
+// A tile launch: jobs [j0, j1) of one (dtype, function, lines) group.
+struct Group {
+  rz_dtype dtype;
+  int fun;
+  bool lines;
+  uint32_t t_lo, t_hi;
+};
+
+rz_status check_job(const rz_job& j) {
+  if (!j.geometry || !j.out_dev) return rz::fail(RZ_VALUE_ERROR, "NULL geometry or output of a job");
+  if ((unsigned)j.dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
+  if ((unsigned)j.fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
+  const uint32_t nl = j.layer_dev ? j.n_layers : 1u;
+  if (nl == 0 || nl >= (1u << 29)) return rz::fail(RZ_VALUE_ERROR, "n_layers must be in [1, 2^29)");
+  if (j.info.nrows >= (uint64_t)INT32_MAX - TILE_H || j.info.ncols >= (uint64_t)INT32_MAX - 256)
+    return rz::fail(RZ_CAPACITY_ERROR, "raster dimensions exceed 2^31 rows or columns");
+  return RZ_OK;
+}
+
 }  // namespace
 
-extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry* dg,
-                                         const rz_raster_info* info, rz_dtype dtype, rz_fun fun,
-                                         const void* field_dev, const void* scalar_host,
-                                         const uint32_t* layer_dev, uint32_t n_layers,
-                                         const void* bg_host, int all_touched, rz_sum_mode sum_mode,
-                                         void* out_dev) {
-  if (!c || !dg || !info || !bg_host || !out_dev)
-    return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_rasterize_device");
-  if ((unsigned)dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
-  if ((unsigned)fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
-  if (!field_dev && !scalar_host) return rz::fail(RZ_VALUE_ERROR, "no field and no scalar value");
-  if (all_touched)
-    return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not implemented on the device path yet");
-  (void)sum_mode;
-  if (!layer_dev) n_layers = 1;
-  if (n_layers == 0 || n_layers >= (1u << 29)) return rz::fail(RZ_VALUE_ERROR, "n_layers must be in [1, 2^29)");
-  if (info->nrows == 0 || info->ncols == 0) return RZ_OK;  // empty raster: nothing to write
-  if (info->nrows >= (uint64_t)INT32_MAX - TILE_H || info->ncols >= (uint64_t)INT32_MAX - 256)
-    return rz::fail(RZ_CAPACITY_ERROR, "raster dimensions exceed 2^31 rows or columns");
+// The pipeline of a batch.  Jobs are ordered by (dtype, function, lines) so
+// every group's tiles form one contiguous range; vertex, item and tile
+// indices are global over the batch.
+static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
+  if (n_in <= 0) return RZ_OK;
+  if (n_in >= (int64_t)1 << 24) return rz::fail(RZ_CAPACITY_ERROR, "more than 2^24 jobs in one batch");
+  for (int64_t k = 0; k < n_in; ++k) {
+    rz_status st = check_job(jobs_in[k]);
+    if (st) return st;
+  }
   HIP_TRY(hipSetDevice(c->device));
-  const size_t esz = rz_dtype_size(dtype);
-  const int TW = tile_w((int)esz);
-  const int32_t nrows = (int32_t)info->nrows, ncols = (int32_t)info->ncols;
-  const int32_t ntx = (ncols + TW - 1) / TW, nty = (nrows + TILE_H - 1) / TILE_H;
-  const uint64_t tpl = (uint64_t)ntx * (uint64_t)nty;
-  const uint64_t n_tiles64 = tpl * n_layers;
-  if (n_tiles64 >= (uint64_t)UINT32_MAX)
-    return rz::fail(RZ_CAPACITY_ERROR, "too many output tiles (> 2^32)");
-  const uint32_t n_tiles = (uint32_t)n_tiles64;
-  const int64_t I = dg->I, V = dg->V;
   hipStream_t s = c->stream;
+  // RUSTERIZE_BINNING=always|never overrides the size heuristic (tests use it
+  // to run both item-list forms on the same inputs).
+  int binning = 0;
+  if (const char* bm = std::getenv("RUSTERIZE_BINNING")) {
+    if (std::strcmp(bm, "always") == 0) binning = 1;
+    else if (std::strcmp(bm, "never") == 0) binning = -1;
+  }
+  auto has_lines = [](const rz_job& j) { return j.geometry->has_line || j.geometry->has_point; };
+  std::vector<int64_t> order((size_t)n_in);
+  for (int64_t k = 0; k < n_in; ++k) order[(size_t)k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    const rz_job &x = jobs_in[a], &y = jobs_in[b];
+    if (x.dtype != y.dtype) return x.dtype < y.dtype;
+    if (x.fun != y.fun) return x.fun < y.fun;
+    return has_lines(x) < has_lines(y);
+  });
+
+  // job table: JobDev[n] then the vertex / item / tile prefix ends (int64 x n each)
+  const size_t n = (size_t)n_in;
+  const size_t tab_bytes = n * sizeof(JobDev) + 3 * n * sizeof(int64_t);
+  c->tab_build.assign(tab_bytes, 0);
+  JobDev* tab = (JobDev*)c->tab_build.data();
+  int64_t* vend = (int64_t*)(tab + n);
+  int64_t* iend = vend + n;
+  int64_t* tend = iend + n;
+  std::vector<Group> groups;
+  int64_t V = 0, I = 0;
+  uint64_t T = 0;
+  bool any_binned = false, any_poly = false, any_lp = false;
+  for (size_t k = 0; k < n; ++k) {
+    const rz_job& j = jobs_in[order[k]];
+    const rz_device_geometry* dg = j.geometry;
+    const int esz = (int)rz_dtype_size(j.dtype);
+    const int TW = tile_w(esz);
+    const uint32_t nl = j.layer_dev ? j.n_layers : 1u;
+    const int32_t nrows = (int32_t)j.info.nrows, ncols = (int32_t)j.info.ncols;
+    const int32_t ntx = (ncols + TW - 1) / TW, nty = (nrows + TILE_H - 1) / TILE_H;
+    const uint64_t tpl = (uint64_t)ntx * (uint64_t)nty, jt = tpl * nl;
+    JobDev& J = tab[k];
+    std::memset(&J, 0, sizeof(J));
+    J.xy = dg->xy;
+    J.vert_ring = dg->vert_ring;
+    J.ring_voff = dg->ring_voff;
+    J.ring_item = dg->ring_item;
+    J.item_kind = dg->item_kind;
+    J.item_geom = dg->item_geom;
+    J.item_vbeg = dg->item_vbeg;
+    J.v_base = V;
+    J.i_base = I;
+    J.nv = dg->V;
+    J.ni = dg->I;
+    J.xmin = j.info.xmin;
+    J.ymax = j.info.ymax;
+    J.xres = j.info.xres;
+    J.yres = j.info.yres;
+    J.nrows = nrows;
+    J.ncols = ncols;
+    J.ntx = ntx;
+    J.nty = nty;
+    J.esz = esz;
+    J.tile_w = TW;
+    J.tiles_per_layer = (uint32_t)std::min<uint64_t>(tpl, UINT32_MAX);
+    J.tile_base = (uint32_t)std::min<uint64_t>(T, UINT32_MAX);
+    J.n_tiles = (uint32_t)std::min<uint64_t>(jt, UINT32_MAX);
+    J.fun = (int32_t)j.fun;
+    J.dedup_lines = j.info.xres != j.info.yres;
+    const uint64_t ni = (uint64_t)dg->I;
+    // scan: every strip tests every item box; otherwise the items are binned
+    // into per-tile lists.  Binning a lone call costs a host round trip and
+    // about eight small launches, so a single call scans up to ~4M
+    // item-tile tests; in a batch that cost is shared by every job.
+    const uint64_t scan_limit = n_in == 1 ? ((uint64_t)1 << 22) : ((uint64_t)1 << 16);
+    bool scan = ni <= 64 || ni * std::max<uint64_t>(jt, 1) <= scan_limit;
+    if (binning == 1) scan = ni == 0;
+    else if (binning == -1) scan = true;
+    J.scan = scan ? 1 : 0;
+    any_binned |= !scan && ni > 0;
+    J.field = j.field_dev;
+    J.scalar_bits = j.scalar_bits;
+    J.bg_bits = j.background_bits;
+    J.layer_of_geom = j.layer_dev;
+    J.out = j.out_dev;
+    const bool lines = has_lines(j);
+    any_poly |= dg->has_poly;
+    any_lp |= lines;
+    if (groups.empty() || groups.back().dtype != j.dtype || groups.back().fun != (int)j.fun ||
+        groups.back().lines != lines)
+      groups.push_back(Group{j.dtype, (int)j.fun, lines, (uint32_t)T, (uint32_t)T});
+    V += dg->V;
+    I += dg->I;
+    T += jt;
+    if (T >= ((uint64_t)1 << 29))
+      return rz::fail(RZ_CAPACITY_ERROR, "more than 2^29 output tiles in one call or batch");
+    groups.back().t_hi = (uint32_t)T;
+    vend[k] = V;
+    iend[k] = I;
+    tend[k] = (int64_t)T;
+  }
+  if (I >= (int64_t)UINT32_MAX) return rz::fail(RZ_CAPACITY_ERROR, "more than 2^32 items in one batch");
+  const uint32_t n_tiles = (uint32_t)T;
 
   // workspace
+  HIP_TRY(ensure(c->jobtab, tab_bytes));
   const size_t misc_bytes = 64 + kBurnSlots * sizeof(unsigned long long);
   HIP_TRY(ensure(c->misc, misc_bytes));
+  const size_t queue_words = groups.size() * kQueueWords;
+  HIP_TRY(ensure(c->queue, queue_words * 4));
   uint32_t* d_status = (uint32_t*)c->misc.p;
   unsigned long long* d_burns = (unsigned long long*)((char*)c->misc.p + 64);
-  HIP_TRY(hipMemsetAsync(c->misc.p, 0, c->profiling > 1 ? misc_bytes : 64, s));
-  if (dg->has_poly) HIP_TRY(ensure(c->edges, (size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)));
-  if (dg->has_line || dg->has_point) HIP_TRY(ensure(c->px, (size_t)std::max<int64_t>(V, 1) * 16));
-  if (dg->has_line || dg->has_point) HIP_TRY(ensure(c->vflags, (size_t)std::max<int64_t>(V, 1)));
+  if (any_poly) HIP_TRY(ensure(c->edges, (size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)));
+  if (any_lp) HIP_TRY(ensure(c->px, (size_t)std::max<int64_t>(V, 1) * 16));
+  if (any_lp) HIP_TRY(ensure(c->vflags, (size_t)std::max<int64_t>(V, 1)));
   HIP_TRY(ensure(c->item_box, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox)));
   HIP_TRY(ensure(c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
   HIP_TRY(ensure(c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
-
-  // RUSTERIZE_BINNING=always|never overrides the size heuristic (tests use it
-  // to run both item-list forms on the same inputs).
-  bool scan_mode = I <= 256 || (uint64_t)I * (uint64_t)n_tiles <= ((uint64_t)1 << 24);
-  if (const char* bm = std::getenv("RUSTERIZE_BINNING")) {
-    if (std::strcmp(bm, "always") == 0) scan_mode = I == 0;
-    else if (std::strcmp(bm, "never") == 0) scan_mode = true;
+  if (any_binned) {
+    HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
   }
+  const JobDev* d_jobs = (const JobDev*)c->jobtab.p;
+  const int64_t* d_vend = (const int64_t*)(d_jobs + n);
+  const int64_t* d_iend = d_vend + n;
+  const int64_t* d_tend = d_iend + n;
 
   if (c->profiling) {
     HIP_TRY(c->take_events());
     HIP_TRY(hipEventRecord(c->ev[0], s));
   }
-  // 1. edges / pixel coordinates / item boxes, then per-call item descriptors
-  if (I > 0) {
+  // upload the job table unless the device already holds these exact bytes
+  // (repeated calls with the same arguments skip the copy)
+  if (!(c->tab_dev == c->jobtab.p && c->tab_dev_bytes == tab_bytes &&
+        std::memcmp(c->host_tab, c->tab_build.data(), tab_bytes) == 0)) {
+    if (c->tab_pending) HIP_TRY(hipEventSynchronize(c->tab_ev));  // staging still being read
+    c->tab_pending = false;
+    if (c->host_tab_cap < tab_bytes) {
+      if (c->host_tab) HIP_TRY(hipHostFree(c->host_tab));
+      c->host_tab = nullptr;
+      c->host_tab_cap = 0;
+      HIP_TRY(hipHostMalloc(&c->host_tab, tab_bytes, hipHostMallocDefault));
+      c->host_tab_cap = tab_bytes;
+    }
+    std::memcpy(c->host_tab, c->tab_build.data(), tab_bytes);
+    c->tab_dev = nullptr;
+    HIP_TRY(hipMemcpyAsync(c->jobtab.p, c->host_tab, tab_bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(c->tab_ev, s));
+    c->tab_pending = true;
+    c->tab_dev = c->jobtab.p;
+    c->tab_dev_bytes = tab_bytes;
+  }
+  zero_words(c->misc.p, (c->profiling > 1 ? misc_bytes : 64) / 4, s);
+
+  // 1. edges / pixel coordinates / item boxes, then the item descriptors
+  if (I > 0)
     hipLaunchKernelGGL(k_init_items, dim3(blocks_for(I)), dim3(256), 0, s, (ItemBox*)c->item_box.p,
                        (unsigned long long*)c->item_last.p, I);
-  }
   if (V > 0) {
     PrepArgs pa;
-    pa.xy = dg->xy;
-    pa.vert_ring = dg->vert_ring;
-    pa.ring_voff = dg->ring_voff;
-    pa.ring_item = dg->ring_item;
-    pa.item_kind = dg->item_kind;
+    pa.jobs = d_jobs;
+    pa.job_vend = d_vend;
+    pa.n_jobs = (int)n;
     pa.n_vertices = V;
-    pa.xmin = info->xmin;
-    pa.ymax = info->ymax;
-    pa.xres = info->xres;
-    pa.yres = info->yres;
-    pa.nrows = nrows;
-    pa.ncols = ncols;
     pa.edges = (PolyEdge*)c->edges.p;
     pa.px = (double2*)c->px.p;
     pa.vflags = (uint8_t*)c->vflags.p;
@@ -495,46 +645,27 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
     pa.item_last = (unsigned long long*)c->item_last.p;
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
   }
-  if (!scan_mode) {
-    HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
-    HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
-    HIP_TRY(hipMemsetAsync((uint32_t*)c->counts.p + I, 0, 4, s));
-  }
+  if (any_binned) zero_words((uint32_t*)c->counts.p + I, 1, s);
   if (I > 0) {
     DescArgs da;
-    da.item_box = (const ItemBox*)c->item_box.p;
-    da.item_kind = dg->item_kind;
-    da.item_geom = dg->item_geom;
-    da.item_vbeg = dg->item_vbeg;
-    da.item_last = (const unsigned long long*)c->item_last.p;
-    da.layer_of_geom = layer_dev;
-    da.field = field_dev;
-    da.scalar_bits = 0;
-    if (!field_dev) std::memcpy(&da.scalar_bits, scalar_host, esz);
-    da.esz = (int32_t)esz;
+    da.jobs = d_jobs;
+    da.job_iend = d_iend;
+    da.n_jobs = (int)n;
     da.n_items = I;
+    da.item_box = (const ItemBox*)c->item_box.p;
+    da.item_last = (const unsigned long long*)c->item_last.p;
     da.desc = (ItemDesc*)c->desc.p;
-    da.ntx = ntx;
-    da.nty = nty;
-    da.tile_w = TW;
-    da.counts = scan_mode ? nullptr : (uint32_t*)c->counts.p;
+    da.counts = any_binned ? (uint32_t*)c->counts.p : nullptr;
     hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
 
-  // 2. binning: scan mode for few items, stable sort-middle bins otherwise
+  // 2. binning: stable sort-middle item lists per tile for binned jobs
   TileArgs ta;
   std::memset(&ta, 0, sizeof(ta));
   uint64_t n_pairs = 0;
-  if (!scan_mode) {
-    BinArgs ba;
-    ba.desc = (const ItemDesc*)c->desc.p;
-    ba.n_items = I;
-    ba.ntx = ntx;
-    ba.nty = nty;
-    ba.tile_w = TW;
-    ba.tiles_per_layer = (uint32_t)tpl;
-    ba.offsets = (const uint64_t*)c->offsets.p;
+  if (any_binned) {
+    const uint64_t* d_total = (const uint64_t*)c->offsets.p + I;
     size_t tb = 0;
     HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->counts.p,
                                     (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
@@ -543,18 +674,27 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
     HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->counts.p,
                                     (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
                                     rocprim::plus<uint64_t>(), s));
-    HIP_TRY(hipMemcpyAsync(&c->host_misc[0], (uint64_t*)c->offsets.p + I, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&c->host_misc[0], d_total, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     n_pairs = c->host_misc[0];
     if (n_pairs >= (uint64_t)UINT32_MAX)
       return rz::fail(RZ_CAPACITY_ERROR, "more than 2^32 tile/item pairs in one call");
-    HIP_TRY(ensure(c->keys_a, std::max<uint64_t>(n_pairs, 1) * 4));
-    HIP_TRY(ensure(c->vals_a, std::max<uint64_t>(n_pairs, 1) * 4));
-    HIP_TRY(ensure(c->keys_b, std::max<uint64_t>(n_pairs, 1) * 4));
-    HIP_TRY(ensure(c->vals_b, std::max<uint64_t>(n_pairs, 1) * 4));
+    const uint64_t cap = std::max<uint64_t>(n_pairs, 1);
+    HIP_TRY(ensure(c->keys_a, cap * 4));
+    HIP_TRY(ensure(c->vals_a, cap * 4));
+    HIP_TRY(ensure(c->keys_b, cap * 4));
+    HIP_TRY(ensure(c->vals_b, cap * 4));
     HIP_TRY(ensure(c->tile_off, ((size_t)n_tiles + 1) * 4));
+    HIP_TRY(ensure(c->pair_desc, cap * sizeof(ItemDesc)));
+    BinArgs ba;
+    ba.jobs = d_jobs;
+    ba.desc = (const ItemDesc*)c->desc.p;
+    ba.n_items = I;
+    ba.offsets = (const uint64_t*)c->offsets.p;
+    ba.cap = n_pairs;
     ba.keys = (uint32_t*)c->keys_a.p;
     ba.vals = (uint32_t*)c->vals_a.p;
+    ba.status = d_status;
     hipLaunchKernelGGL(k_bin_emit, dim3(blocks_for(I)), dim3(256), 0, s, ba);
     if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], s));
     // stable LSD radix sort by tile key keeps geometry order inside each tile
@@ -571,42 +711,37 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
                                         (uint32_t*)c->vals_b.p, (size_t)n_pairs, 0, end_bit, s));
     }
     hipLaunchKernelGGL(k_tile_offsets, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
-                       (const uint32_t*)c->keys_b.p, (uint32_t)n_pairs, n_tiles,
-                       (uint32_t*)c->tile_off.p);
-    ta.tile_off = (const uint32_t*)c->tile_off.p;
-    if (n_pairs > 0) {
-      HIP_TRY(ensure(c->pair_desc, (size_t)n_pairs * sizeof(ItemDesc)));
+                       (const uint32_t*)c->keys_b.p, d_total, n_pairs, (const uint32_t*)d_status,
+                       n_tiles, (uint32_t*)c->tile_off.p);
+    if (n_pairs > 0)
       hipLaunchKernelGGL(k_gather_desc, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
-                         (const ItemDesc*)c->desc.p, (const uint32_t*)c->vals_b.p,
-                         (uint32_t)n_pairs, (ItemDesc*)c->pair_desc.p);
-    } else {
-      HIP_TRY(ensure(c->pair_desc, sizeof(ItemDesc)));
-    }
+                         (const ItemDesc*)c->desc.p, I, (const uint32_t*)c->vals_b.p, d_total,
+                         n_pairs, (ItemDesc*)c->pair_desc.p);
+    ta.tile_off = (const uint32_t*)c->tile_off.p;
     ta.pair_desc = (const ItemDesc*)c->pair_desc.p;
   } else if (c->profiling) {
     HIP_TRY(hipEventRecord(c->ev[2], s));
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
 
-  // 3. tiles
-  ta.nrows = nrows;
-  ta.ncols = ncols;
-  ta.ntx = ntx;
-  ta.nty = nty;
-  ta.tiles_per_layer = (uint32_t)tpl;
-  ta.n_tiles = n_tiles;
-  ta.n_items = I;
+  // 3. tiles: one persistent work-queue launch per group
+  zero_words(c->queue.p, queue_words, s);
+  ta.jobs = d_jobs;
+  ta.job_tend = d_tend;
+  ta.n_jobs = (int)n;
   ta.desc = (const ItemDesc*)c->desc.p;
   ta.edges = (const PolyEdge*)c->edges.p;
   ta.px = (const double2*)c->px.p;
   ta.vflags = (const uint8_t*)c->vflags.p;
-  ta.bg_bits = 0;
-  std::memcpy(&ta.bg_bits, bg_host, esz);
-  ta.dedup_lines = info->xres != info->yres;
-  ta.out = out_dev;
   ta.burns = c->profiling > 1 ? d_burns : nullptr;
   ta.status = d_status;
-  launch_tile_dispatch(dtype, (int)fun, ta, dg->has_line || dg->has_point, s);
+  for (size_t g = 0; g < groups.size(); ++g) {
+    if (groups[g].t_hi == groups[g].t_lo) continue;
+    ta.t_lo = groups[g].t_lo;
+    ta.t_hi = groups[g].t_hi;
+    ta.queue = (uint32_t*)c->queue.p + g * kQueueWords;
+    launch_tile_dispatch(groups[g].dtype, groups[g].fun, ta, groups[g].lines, s);
+  }
   HIP_TRY(hipGetLastError());
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
   if (c->profiling) c->accumulate_times();
@@ -614,6 +749,42 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
   c->last_pairs = n_pairs;
   c->last_tiles = n_tiles;
   return RZ_OK;
+}
+
+extern "C" rz_status rz_batch_rasterize(rz_context* c, const rz_job* jobs, int64_t n_jobs) {
+  if (!c || (!jobs && n_jobs > 0)) return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_batch_rasterize");
+  return run_batch(c, jobs, n_jobs);
+}
+
+extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry* dg,
+                                         const rz_raster_info* info, rz_dtype dtype, rz_fun fun,
+                                         const void* field_dev, const void* scalar_host,
+                                         const uint32_t* layer_dev, uint32_t n_layers,
+                                         const void* bg_host, int all_touched, rz_sum_mode sum_mode,
+                                         void* out_dev) {
+  if (!c || !dg || !info || !bg_host || !out_dev)
+    return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_rasterize_device");
+  if ((unsigned)dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
+  if ((unsigned)fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
+  if (!field_dev && !scalar_host) return rz::fail(RZ_VALUE_ERROR, "no field and no scalar value");
+  if (all_touched)
+    return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not implemented on the device path yet");
+  (void)sum_mode;  // the ordered fold is exact, so it also serves the unordered mode
+  if (info->nrows == 0 || info->ncols == 0) return RZ_OK;  // empty raster: nothing to write
+  const size_t esz = rz_dtype_size(dtype);
+  rz_job j;
+  std::memset(&j, 0, sizeof(j));
+  j.geometry = dg;
+  j.info = *info;
+  j.dtype = dtype;
+  j.fun = fun;
+  j.field_dev = field_dev;
+  if (!field_dev) std::memcpy(&j.scalar_bits, scalar_host, esz);
+  j.layer_dev = layer_dev;
+  j.n_layers = layer_dev ? n_layers : 1u;
+  std::memcpy(&j.background_bits, bg_host, esz);
+  j.out_dev = out_dev;
+  return run_batch(c, &j, 1);
 }
 
 // ------------------------------------------------- host-in / host-out ----

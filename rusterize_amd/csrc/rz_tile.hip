@@ -30,19 +30,37 @@ __device__ inline int32_t quad_max_all(int32_t x) {
   return x;
 }
 
-// Persistent tile kernel.
+// One 64-byte descriptor from a read-only table (four 16-byte loads).
+__device__ inline ItemDesc load_desc(const RZ_CONST ItemDesc* p) {
+  const RZ_CONST u32x4* q = (const RZ_CONST u32x4*)p;
+  const u32x4 w[4] = {q[0], q[1], q[2], q[3]};
+  ItemDesc d;
+  __builtin_memcpy(&d, w, sizeof(d));
+  return d;
+}
+
+// First index of XCD x's contiguous share of n.
+__device__ inline uint32_t xcd_unit_lo(uint32_t x, uint32_t n) {
+  const uint32_t per = n / NUM_XCD, rem = n % NUM_XCD;
+  return x * per + min(x, rem);
+}
+
+// Persistent work-queue tile kernel, one launch per pixel-function group of
+// a batch (a single call is a batch of one job).
 //
-// Scheduling: workgroup b runs on XCD b % 8; each XCD owns a contiguous
-// eighth of the tiles (neighbouring tiles share cache lines and items in
-// that XCD's L2) and its workgroups walk that range with a stride.  The next
-// tile's item range is fetched with scalar loads before the stores go out.
-// The four waves of a workgroup work on the four 16-row strips of a tile
-// independently (no workgroup barrier).
+// Scheduling: a unit of work is one 16-row strip of one output tile, and
+// every wave works alone (no workgroup barrier).  Workgroup b runs on XCD
+// b % 8; each XCD owns a contiguous eighth of the launch's tiles, so
+// neighbouring tiles share cache lines and items in that XCD's L2.  Waves
+// take blocks of tiles from per-(XCD, strip) counters and, when their XCD's
+// range is used up, from the other XCDs' counters, so jobs of very uneven
+// size still end together.  Every wave leaves when all eight ranges of its
+// strip are used up.
 //
-// Per tile (64 rows x TW columns): the background fill of
-// RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and every write of
-// every item meeting the tile, folded in geometry order in registers, then
-// written to HBM once.
+// Per unit (16 rows x TW columns of the job's output): the background fill
+// of RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and every write
+// of every item meeting the strip, folded in geometry order in registers,
+// then written to HBM once.
 //
 // Two lane layouts per wave:
 //  * coverage: lane (rl = lane/4, q = lane%4) takes every 4th active edge of
@@ -71,8 +89,6 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
   const int rl = lane >> 2, q = lane & 3;            // coverage layout
   const int srow = lane / LPR, ch = lane % LPR;      // store layout
   const int ccol = ch * PPC;                         // first local column of the chunk
-  const double ncols_f = (double)a.ncols;
-  const T bg = from_bits<T>(a.bg_bits);
   PolyEdge* stage = s_edge[wave];
   uint32_t* cnt = s_cnt[wave];
   if (LINES) {
@@ -82,37 +98,107 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
   unsigned long long burns = 0;
   int mbuf = 0;  // alternating row-mask buffer: one wave_sync per polygon item
 
-  // this XCD's tile range and this workgroup's slot in it
+  // ---- work queue.  Wave w of every workgroup rasterises strip w (rows
+  // 16w..16w+15) of its tiles.  Each XCD owns a contiguous eighth of the
+  // launch's tiles.  A wave's first tile is its workgroup's slot in that
+  // range (no atomic); after the first `slots` tiles, queue (x, w) hands out
+  // blocks of a.chunk consecutive tiles, so the waves working on one strip
+  // index walk neighbouring tiles together (their stores fill neighbouring
+  // lines).  A grab is issued one block ahead, so its latency overlaps the
+  // current block.  A wave whose own XCD range is used up reads all eight
+  // counters of its strip with one relaxed load (counters only grow: a range
+  // that reads as used up is used up) and continues on the next XCD range
+  // with work left.
+  const uint32_t n_loc = a.t_hi - a.t_lo;
   const uint32_t xcd = blockIdx.x % NUM_XCD, slot = blockIdx.x / NUM_XCD;
   const uint32_t slots = gridDim.x / NUM_XCD;
-  const uint32_t per = a.n_tiles / NUM_XCD, rem = a.n_tiles % NUM_XCD;
-  const uint32_t lo = xcd * per + min(xcd, rem), hi = lo + per + (xcd < rem ? 1u : 0u);
-  const bool binned = a.pair_desc != nullptr;
-
-  uint32_t t = lo + slot;
+  const uint32_t B = a.chunk;
+  uint32_t* const qbase = a.queue + (uint32_t)wave * kQueueStride;  // queue (x, w) at qbase + 4x*stride
+  uint32_t cur_x = xcd;
+  uint32_t pending = 0;  // lane 0: result of the grab in flight on cur_x
+  auto issue = [&](uint32_t x) {
+    if (lane == 0) pending = atomicAdd(qbase + x * 4u * kQueueStride, 1u);
+  };
+  // first tile handed out by queue x's grab number c
+  auto grab_tile = [&](uint32_t x, uint32_t c) -> uint64_t {
+    return (uint64_t)xcd_unit_lo(x, n_loc) + slots + (uint64_t)c * B;
+  };
+  uint32_t t_cur = xcd_unit_lo(xcd, n_loc) + slot;  // current block [t_cur, t_end)
+  uint32_t t_end = t_cur + 1u;
+  auto take = [&](uint32_t x) -> bool {
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)pending, 0);
+    const uint64_t s = grab_tile(x, c), xhi = xcd_unit_lo(x + 1u, n_loc);
+    if (s >= xhi) return false;
+    t_cur = (uint32_t)s;
+    t_end = (uint32_t)min(s + (uint64_t)B, xhi);
+    return true;
+  };
+  // (a grab on cur_x is always in flight when next_block runs)
+  auto next_block = [&]() -> bool {
+    if (take(cur_x)) return true;
+    for (;;) {
+      bool avail = false;
+      if (lane < NUM_XCD) {
+        const uint32_t seen = __hip_atomic_load(qbase + (uint32_t)lane * 4u * kQueueStride,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        avail = grab_tile((uint32_t)lane, seen) < (uint64_t)xcd_unit_lo((uint32_t)lane + 1u, n_loc);
+      }
+      const uint32_t m = (uint32_t)__ballot(avail) & 0xFFu;
+      if (m == 0u) return false;
+      const uint32_t rot = ((m >> xcd) | (m << (NUM_XCD - xcd))) & 0xFFu;
+      const uint32_t x = (xcd + (uint32_t)__builtin_ctz(rot)) % NUM_XCD;
+      issue(x);
+      if (take(x)) {
+        cur_x = x;
+        return true;
+      }
+    }
+  };
+  const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
+  const RZ_CONST int64_t* tend_c = as_const(a.job_tend);
+  const RZ_CONST uint32_t* toff_c = as_const(a.tile_off);
+  const RZ_CONST ItemDesc* pdesc_c = as_const(a.pair_desc);
+  const RZ_CONST ItemDesc* desc_c = as_const(a.desc);
   uint32_t pb = 0, pe = 0;
-  if (t < hi && binned) {
-    pb = a.tile_off[t];
-    pe = a.tile_off[t + 1];
+  auto fetch_range = [&](uint32_t tl) {
+    if (a.tile_off) {
+      pb = toff_c[a.t_lo + tl];
+      pe = toff_c[a.t_lo + tl + 1u];
+    }
+  };
+  bool have = t_cur < xcd_unit_lo(xcd + 1u, n_loc);
+  issue(xcd);  // the grab after the first tile (or the first block, if there is no first tile)
+  if (!have) {
+    have = next_block();
+    if (have) issue(cur_x);
   }
-  for (; t < hi; t += slots) {
-    const uint32_t layer = t / a.tiles_per_layer;
-    const uint32_t tl = t - layer * a.tiles_per_layer;
-    const int32_t ty = (int32_t)(tl / (uint32_t)a.ntx), tx = (int32_t)(tl % (uint32_t)a.ntx);
-    const int32_t R0 = ty * TILE_H, C0 = tx * TW;
-    const int32_t Rw = R0 + wave * WAVE_ROWS;  // first row of this wave's strip
-    const int32_t r = Rw + rl;                 // coverage-layout row of this lane
-    const int32_t strip_r1 = min(Rw + WAVE_ROWS, a.nrows);
-    const int32_t tile_r1 = min(R0 + TILE_H, a.nrows);
-    const int32_t tile_c1 = min(C0 + TW, a.ncols);
+  if (have) fetch_range(t_cur);
+  const int32_t strip = wave;
+
+  while (have) {
+    const uint32_t t = a.t_lo + t_cur++;
+    const RZ_CONST JobDev& J = jobs_c[find_job(tend_c, a.n_jobs, (int64_t)t)];
+    const int32_t nrows = J.nrows, ncols = J.ncols;
+    const double ncols_f = (double)ncols, nrows_f = (double)nrows;
+    const T bg = from_bits<T>(J.bg_bits);
+    const uint32_t tj = t - J.tile_base;
+    const uint32_t layer = tj / J.tiles_per_layer;
+    const uint32_t tl = tj - layer * J.tiles_per_layer;
+    const int32_t ty = (int32_t)(tl / (uint32_t)J.ntx), tx = (int32_t)(tl % (uint32_t)J.ntx);
+    const int32_t C0 = tx * TW;
+    const int32_t Rw = ty * TILE_H + strip * WAVE_ROWS;  // first row of the strip
+    const int32_t r = Rw + rl;                          // coverage-layout row of this lane
+    const int32_t strip_r1 = min(Rw + WAVE_ROWS, nrows);
+    const int32_t tile_c1 = min(C0 + TW, ncols);
     const bool row_ok = r < strip_r1;
+    const bool dedup_job = J.dedup_lines != 0;
 
     T acc[NPIX];  // acc[k * PPC + j]: row srow + k*RSTEP, column ccol + j
 #pragma unroll
     for (int j = 0; j < NPIX; ++j) acc[j] = bg;
 
-    auto process_item = [&](const ItemDesc& d) {
-      // wave-uniform skip: the item box misses this wave's strip or the tile
+    auto process_item = [&](const ItemDesc d) {
+      // wave-uniform skip: the item box misses this strip
       if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) return;
       const int32_t kind = (int32_t)(d.kind_layer & 3u);
       const T v = from_bits<T>(d.value);
@@ -126,15 +212,23 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         int32_t amax = -1;
         for (int64_t cb = vb; cb < ve; cb += 64) {
           // stage the edges of this chunk that reach the strip, compacted
+          // (two 16-byte vector loads and stores: no struct copy through scratch)
           const int64_t e = cb + lane;
-          PolyEdge ed;
+          uint4 e_lo = make_uint4(0u, 0u, 0u, 0u), e_hi = make_uint4(0u, 0u, 0u, 0u);
           bool act = false;
           if (e < ve) {
-            ed = a.edges[e];
-            act = ed.ystart < strip_r1 && ed.yend > Rw && ed.ystart < ed.yend;
+            const uint4* src = reinterpret_cast<const uint4*>(a.edges + e);
+            e_lo = src[0];
+            e_hi = src[1];  // dxdy, ystart (z), yend (w)
+            const int32_t ys = (int32_t)e_hi.z, ye = (int32_t)e_hi.w;
+            act = ys < strip_r1 && ye > Rw && ys < ye;
           }
           const uint64_t m = __ballot(act);
-          if (act) stage[__popcll(m & ((1ull << lane) - 1ull))] = ed;
+          if (act) {
+            uint4* dst = reinterpret_cast<uint4*>(stage + __popcll(m & ((1ull << lane) - 1ull)));
+            dst[0] = e_lo;
+            dst[1] = e_hi;
+          }
           wave_sync();
           const int n = __popcll(m);
           for (int k0 = 0; k0 < n; k0 += 4) {  // uniform trip count, predicated lanes
@@ -177,14 +271,14 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         const uint32_t idx = (uint32_t)(cy - Rw) * TW + (uint32_t)(cx - C0);
         atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
       };
-      const bool dedup = kind == 1 && a.dedup_lines;
+      const bool dedup = kind == 1 && dedup_job;
       uint64_t seen = 0u;  // PixelCache bits of this lane's pixels
       for (int64_t cs = vb; cs < ve; cs += 65535) {  // <= 65535 writes per pixel per chunk
         const int64_t ce = min(ve, cs + 65535);
         for (int64_t s = cs + lane; s < ce; s += 64) {
           const double2 p0 = a.px[s];
           if (kind == 2) {  // point: extract_point + burn_point
-            if (p0.x >= 0.0 && p0.x < ncols_f && p0.y >= 0.0 && p0.y < (double)a.nrows)
+            if (p0.x >= 0.0 && p0.x < ncols_f && p0.y >= 0.0 && p0.y < nrows_f)
               emit((int64_t)p0.x, (int64_t)p0.y);
             continue;
           }
@@ -192,7 +286,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
           if (!(f & VF_HAS_NEXT)) continue;
           const double2 p1 = a.px[s + 1];
           if (!(fmin(p0.x, p1.x) < ncols_f && fmax(p0.x, p1.x) >= 0.0 &&
-                fmin(p0.y, p1.y) < (double)a.nrows && fmax(p0.y, p1.y) >= 0.0))
+                fmin(p0.y, p1.y) < nrows_f && fmax(p0.y, p1.y) >= 0.0))
             continue;  // extract_line bbox filter, edges.rs:130
           const int64_t ix0 = sat_isize(floor(p0.x)), iy0 = sat_isize(floor(p0.y));
           const int64_t ix1 = sat_isize(floor(p1.x)), iy1 = sat_isize(floor(p1.y));
@@ -243,50 +337,51 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
       }
     };
 
-    if (binned) {
-      for (uint32_t i = pb; i < pe; ++i) process_item(a.pair_desc[i]);
+    if (!J.scan) {
+      for (uint32_t i = pb; i < pe; ++i) process_item(load_desc(pdesc_c + i));
     } else {
-      // scan mode: test every item's box in order (small item counts)
-      for (int64_t base = 0; base < a.n_items; base += 64) {
+      // scan job: test every item's box in order (small item counts)
+      const int64_t ib = J.i_base, ie = J.i_base + J.ni;
+      for (int64_t base = ib; base < ie; base += 64) {
         const int64_t it = base + lane;
         bool hit = false;
-        if (it < a.n_items) {
-          const ItemDesc d = a.desc[it];
-          hit = (d.kind_layer >> 2) == layer && d.r0 < tile_r1 && d.r1 > R0 && d.c0 < tile_c1 &&
+        if (it < ie) {
+          const ItemDesc d = load_desc(desc_c + it);
+          hit = (d.kind_layer >> 2) == layer && d.r0 < strip_r1 && d.r1 > Rw && d.c0 < tile_c1 &&
                 d.c1 > C0 && d.r0 < d.r1 && d.c0 < d.c1;
         }
         uint64_t m = __ballot(hit);
         while (m) {
           const int bit = __builtin_ctzll(m);
           m &= m - 1;
-          process_item(a.desc[base + bit]);
+          process_item(load_desc(desc_c + base + bit));
         }
       }
     }
 
-    // fetch the next tile's item range before this tile's stores
-    const uint32_t tn = t + slots;
-    if (tn < hi && binned) {
-      pb = a.tile_off[tn];
-      pe = a.tile_off[tn + 1];
+    // take the next tile (and its item range) before this strip's stores
+    if (t_cur == t_end) {
+      have = next_block();
+      if (have) issue(cur_x);
     }
+    if (have) fetch_range(t_cur);
 
-    // ---- write the tile once (background + burns): each store instruction
+    // ---- write the strip once (background + burns): each store instruction
     // covers RSTEP whole rows of the tile
     const int32_t c = C0 + ccol;
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
       const int32_t row = Rw + srow + k * RSTEP;
-      if (row < a.nrows) {
-        T* orow = (T*)a.out + ((uint64_t)layer * (uint64_t)a.nrows + (uint64_t)row) * (uint64_t)a.ncols;
-        if (c + PPC <= a.ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
+      if (row < nrows) {
+        T* orow = (T*)J.out + ((uint64_t)layer * (uint64_t)nrows + (uint64_t)row) * (uint64_t)ncols;
+        if (c + PPC <= ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
           u32x4 vec;
           __builtin_memcpy(&vec, &acc[k * PPC], 16);
           __builtin_nontemporal_store(vec, (u32x4*)(orow + c));
         } else {
 #pragma unroll
           for (int j = 0; j < PPC; ++j)
-            if (c + j < a.ncols) orow[c + j] = acc[k * PPC + j];
+            if (c + j < ncols) orow[c + j] = acc[k * PPC + j];
         }
       }
     }

@@ -1,0 +1,187 @@
+"""Device-resident runtime of librusterize_hip from Python: contexts, device
+buffers, resident geometry sets, single calls and batches.
+
+This is the layer a serving or batching host uses: geometry sets are
+uploaded once and rasterised any number of times with new fields and grids;
+outputs stay on the device until downloaded.  Every call goes through the C
+ABI of include/rusterize_hip.h; there is no CPU fallback (a missing GPU is a
+RusterizeNativeError from rz_context_create).
+"""
+from __future__ import annotations
+
+from ctypes import byref, c_double, c_uint64, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import Geometry, RzJob, RzRasterInfo, check
+
+
+def value_bits(value, dtype: str) -> int:
+    """Raw little-endian bits of `value` cast to `dtype` (the C ABI's value form)."""
+    b = np.array([value]).astype(_lib.np_dtype(dtype)).tobytes()
+    return int.from_bytes(b.ljust(8, b"\0"), "little")
+
+
+class Context:
+    """rz_context: one HIP stream and a grow-only workspace on `device`."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.lib()
+        self.ptr = c_void_p()
+        check(self.L.rz_context_create(int(device), byref(self.ptr)))
+
+    def close(self) -> None:
+        if self.ptr:
+            self.L.rz_context_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self) -> None:
+        check(self.L.rz_context_synchronize(self.ptr))
+
+    def set_profiling(self, level: int) -> None:
+        check(self.L.rz_context_set_profiling(self.ptr, int(level)))
+
+    def stage_times(self) -> dict:
+        t = (c_double * 8)()
+        n = self.L.rz_context_stage_times(self.ptr, t, 8)
+        return {self.L.rz_stage_name(i).decode(): t[i] for i in range(n)}
+
+    def reset_stage_times(self) -> None:
+        self.L.rz_context_reset_stage_times(self.ptr)
+
+    def last_counters(self) -> tuple[int, int, int]:
+        b, p, t = c_uint64(), c_uint64(), c_uint64()
+        check(self.L.rz_context_last_counters(self.ptr, byref(b), byref(p), byref(t)))
+        return b.value, p.value, t.value
+
+    def workspace_bytes(self) -> int:
+        return int(self.L.rz_context_workspace_bytes(self.ptr))
+
+
+class DeviceBuffer:
+    """Device memory owned by a context (rz_device_malloc / rz_device_free)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        self.ptr = c_void_p()
+        check(ctx.L.rz_device_malloc(ctx.ptr, max(self.nbytes, 1), byref(self.ptr)))
+
+    @classmethod
+    def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(ctx, a.nbytes)
+        buf.upload(a)
+        return buf
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        check(self.ctx.L.rz_memcpy_h2d(self.ctx.ptr, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested array larger than the device buffer")
+        check(self.ctx.L.rz_memcpy_d2h(self.ctx.ptr, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx.L.rz_device_free(self.ctx.ptr, self.ptr)
+            self.ptr = c_void_p()
+
+
+class DeviceGeometry:
+    """A geometry set resident on the device (rz_device_geometry_upload)."""
+
+    def __init__(self, ctx: Context, geom: Geometry):
+        self.ctx = ctx
+        self.ptr = c_void_p()
+        self.n_geoms = geom.n_geoms
+        self.n_items, self.n_vertices = geom.n_items, geom.n_vertices
+        s = geom.as_struct()
+        check(ctx.L.rz_device_geometry_upload(ctx.ptr, byref(s), byref(self.ptr)))
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx.L.rz_device_geometry_free(self.ptr)
+            self.ptr = c_void_p()
+
+
+@dataclass
+class Job:
+    """One rasterise job: the arguments of rz_rasterize_device."""
+    geometry: DeviceGeometry
+    info: RzRasterInfo
+    dtype: str
+    fun: str
+    out: DeviceBuffer
+    field: DeviceBuffer | None = None
+    scalar: float = 1
+    layers: DeviceBuffer | None = None
+    n_layers: int = 1
+    background: float = 0
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        return (self.n_layers if self.layers is not None else 1, int(self.info.nrows), int(self.info.ncols))
+
+    @property
+    def out_bytes(self) -> int:
+        b, r, c = self.shape
+        return b * r * c * _lib.np_dtype(self.dtype).itemsize
+
+    def as_struct(self) -> RzJob:
+        j = RzJob()
+        j.geometry = self.geometry.ptr
+        j.info = self.info
+        j.dtype = _lib.dtype_code(self.dtype)
+        j.fun = _lib.fun_code(self.fun)
+        j.field_dev = self.field.ptr if self.field is not None else None
+        j.scalar_bits = value_bits(self.scalar, self.dtype)
+        j.layer_dev = self.layers.ptr if self.layers is not None else None
+        j.n_layers = self.n_layers if self.layers is not None else 1
+        j.background_bits = value_bits(self.background, self.dtype)
+        j.out_dev = self.out.ptr
+        return j
+
+    def result(self) -> np.ndarray:
+        return self.out.download(self.shape, _lib.np_dtype(self.dtype))
+
+
+def rasterize_device(ctx: Context, job: Job) -> None:
+    """One call (rz_rasterize_device), asynchronous on the context's stream."""
+    dt = _lib.np_dtype(job.dtype)
+    scalar = np.array([job.scalar]).astype(dt)
+    bg = np.array([job.background]).astype(dt)
+    check(ctx.L.rz_rasterize_device(
+        ctx.ptr, job.geometry.ptr, byref(job.info), _lib.dtype_code(job.dtype), _lib.fun_code(job.fun),
+        job.field.ptr if job.field is not None else None, None if job.field is not None else scalar.ctypes.data,
+        job.layers.ptr if job.layers is not None else None, job.n_layers if job.layers is not None else 1,
+        bg.ctypes.data, 0, 0, job.out.ptr))
+
+
+def batch_rasterize(ctx: Context, jobs: list[Job]) -> None:
+    """All jobs as one pipeline (rz_batch_rasterize), asynchronous."""
+    arr = (RzJob * max(len(jobs), 1))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.as_struct()
+    check(ctx.L.rz_batch_rasterize(ctx.ptr, arr, len(jobs)))
+
+
+def grid(ncols: int, nrows: int, xmin: float = 0.0, ymin: float = 0.0, res: float = 1.0) -> RzRasterInfo:
+    """An explicit grid: ncols x nrows pixels of `res` from (xmin, ymin)."""
+    return _lib.raster_info(None, resolution=(res, res), extent=(xmin, ymin, xmin + ncols * res, ymin + nrows * res))
+
+
+__all__ = ["Context", "DeviceBuffer", "DeviceGeometry", "Job", "rasterize_device", "batch_rasterize", "grid",
+           "value_bits"]

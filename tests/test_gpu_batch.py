@@ -1,0 +1,167 @@
+"""GPU parity of batches (rz_batch_rasterize): every job of a batch of uneven
+extents, dtypes, pixel functions and geometry sets equals the oracle and its
+own single call bit for bit, whatever the batch composition or order."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from rusterize_amd import _lib
+from rusterize_amd._lib import Geometry
+from rusterize_amd.device import Context, DeviceBuffer, DeviceGeometry, Job, batch_rasterize, grid, rasterize_device
+from tests.test_gpu_parity import assert_same, field_for, mixed_geometry
+
+pytestmark = pytest.mark.gpu
+
+
+def geometry_sets():
+    return [
+        Geometry.synth(1, 31, 400, 1000.0, 700.0, 3.0, 60.0),                    # stars
+        Geometry.synth(2, 32, 2500, 1000.0, 700.0, 2.0, 120.0, 0.1, 0.3),       # convex, multi, holes
+        Geometry.synth(3, 33, 0, 1000.0, 700.0, 300, 800, 60),                  # lines, points, collections
+        mixed_geometry(),                                                        # all kinds + edge cases
+        Geometry.from_wkt(["POLYGON EMPTY", "POINT (3.5 2.5)"]),
+        Geometry.synth(2, 34, 20, 1000.0, 700.0, 50.0, 400.0, 0.0, 0.5),        # few large polygons
+    ]
+
+
+# (geometry set, ncols, nrows, xmin, ymin, dtype, fun, bands, background)
+SPECS = [
+    (0, 1000, 700, 0, 0, "float64", "sum", 1, float("nan")),
+    (0, 257, 129, 100.5, 300.25, "uint8", "count", 1, 0),
+    (1, 1000, 700, 0, 0, "int32", "first", 4, 0),
+    (1, 333, 517, -20, -13, "float32", "max", 1, float("nan")),
+    (1, 64, 64, 500, 500, "uint16", "last", 1, 7),
+    (2, 1000, 700, 0, 0, "float32", "sum", 1, float("nan")),
+    (2, 800, 300, 50, 200, "int16", "min", 2, 0),
+    (3, 300, 200, 0, 0, "uint8", "any", 1, 0),
+    (3, 300, 200, 0, 0, "float64", "last", 3, 0),
+    (3, 129, 65, 10, 20, "int64", "sum", 1, 0),
+    (4, 8, 6, 0, 0, "float64", "sum", 1, 0),
+    (5, 1000, 700, 0, 0, "uint32", "min", 1, 0),
+    (5, 2048, 1100, -500, -200, "int8", "count", 1, 0),
+    (0, 1, 1, 10, 10, "float32", "first", 1, 0),
+]
+
+
+class Batch:
+    def __init__(self, ctx, specs, seed=0):
+        self.ctx = ctx
+        self.host_geoms = geometry_sets()
+        self.dev_geoms = [DeviceGeometry(ctx, g) for g in self.host_geoms]
+        self.jobs, self.refs = [], []
+        rng = np.random.default_rng(seed)
+        self.buffers = []
+        for k, (gi, nc, nr, x0, y0, dt, fun, bands, bg) in enumerate(specs):
+            g = self.host_geoms[gi]
+            field = field_for(dt, g.n_geoms, seed=100 + k)
+            if g.n_geoms:
+                field[:: 13] = 0
+            layers = rng.integers(0, bands, g.n_geoms).astype(np.uint32) if bands > 1 else None
+            info = grid(nc, nr, x0, y0)
+            d_field = DeviceBuffer.from_array(ctx, field)
+            d_layers = DeviceBuffer.from_array(ctx, layers) if layers is not None else None
+            job = Job(self.dev_geoms[gi], info, dt, fun, DeviceBuffer(ctx, bands * nr * nc * np.dtype(dt).itemsize),
+                      field=d_field, layers=d_layers, n_layers=bands, background=bg)
+            self.jobs.append(job)
+            self.buffers += [d_field, job.out] + ([d_layers] if d_layers is not None else [])
+            self.refs.append(oracle.rasterize(g, oracle.copy_info(info), dt, fun, field=field, layer_of_geom=layers,
+                                              n_layers=bands, background=bg))
+
+    def free(self):
+        for b in self.buffers:
+            b.free()
+        for g in self.dev_geoms:
+            g.free()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("binning", [None, "always", "never"])
+def test_batch_matches_oracle_and_single_calls(ctx, binning, monkeypatch):
+    if binning:
+        monkeypatch.setenv("RUSTERIZE_BINNING", binning)
+    b = Batch(ctx, SPECS)
+    try:
+        batch_rasterize(ctx, b.jobs)
+        ctx.synchronize()
+        batched = [j.result() for j in b.jobs]
+        for k, (got, ref) in enumerate(zip(batched, b.refs)):
+            assert_same(got, ref, f"job {k} {SPECS[k]} vs oracle")
+        for k, j in enumerate(b.jobs):
+            rasterize_device(ctx, j)
+            ctx.synchronize()
+            assert_same(j.result(), batched[k], f"job {k} {SPECS[k]} single vs batch")
+    finally:
+        b.free()
+
+
+def test_batch_composition_and_order_do_not_matter(ctx):
+    b = Batch(ctx, SPECS, seed=1)
+    try:
+        order = list(range(len(SPECS)))[::-1]
+        batch_rasterize(ctx, [b.jobs[k] for k in order])
+        ctx.synchronize()
+        for k in order:
+            assert_same(b.jobs[k].result(), b.refs[k], f"reversed job {k}")
+        for part in (order[::2], order[1::2], [3], [10, 0]):
+            for k in part:  # clobber the outputs first
+                DeviceBuffer.upload(b.jobs[k].out, np.full(b.jobs[k].out_bytes, 0xA5, np.uint8))
+            batch_rasterize(ctx, [b.jobs[k] for k in part])
+            ctx.synchronize()
+            for k in part:
+                assert_same(b.jobs[k].result(), b.refs[k], f"subset {part} job {k}")
+    finally:
+        b.free()
+
+
+def test_batch_of_many_small_jobs(ctx):
+    # 300 jobs over one resident geometry set, each a different window / function
+    g = Geometry.synth(1, 41, 300, 1000.0, 700.0, 3.0, 40.0)
+    dg = DeviceGeometry(ctx, g)
+    rng = np.random.default_rng(7)
+    field = field_for("float64", g.n_geoms, seed=8)
+    d_field = DeviceBuffer.from_array(ctx, field)
+    jobs, refs, bufs = [], [], []
+    try:
+        for k in range(300):
+            nc, nr = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+            x0, y0 = float(rng.integers(-50, 900)), float(rng.integers(-50, 600))
+            fun = _lib.FUNS[k % 7]
+            info = grid(nc, nr, x0, y0)
+            out = DeviceBuffer(ctx, nr * nc * 8)
+            bufs.append(out)
+            jobs.append(Job(dg, info, "float64", fun, out, field=d_field, background=float("nan")))
+            refs.append(oracle.rasterize(g, oracle.copy_info(info), "float64", fun, field=field,
+                                         background=float("nan")))
+        batch_rasterize(ctx, jobs)
+        ctx.synchronize()
+        for k, (j, ref) in enumerate(zip(jobs, refs)):
+            assert_same(j.result(), ref, f"small job {k}")
+    finally:
+        for b in bufs:
+            b.free()
+        d_field.free()
+        dg.free()
+
+
+def test_batch_rejects_bad_jobs(ctx):
+    g = Geometry.from_wkt(["POINT (1 1)"])
+    dg = DeviceGeometry(ctx, g)
+    out = DeviceBuffer(ctx, 64)
+    try:
+        ok = Job(dg, grid(4, 4), "uint8", "sum", out)
+        bad = Job(dg, grid(4, 4), "uint8", "sum", out)
+        s = bad.as_struct()
+        s.fun = 9
+        arr = (_lib.RzJob * 2)(ok.as_struct(), s)
+        with pytest.raises(ValueError, match="Unknown pixel function"):
+            _lib.check(ctx.L.rz_batch_rasterize(ctx.ptr, arr, 2))
+        batch_rasterize(ctx, [])  # an empty batch is a no-op
+    finally:
+        out.free()
+        dg.free()

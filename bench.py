@@ -83,6 +83,22 @@ class Workload:
         return esz * self.bands * self.W * self.H + 32 * g.n_vertices + esz * g.n_geoms
 
 
+TRAFFIC_PROFILES = {"polygons": ROOT / "profiles" / "k_tile_polygons_40000x40000_hbm_traffic.md"}
+
+
+def measured_traffic(wl: Workload) -> dict | None:
+    """HBM bytes per tile launch by dtype, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or
+    None when no summary exists for it or the polygon count differs."""
+    path = TRAFFIC_PROFILES.get(wl.name)
+    if path is None or not path.exists() or wl.geom.n_geoms != 1_000_000:
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return json.loads(line.split(":", 1)[1])
+    return None
+
+
 def make_workload(name: str, n_poly: int) -> Workload:
     L = _lib.lib()
     if name == "polygons":
@@ -280,6 +296,12 @@ def main() -> int:
         burns_per_step += burns.value
         by_dtype[dt]["pairs"], by_dtype[dt]["tiles"] = pairs.value, tiles.value
     check(L.rz_context_set_profiling(ctx, 0))
+    traffic = measured_traffic(wl)
+    if traffic:
+        for dt in by_dtype:
+            if dt in traffic:
+                by_dtype[dt]["traffic_bytes"] = traffic[dt]
+                by_dtype[dt]["traffic_over_alg"] = round(traffic[dt] / wl.b_alg_tile(np.dtype(dt).itemsize), 4)
     tile_bytes = sum(wl.b_alg_tile(np.dtype(dt).itemsize) for dt, _ in wl.calls)
     call_bytes = sum(wl.b_alg(np.dtype(dt).itemsize) for dt, _ in wl.calls)
     call_ms = sum(stages.values())
@@ -321,7 +343,10 @@ def main() -> int:
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": None,
+            # HBM bytes per tile launch, averaged over the step's launches
+            "traffic": (round(sum(traffic[dt] for dt, _ in wl.calls) / len(wl.calls))
+                        if traffic and all(dt in traffic for dt, _ in wl.calls) else None),
+            "traffic_source": str(TRAFFIC_PROFILES[wl.name].relative_to(ROOT)) if traffic else None,
             "kernel": f"k_tile ({len(wl.calls)} launches per step; bytes-weighted)",
             "by_dtype": by_dtype,
             "call_GB/s": round(call_bytes / (call_ms * 1e-3) / 1e9, 1),

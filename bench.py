@@ -182,7 +182,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines"])
+    ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines", "batch"])
+    ap.add_argument("--jobs", type=int, default=256, help="batch workload: jobs of the seeded job stream")
+    ap.add_argument("--budget-gb", type=float, default=150.0,
+                    help="batch workload: device memory the jobs' inputs, outputs and workspace may use")
     ap.add_argument("--polygons", type=int, default=1_000_000, help="polygon count of the default workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None,
@@ -198,6 +201,8 @@ def main() -> int:
         print("bench.py measures one GPU: the rasteriser shards by output rows with no exchange "
               "(see DESIGN.md); multi-GPU runs are not part of this build", file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return batch_bench(args)
     steps = args.steps if args.steps is not None else {"polygons": 3, "stars": 200, "lines": 5}[args.workload]
 
     L = _lib.lib()
@@ -356,6 +361,120 @@ def main() -> int:
     }
     print(json.dumps(result), flush=True)
     return 0
+
+
+# ------------------------------------------------------------ batched jobs
+
+BATCH_SEED = 4
+
+
+def batch_bench(args) -> int:
+    """The batched workload: the first jobs of the seeded 4,096-job stream
+    (rz_synth_job.h: extents 256..8192 px, 10..50,000 star polygons of
+    4..4,096 vertices, float64 fields, the pixel function drawn per job),
+    generated on the device, run (a) as one plain rz_rasterize_device call
+    per job and (b) as one rz_batch_rasterize pipeline.  Outputs stay
+    resident; both legs are timed from device synchronisation to device
+    synchronisation over the same jobs."""
+    from rusterize_amd.device import (Context, DeviceBuffer, DeviceGeometry, Job, batch_rasterize, grid,
+                                      rasterize_device)
+
+    L = _lib.lib()
+    specs, need = [], 0.0
+    for k in range(args.jobs):
+        sp = _lib.job_spec(BATCH_SEED, k)
+        # output + coordinates and derived maps + edge records + descriptors
+        b = sp["side"] ** 2 * 8 + sp["n_vertices"] * (16 + 4 + 32) + sp["n_poly"] * 200
+        if need + b > args.budget_gb * 1e9:
+            break
+        need += b
+        specs.append(sp)
+    n = len(specs)
+    log(f"batch: {n} jobs (budget {args.budget_gb} GB, {need / 1e9:.1f} GB), "
+        f"{sum(s['n_poly'] for s in specs)} polygons, {sum(s['n_vertices'] for s in specs)} vertices, "
+        f"{sum(s['side'] ** 2 for s in specs) / 1e9:.2f} Gpx")
+    ctx = Context(0)
+    geoms, jobs, bufs = [], [], []
+    for k, sp in enumerate(specs):
+        dg = DeviceGeometry.synth_job(ctx, BATCH_SEED, k)
+        geoms.append(dg)
+        field = DeviceBuffer.from_array(ctx, _lib.job_field(BATCH_SEED, k))
+        out = DeviceBuffer(ctx, sp["side"] ** 2 * 8)
+        bufs += [field, out]
+        jobs.append(Job(dg, grid(sp["side"], sp["side"]), "float64", sp["fun"], out, field=field,
+                        background=float("nan")))
+    ctx.synchronize()
+
+    def plain():
+        for j in jobs:
+            rasterize_device(ctx, j)
+
+    def batched():
+        batch_rasterize(ctx, jobs)
+
+    times, stages = {}, {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        fn()  # warm-up (workspace)
+        ctx.synchronize()
+        reps = max(1, args.steps or 1)
+        ctx.set_profiling(1)
+        ctx.reset_stage_times()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        times[name] = (time.perf_counter() - t0) / reps
+        stages[name] = {k: round(v / reps, 3) for k, v in ctx.stage_times().items()}
+        ctx.set_profiling(0)
+        log(f"{name}: {times[name] * 1e3:.2f} ms for {n} jobs; stages (ms, summed over calls) {stages[name]}")
+    # exact burns of the whole set (untimed): one batched pass at level 2
+    ctx.set_profiling(2)
+    batched()
+    burns = ctx.last_counters()[0]
+    ctx.set_profiling(0)
+    # the two legs give identical outputs: spot-check a few jobs bit for bit
+    check_idx = sorted(set([0, n // 2, n - 1]))
+    same = True
+    for k in check_idx:
+        a = jobs[k].result()
+        rasterize_device(ctx, jobs[k])
+        b = jobs[k].result()
+        same &= bool(np.array_equal(a.view(np.uint64), b.view(np.uint64)))
+    for b_ in bufs:
+        b_.free()
+    for dg in geoms:
+        dg.free()
+    ctx.close()
+    px = sum(s["side"] ** 2 for s in specs)
+    result = {
+        "metric": METRIC,
+        "value": burns / times["batched"],
+        "unit": "pixel-burns/s",
+        "n_gpus": 1,
+        "steps": max(1, args.steps or 1),
+        "warmup": 1,
+        "ms_per_step": round(times["batched"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 crossings; float64 folds",
+        "data": "synthetic (seeded job stream, generated on the device)",
+        "config": {
+            "workload": f"batched: jobs 0..{n - 1} of the seeded 4096-job stream (extent 256..8192 px, "
+                        f"10..50000 star polygons of 4..4096 vertices, float64, fun per job)",
+            "jobs": n, "pixels": px, "burns": burns,
+            "polygons": sum(s["n_poly"] for s in specs), "vertices": sum(s["n_vertices"] for s in specs),
+        },
+        "stages_ms": stages,
+        "batched": {"ms": round(times["batched"] * 1e3, 3), "burns_per_s": burns / times["batched"],
+                    "jobs_per_s": n / times["batched"], "out_GB_per_s": px * 8 / times["batched"] / 1e9},
+        "plain": {"ms": round(times["plain"] * 1e3, 3), "burns_per_s": burns / times["plain"],
+                  "jobs_per_s": n / times["plain"], "out_GB_per_s": px * 8 / times["plain"] / 1e9},
+        "batched_over_plain": round(times["plain"] / times["batched"], 3),
+        "outputs_identical_on_spot_checks": same,
+    }
+    print(json.dumps(result), flush=True)
+    return 0 if same else 1
 
 
 if __name__ == "__main__":

@@ -171,6 +171,22 @@ rz_status rz_synth_geometry(int kind, uint64_t seed, int64_t count, double grid_
 rz_status rz_synth_field(rz_dtype dtype, int dist, uint64_t seed, int64_t n, double lo, double hi,
                          void* out);
 
+/* No reference counterpart: the seeded job stream of the batched and served
+ * workloads (job k: square extent 256..8192 px, 10..50,000 star polygons of
+ * 4..4,096 vertices, all log-uniform; a pixel function; float64 field
+ * U[0, 1)), documented in rusterize_amd/csrc/rz_synth_job.h.  The host and
+ * device generators produce identical coordinates.
+ * rz_synth_job_spec: extent side, polygon count, pixel function (rz_fun) and
+ *   vertex count of job k (any pointer may be NULL).
+ * rz_synth_job_geometry: job k's geometry on the host.
+ * rz_synth_job_field: job k's n_poly field values.
+ * rz_device_synth_job: job k's geometry generated directly on the device
+ *   (declared with the device runtime below). */
+rz_status rz_synth_job_spec(uint64_t seed, int64_t k, int32_t* side, int32_t* n_poly, int32_t* fun,
+                            int64_t* n_vertices);
+rz_status rz_synth_job_geometry(uint64_t seed, int64_t k, rz_geometry_buffers** out);
+rz_status rz_synth_job_field(uint64_t seed, int64_t k, double* out);
+
 /* --------------------------------------------------------- rasterisation -- */
 
 /*
@@ -216,6 +232,9 @@ rz_status rz_device_geometry_upload(rz_context* ctx, const rz_geometry* host,
                                     rz_device_geometry** out);
 rz_status rz_device_geometry_free(rz_device_geometry* dg);
 int64_t rz_device_geometry_count(const rz_device_geometry* dg);
+/* Job k of the seeded job stream (see rz_synth_job_spec), generated on the
+ * device into a resident geometry set. */
+rz_status rz_device_synth_job(rz_context* ctx, uint64_t seed, int64_t k, rz_device_geometry** out);
 
 /* Same semantics as rz_rasterize with every array on the device:
  * field_dev (or NULL + scalar_host), layer_dev (or NULL), out_dev.

@@ -76,6 +76,10 @@ _SIGNATURES = [
     ("rz_synth_geometry", c_int, [c_int, c_uint64, c_int64, c_double, c_double, c_double, c_double, c_double,
                                   c_double, POINTER(c_void_p)]),
     ("rz_synth_field", c_int, [c_int, c_int, c_uint64, c_int64, c_double, c_double, c_void_p]),
+    ("rz_synth_job_spec", c_int, [c_uint64, c_int64, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                  POINTER(c_int64)]),
+    ("rz_synth_job_geometry", c_int, [c_uint64, c_int64, POINTER(c_void_p)]),
+    ("rz_synth_job_field", c_int, [c_uint64, c_int64, c_void_p]),
     ("rz_rasterize", c_int, [POINTER(RzGeometry), POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                              POINTER(c_uint32), c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_context_create", c_int, [c_int, POINTER(c_void_p)]),
@@ -90,6 +94,7 @@ _SIGNATURES = [
     ("rz_device_geometry_upload", c_int, [c_void_p, POINTER(RzGeometry), POINTER(c_void_p)]),
     ("rz_device_geometry_free", c_int, [c_void_p]),
     ("rz_device_geometry_count", c_int64, [c_void_p]),
+    ("rz_device_synth_job", c_int, [c_void_p, c_uint64, c_int64, POINTER(c_void_p)]),
     ("rz_rasterize_device", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_batch_rasterize", c_int, [c_void_p, POINTER(RzJob), c_int64]),
@@ -226,6 +231,13 @@ class Geometry:
         check(lib().rz_synth_geometry(kind, seed, count, grid_w, grid_h, p0, p1, p2, p3, ctypes.byref(h)))
         return cls._from_native(h)
 
+    @classmethod
+    def synth_job(cls, seed: int, k: int) -> "Geometry":
+        """Job k of the seeded batched-workload job stream (rz_synth_job_geometry)."""
+        h = c_void_p()
+        check(lib().rz_synth_job_geometry(seed, k, ctypes.byref(h)))
+        return cls._from_native(h)
+
     # ---- views
     @property
     def n_geoms(self) -> int:
@@ -334,3 +346,19 @@ def device_available() -> bool:
         return torch.cuda.device_count() > 0
     except Exception:
         return os.path.exists("/dev/kfd")
+
+
+def job_spec(seed: int, k: int) -> dict:
+    """Size of job k of the seeded job stream: side, n_poly, fun, n_vertices."""
+    side, n_poly, fun, nv = c_int32(), c_int32(), c_int32(), c_int64()
+    check(lib().rz_synth_job_spec(seed, k, ctypes.byref(side), ctypes.byref(n_poly), ctypes.byref(fun),
+                                  ctypes.byref(nv)))
+    return {"side": side.value, "n_poly": n_poly.value, "fun": FUNS[fun.value], "n_vertices": nv.value}
+
+
+def job_field(seed: int, k: int) -> np.ndarray:
+    """Field values (float64) of job k of the seeded job stream."""
+    n = job_spec(seed, k)["n_poly"]
+    out = np.empty(n, dtype=np.float64)
+    check(lib().rz_synth_job_field(seed, k, out.ctypes.data))
+    return out

@@ -14,6 +14,7 @@
 #include <type_traits>
 
 #include "rz_internal.h"
+#include "rz_synth_job.h"
 
 // ------------------------------------------------------------------ errors --
 
@@ -910,5 +911,49 @@ extern "C" rz_status rz_synth_field(rz_dtype dtype, int dist, uint64_t seed, int
     case RZ_F64: fill_values(rng, dist, n, lo, hi, (double*)out); break;
     default: return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
   }
+  return RZ_OK;
+}
+
+// ------------------------------------------------- batched-job generator --
+
+extern "C" rz_status rz_synth_job_spec(uint64_t seed, int64_t k, int32_t* side, int32_t* n_poly,
+                                       int32_t* fun, int64_t* n_vertices) {
+  if (k < 0) return rz::fail(RZ_VALUE_ERROR, "job index must be >= 0");
+  const rzsynth::JobSpec j = rzsynth::job_spec(seed, (uint64_t)k);
+  if (side) *side = j.side;
+  if (n_poly) *n_poly = j.n_poly;
+  if (fun) *fun = j.fun;
+  if (n_vertices) {
+    int64_t v = 0;
+    for (int32_t p = 0; p < j.n_poly; ++p)
+      v += rzsynth::poly_spec(seed, (uint64_t)k, (uint64_t)p, j.side).n_vert + 1;
+    *n_vertices = v;
+  }
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_synth_job_geometry(uint64_t seed, int64_t k, rz_geometry_buffers** out) {
+  if (k < 0) return rz::fail(RZ_VALUE_ERROR, "job index must be >= 0");
+  auto* b = new (std::nothrow) rz_geometry_buffers();
+  if (!b) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
+  const rzsynth::JobSpec j = rzsynth::job_spec(seed, (uint64_t)k);
+  std::vector<double> pts;
+  for (int32_t p = 0; p < j.n_poly; ++p) {
+    const rzsynth::PolySpec ps = rzsynth::poly_spec(seed, (uint64_t)k, (uint64_t)p, j.side);
+    pts.resize(2 * (size_t)(ps.n_vert + 1));
+    for (int32_t v = 0; v <= ps.n_vert; ++v)
+      rzsynth::vertex(seed, (uint64_t)k, (uint64_t)p, ps, v, &pts[2 * (size_t)v], &pts[2 * (size_t)v + 1]);
+    b->begin_item(RZ_ITEM_POLYGONS);
+    b->add_ring(pts.data(), ps.n_vert + 1);
+    b->end_geometry();
+  }
+  *out = b;
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_synth_job_field(uint64_t seed, int64_t k, double* out) {
+  if (k < 0 || !out) return rz::fail(RZ_VALUE_ERROR, "bad job index or NULL output");
+  const rzsynth::JobSpec j = rzsynth::job_spec(seed, (uint64_t)k);
+  for (int32_t p = 0; p < j.n_poly; ++p) out[p] = rzsynth::field(seed, (uint64_t)k, (uint64_t)p);
   return RZ_OK;
 }

@@ -122,9 +122,19 @@ struct __align__(64) ItemDesc {
   uint32_t kind_layer;      // item kind | output band << 2
   uint64_t value;           // field value bits (dtype of the call)
   uint64_t last;            // lines: last kept segment + 1 (0: none)
-  uint32_t job, pad0;
+  uint32_t job;
+  uint32_t band;            // polygons with banded edges: first band entry + 1 (0: not banded)
   uint64_t pad1;
 };
+
+// Edge bands: a polygon item with more than kBandMinEdges edges that spans
+// two or more 64-row tile rows gets its edges binned by tile row (an edge
+// goes to every tile row its rows [ystart, yend) meet), so a strip visits
+// only the edges of its own tile row instead of the whole ring set.  Band
+// entry band - 1 + (ty - r0 / TILE_H) of such an item holds the edges of
+// tile row ty in band_edges[band_off[e] .. band_off[e + 1]) (any order: the
+// crossing toggles and the maximum commute).
+constexpr int kBandMinEdges = 256;
 
 struct DescArgs {
   const JobDev* jobs;
@@ -135,6 +145,19 @@ struct DescArgs {
   const unsigned long long* item_last;
   ItemDesc* desc;           // [n_items]
   uint32_t* counts;         // [n_items] tiles per item, or null (no binned job)
+  uint32_t* band_n;         // [n_items] tile rows of a banded polygon item (0: not banded)
+};
+
+struct BandArgs {
+  const JobDev* jobs;
+  const int64_t* job_vend;
+  int n_jobs;
+  int64_t n_vertices;
+  const ItemDesc* desc;     // [n_items], band fields assigned
+  const PolyEdge* edges;    // [n_vertices]
+  uint32_t* band_cnt;       // [n_bands] counts (count pass) / remaining slots (scatter pass)
+  const uint64_t* band_off; // [n_bands + 1] (scatter pass)
+  PolyEdge* band_edges;     // (scatter pass)
 };
 
 struct BinArgs {
@@ -163,6 +186,8 @@ struct TileArgs {
   // binned jobs: pairs sorted by tile, tile_off[t]..tile_off[t+1]
   const uint32_t* tile_off;  // [n_tiles + 1], or null when no job is binned
   const ItemDesc* pair_desc;  // descriptors in pair order
+  const uint64_t* band_off;   // edge bands of banded polygon items
+  const PolyEdge* band_edges;
   // scan jobs: every item of the job is tested against the tile
   const ItemDesc* desc;
   const PolyEdge* edges;

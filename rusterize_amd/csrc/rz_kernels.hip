@@ -247,15 +247,91 @@ __global__ void k_item_desc(DescArgs a) {
   d.value = v;
   d.last = a.item_last[i];
   d.job = (uint32_t)j;
-  d.pad0 = 0;
+  d.band = 0;
   d.pad1 = 0;
   a.desc[i] = d;
+  if (a.band_n) {
+    const bool banded = J.item_kind[li] == 0 && d.nv > kBandMinEdges && d.r0 < d.r1 &&
+                        (d.r1 - 1) / TILE_H > d.r0 / TILE_H;
+    a.band_n[i] = banded ? (uint32_t)((d.r1 - 1) / TILE_H - d.r0 / TILE_H + 1) : 0u;
+  }
   if (a.counts) {
     int32_t tx0, tx1, ty0, ty1;
     uint32_t c = 0;
     if (!J.scan && tile_rect(d.r0, d.r1, d.c0, d.c1, J.tile_w, tx0, tx1, ty0, ty1))
       c = (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1);
     a.counts[i] = c;
+  }
+}
+
+// ------------------------------------------------------------ edge bands --
+
+__global__ void k_band_assign(ItemDesc* desc, const uint32_t* band_n, const uint32_t* band_base,
+                              int64_t n_items) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_items && band_n[i]) desc[i].band = band_base[i] + 1u;
+}
+
+// The band entry of edge v's first tile row and the number of tile rows its
+// rows meet (0: the edge is not in a banded item or is empty).
+__device__ inline uint32_t edge_bands(const BandArgs& a, int64_t v, uint32_t& first) {
+  const JobDev& J = a.jobs[find_job(a.job_vend, a.n_jobs, v)];
+  const int64_t lv = v - J.v_base;
+  const ItemDesc& d = a.desc[J.i_base + J.ring_item[J.vert_ring[lv]]];
+  if (!d.band) return 0u;
+  const PolyEdge& e = a.edges[v];
+  if (!(e.ystart < e.yend)) return 0u;
+  const int32_t ty0 = d.r0 / TILE_H;
+  const int32_t b0 = max(e.ystart / TILE_H, ty0), b1 = min((e.yend - 1) / TILE_H, (d.r1 - 1) / TILE_H);
+  if (b1 < b0) return 0u;
+  first = d.band - 1u + (uint32_t)(b0 - ty0);
+  return (uint32_t)(b1 - b0 + 1);
+}
+
+// Count pass: one thread per vertex; lanes of a wave adding to the same
+// band entry first combine their counts (ballot per distinct key).
+__global__ __launch_bounds__(256) void k_band_count(BandArgs a) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t first = 0, nb = 0;
+  if (v < a.n_vertices) nb = edge_bands(a, v, first);
+  uint64_t todo = __ballot(nb > 0);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const uint32_t key = (uint32_t)__shfl((int)first, leader, 64);
+    const uint64_t m = __ballot(nb > 0 && first == key);
+    if (lane == leader) atomicAdd(&a.band_cnt[key], (uint32_t)__popcll(m));
+    todo &= ~m;
+  }
+  for (uint32_t b = 1; b < nb; ++b) atomicAdd(&a.band_cnt[first + b], 1u);
+}
+
+// Scatter pass: copies every edge of a banded item into each of its bands;
+// band_cnt counts down the free slots of every entry.
+__global__ __launch_bounds__(256) void k_band_scatter(BandArgs a) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t first = 0, nb = 0;
+  if (v < a.n_vertices) nb = edge_bands(a, v, first);
+  PolyEdge e;
+  if (nb) e = a.edges[v];
+  uint64_t todo = __ballot(nb > 0);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const uint32_t key = (uint32_t)__shfl((int)first, leader, 64);
+    const uint64_t m = __ballot(nb > 0 && first == key);
+    uint32_t old = 0;
+    if (lane == leader) old = atomicSub(&a.band_cnt[key], (uint32_t)__popcll(m));
+    old = (uint32_t)__shfl((int)old, leader, 64);
+    if ((m >> lane) & 1ull) {
+      const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      a.band_edges[a.band_off[key] + (old - (uint32_t)__popcll(m) + rank)] = e;
+    }
+    todo &= ~m;
+  }
+  for (uint32_t b = 1; b < nb; ++b) {
+    const uint32_t old = atomicSub(&a.band_cnt[first + b], 1u);
+    a.band_edges[a.band_off[first + b] + old - 1u] = e;
   }
 }
 

@@ -14,6 +14,7 @@
 
 #include "rz_internal.h"
 #include "rz_kernels.h"
+#include "rz_synth_job.h"
 
 // The kernels are compiled in this translation unit so every k_tile<T, FUN>
 // instantiation used by the dispatch below is emitted here.
@@ -69,8 +70,8 @@ struct rz_context {
   hipStream_t stream = nullptr;
   // workspace
   Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
-      vals_b, tile_off, temp, misc, jobtab, queue;
-  uint64_t* host_misc = nullptr;  // pinned: [0] total pairs, [1] status
+      vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges;
+  uint64_t* host_misc = nullptr;  // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
   // the end of its last upload); tab_dev / tab_dev_bytes say which device
   // buffer holds the staged bytes
@@ -91,7 +92,8 @@ struct rz_context {
   bool burns_pending = false;
   std::vector<Buf*> all() {
     return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
-            &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue};
+            &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue,
+            &band_n, &band_base, &band_cnt, &band_off, &band_edges};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -136,6 +138,7 @@ struct rz_device_geometry {
   int32_t* vert_ring = nullptr;
   int64_t* item_vbeg = nullptr;
   bool has_poly = false, has_line = false, has_point = false;
+  int64_t max_poly_nv = 0;  // most vertices of one polygon item
 };
 
 // ------------------------------------------------------------- contexts --
@@ -301,6 +304,8 @@ extern "C" rz_status rz_device_geometry_free(rz_device_geometry* dg) {
 
 extern "C" int64_t rz_device_geometry_count(const rz_device_geometry* dg) { return dg ? dg->G : 0; }
 
+static rz_status finish_geometry(rz_context* c, rz_device_geometry* dg, rz_device_geometry** out);
+
 extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry* g,
                                                rz_device_geometry** out) {
   rz_status st = rz::check_geometry(g);
@@ -319,6 +324,10 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
   for (int64_t i = 0; i < g->n_items; ++i) {
     int k = g->item_kind[i];
     if (g->item_ring_offsets[i + 1] == g->item_ring_offsets[i]) continue;
+    if (k == RZ_ITEM_POLYGONS)
+      dg->max_poly_nv = std::max<int64_t>(dg->max_poly_nv,
+                                          g->ring_vertex_offsets[g->item_ring_offsets[i + 1]] -
+                                              g->ring_vertex_offsets[g->item_ring_offsets[i]]);
     dg->has_poly |= k == RZ_ITEM_POLYGONS;
     dg->has_line |= k == RZ_ITEM_LINES;
     dg->has_point |= k == RZ_ITEM_POINTS;
@@ -343,6 +352,14 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
     rz_device_geometry_free(dg);
     return rz::fail(RZ_DEVICE_ERROR, std::string("geometry upload failed: ") + hipGetErrorString(e));
   }
+  return finish_geometry(c, dg, out);
+}
+
+// Derived maps of a resident geometry set whose offsets and coordinates are
+// on the device (or queued on the context's stream): item -> geometry,
+// ring -> item, item -> first vertex, vertex -> ring.  Frees dg on failure.
+static rz_status finish_geometry(rz_context* c, rz_device_geometry* dg, rz_device_geometry** out) {
+  hipError_t e = hipSuccess;
   if (dg->G)
     hipLaunchKernelGGL(k_item_geom, dim3(blocks_for(dg->G)), dim3(256), 0, c->stream,
                        dg->geom_item_off, dg->G, dg->item_geom);
@@ -362,6 +379,68 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
   }
   *out = dg;
   return RZ_OK;
+}
+
+// One thread per vertex of job k of the batched-workload generator: the
+// coordinates of rz_synth_job.h, bit-identical to the host generator.
+__global__ void k_synth_job_xy(uint64_t seed, uint64_t k, int32_t side, const int64_t* ring_voff,
+                               int64_t n_rings, int64_t V, double* xy) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  int64_t lo = 0, hi = n_rings - 1;  // last ring with ring_voff[r] <= v
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (ring_voff[mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  const rzsynth::PolySpec ps = rzsynth::poly_spec(seed, k, (uint64_t)lo, side);
+  double x, y;
+  rzsynth::vertex(seed, k, (uint64_t)lo, ps, (int32_t)(v - ring_voff[lo]), &x, &y);
+  xy[2 * v] = x;
+  xy[2 * v + 1] = y;
+}
+
+extern "C" rz_status rz_device_synth_job(rz_context* c, uint64_t seed, int64_t k,
+                                         rz_device_geometry** out) {
+  if (!c || !out || k < 0) return rz::fail(RZ_VALUE_ERROR, "bad argument to rz_device_synth_job");
+  HIP_TRY(hipSetDevice(c->device));
+  const rzsynth::JobSpec js = rzsynth::job_spec(seed, (uint64_t)k);
+  const int64_t G = js.n_poly;
+  std::vector<int64_t> seq((size_t)G + 1), voff((size_t)G + 1, 0);
+  std::vector<int32_t> kinds((size_t)G, RZ_ITEM_POLYGONS);
+  for (int64_t p = 0; p <= G; ++p) seq[(size_t)p] = p;
+  int64_t max_nv = 0;
+  for (int64_t p = 0; p < G; ++p) {
+    const int64_t nv = rzsynth::poly_spec(seed, (uint64_t)k, (uint64_t)p, js.side).n_vert + 1;
+    voff[(size_t)p + 1] = voff[(size_t)p] + nv;
+    max_nv = std::max(max_nv, nv);
+  }
+  auto* dg = new (std::nothrow) rz_device_geometry();
+  if (!dg) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
+  dg->ctx = c;
+  dg->G = dg->I = dg->R = G;
+  dg->V = voff[(size_t)G];
+  dg->has_poly = G > 0;
+  dg->max_poly_nv = max_nv;
+  hipError_t e = upload(c, &dg->geom_item_off, seq.data(), G + 1);
+  if (e == hipSuccess) e = upload(c, &dg->item_kind, kinds.data(), G);
+  if (e == hipSuccess) e = upload(c, &dg->item_ring_off, seq.data(), G + 1);
+  if (e == hipSuccess) e = upload(c, &dg->ring_voff, voff.data(), G + 1);
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->xy, std::max<size_t>(16, 16 * (size_t)dg->V));
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->item_geom, std::max<size_t>(4, 4 * (size_t)G));
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->ring_item, std::max<size_t>(4, 4 * (size_t)G));
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->vert_ring, std::max<size_t>(4, 4 * (size_t)dg->V));
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->item_vbeg, 8 * ((size_t)G + 1));
+  if (e != hipSuccess) {
+    rz_device_geometry_free(dg);
+    return rz::fail(RZ_DEVICE_ERROR, std::string("job geometry allocation failed: ") + hipGetErrorString(e));
+  }
+  if (dg->V > 0)
+    hipLaunchKernelGGL(k_synth_job_xy, dim3(blocks_for(dg->V)), dim3(256), 0, c->stream, seed,
+                       (uint64_t)k, js.side, (const int64_t*)dg->ring_voff, G, dg->V, dg->xy);
+  // the host vectors above must outlive the asynchronous uploads
+  rz_status st = finish_geometry(c, dg, out);
+  return st;
 }
 
 // ------------------------------------------------------------ rasterise --
@@ -505,7 +584,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   std::vector<Group> groups;
   int64_t V = 0, I = 0;
   uint64_t T = 0;
-  bool any_binned = false, any_poly = false, any_lp = false;
+  bool any_binned = false, any_poly = false, any_lp = false, any_band = false;
   for (size_t k = 0; k < n; ++k) {
     const rz_job& j = jobs_in[order[k]];
     const rz_device_geometry* dg = j.geometry;
@@ -561,6 +640,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     J.out = j.out_dev;
     const bool lines = has_lines(j);
     any_poly |= dg->has_poly;
+    any_band |= dg->has_poly && dg->max_poly_nv > kBandMinEdges;
     any_lp |= lines;
     if (groups.empty() || groups.back().dtype != j.dtype || groups.back().fun != (int)j.fun ||
         groups.back().lines != lines)
@@ -592,6 +672,10 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   HIP_TRY(ensure(c->item_box, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox)));
   HIP_TRY(ensure(c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
   HIP_TRY(ensure(c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
+  if (any_band) {
+    HIP_TRY(ensure(c->band_n, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->band_base, (size_t)(I + 1) * 4));
+  }
   if (any_binned) {
     HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
@@ -646,6 +730,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
   }
   if (any_binned) zero_words((uint32_t*)c->counts.p + I, 1, s);
+  if (any_band) zero_words((uint32_t*)c->band_n.p + I, 1, s);
   if (I > 0) {
     DescArgs da;
     da.jobs = d_jobs;
@@ -656,6 +741,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     da.item_last = (const unsigned long long*)c->item_last.p;
     da.desc = (ItemDesc*)c->desc.p;
     da.counts = any_binned ? (uint32_t*)c->counts.p : nullptr;
+    da.band_n = any_band ? (uint32_t*)c->band_n.p : nullptr;
     hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
@@ -664,8 +750,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   TileArgs ta;
   std::memset(&ta, 0, sizeof(ta));
   uint64_t n_pairs = 0;
+  // prefix sums of the tile counts (pairs) and band counts, then one host
+  // round trip for both totals
   if (any_binned) {
-    const uint64_t* d_total = (const uint64_t*)c->offsets.p + I;
     size_t tb = 0;
     HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->counts.p,
                                     (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
@@ -674,8 +761,65 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->counts.p,
                                     (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
                                     rocprim::plus<uint64_t>(), s));
-    HIP_TRY(hipMemcpyAsync(&c->host_misc[0], d_total, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpyAsync(&c->host_misc[0], (const uint64_t*)c->offsets.p + I, 8,
+                           hipMemcpyDeviceToHost, s));
+  }
+  if (any_band) {
+    size_t tb = 0;
+    HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->band_n.p,
+                                    (uint32_t*)c->band_base.p, 0u, (size_t)(I + 1),
+                                    rocprim::plus<uint32_t>(), s));
+    HIP_TRY(ensure(c->temp, tb));
+    HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->band_n.p,
+                                    (uint32_t*)c->band_base.p, 0u, (size_t)(I + 1),
+                                    rocprim::plus<uint32_t>(), s));
+    c->host_misc[2] = 0;  // the copy below fills its low 4 bytes
+    HIP_TRY(hipMemcpyAsync(&c->host_misc[2], (const uint32_t*)c->band_base.p + I, 4,
+                           hipMemcpyDeviceToHost, s));
+  }
+  if (any_binned || any_band) HIP_TRY(hipStreamSynchronize(s));
+  if (any_band) {
+    // edge bands of long polygon rings (see kBandMinEdges)
+    const uint64_t nb = (uint32_t)c->host_misc[2];
+    if (nb > 0) {
+      HIP_TRY(ensure(c->band_cnt, (nb + 1) * 4));
+      HIP_TRY(ensure(c->band_off, (nb + 1) * 8));
+      zero_words(c->band_cnt.p, nb + 1, s);
+      hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I)), dim3(256), 0, s, (ItemDesc*)c->desc.p,
+                         (const uint32_t*)c->band_n.p, (const uint32_t*)c->band_base.p, I);
+      BandArgs bd;
+      bd.jobs = d_jobs;
+      bd.job_vend = d_vend;
+      bd.n_jobs = (int)n;
+      bd.n_vertices = V;
+      bd.desc = (const ItemDesc*)c->desc.p;
+      bd.edges = (const PolyEdge*)c->edges.p;
+      bd.band_cnt = (uint32_t*)c->band_cnt.p;
+      bd.band_off = (const uint64_t*)c->band_off.p;
+      bd.band_edges = nullptr;
+      hipLaunchKernelGGL(k_band_count, dim3(blocks_for(V)), dim3(256), 0, s, bd);
+      size_t tb = 0;
+      HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->band_cnt.p,
+                                      (uint64_t*)c->band_off.p, (uint64_t)0, (size_t)(nb + 1),
+                                      rocprim::plus<uint64_t>(), s));
+      HIP_TRY(ensure(c->temp, tb));
+      HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->band_cnt.p,
+                                      (uint64_t*)c->band_off.p, (uint64_t)0, (size_t)(nb + 1),
+                                      rocprim::plus<uint64_t>(), s));
+      HIP_TRY(hipMemcpyAsync(&c->host_misc[3], (const uint64_t*)c->band_off.p + nb, 8,
+                             hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      const uint64_t ne = c->host_misc[3];
+      HIP_TRY(ensure(c->band_edges, std::max<uint64_t>(ne, 1) * sizeof(PolyEdge)));
+      bd.band_edges = (PolyEdge*)c->band_edges.p;
+      if (ne > 0) hipLaunchKernelGGL(k_band_scatter, dim3(blocks_for(V)), dim3(256), 0, s, bd);
+      ta.band_off = (const uint64_t*)c->band_off.p;
+      ta.band_edges = (const PolyEdge*)c->band_edges.p;
+    }
+  }
+  if (any_binned) {
+    const uint64_t* d_total = (const uint64_t*)c->offsets.p + I;
+    size_t tb = 0;
     n_pairs = c->host_misc[0];
     if (n_pairs >= (uint64_t)UINT32_MAX)
       return rz::fail(RZ_CAPACITY_ERROR, "more than 2^32 tile/item pairs in one call");

@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
   const RZ_CONST uint32_t* toff_c = as_const(a.tile_off);
   const RZ_CONST ItemDesc* pdesc_c = as_const(a.pair_desc);
   const RZ_CONST ItemDesc* desc_c = as_const(a.desc);
+  const RZ_CONST uint64_t* band_c = as_const(a.band_off);
   uint32_t pb = 0, pe = 0;
   auto fetch_range = [&](uint32_t tl) {
     if (a.tile_off) {
@@ -204,20 +205,29 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
       const T v = from_bits<T>(d.value);
       const int64_t vb = d.vbeg, ve = d.vbeg + d.nv;
       if (kind == 0) {
+        // the item's edges, or for a banded item only those of this tile row
+        const PolyEdge* ebase = a.edges + vb;
+        uint32_t n_e = (uint32_t)d.nv;
+        if (d.band) {
+          const uint32_t bi = d.band - 1u + (uint32_t)(ty - d.r0 / TILE_H);
+          const uint64_t b0 = band_c[bi];
+          ebase = a.band_edges + b0;
+          n_e = (uint32_t)(band_c[bi + 1u] - b0);
+        }
         // ---- polygon: even-odd coverage of every row from crossing counts
         uint64_t tog[NWORD];
 #pragma unroll
         for (int w = 0; w < NWORD; ++w) tog[w] = 0ull;
         uint32_t lp = 0;
         int32_t amax = -1;
-        for (int64_t cb = vb; cb < ve; cb += 64) {
+        for (uint32_t cb = 0; cb < n_e; cb += 64) {
           // stage the edges of this chunk that reach the strip, compacted
           // (two 16-byte vector loads and stores: no struct copy through scratch)
-          const int64_t e = cb + lane;
+          const uint32_t e = cb + (uint32_t)lane;
           uint4 e_lo = make_uint4(0u, 0u, 0u, 0u), e_hi = make_uint4(0u, 0u, 0u, 0u);
           bool act = false;
-          if (e < ve) {
-            const uint4* src = reinterpret_cast<const uint4*>(a.edges + e);
+          if (e < n_e) {
+            const uint4* src = reinterpret_cast<const uint4*>(ebase + e);
             e_lo = src[0];
             e_hi = src[1];  // dxdy, ystart (z), yend (w)
             const int32_t ys = (int32_t)e_hi.z, ye = (int32_t)e_hi.w;

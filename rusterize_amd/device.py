@@ -111,6 +111,18 @@ class DeviceGeometry:
         s = geom.as_struct()
         check(ctx.L.rz_device_geometry_upload(ctx.ptr, byref(s), byref(self.ptr)))
 
+    @classmethod
+    def synth_job(cls, ctx: Context, seed: int, k: int) -> "DeviceGeometry":
+        """Job k of the seeded job stream, generated on the device (rz_device_synth_job)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.ptr = c_void_p()
+        spec = _lib.job_spec(seed, k)
+        self.n_geoms = self.n_items = spec["n_poly"]
+        self.n_vertices = spec["n_vertices"]
+        check(ctx.L.rz_device_synth_job(ctx.ptr, seed, k, byref(self.ptr)))
+        return self
+
     def free(self) -> None:
         if self.ptr:
             self.ctx.L.rz_device_geometry_free(self.ptr)

@@ -188,3 +188,25 @@ def test_device_entry_points_fail_loudly_without_gpu():
     info = _lib.raster_info(g, shape=(4, 4))
     with pytest.raises(_lib.RusterizeNativeError, match="HIP|device"):
         _lib.rasterize(g, info, "float64", "last", None, 1, None, 1, 0.0)
+
+
+def test_job_stream_generator():
+    # the seeded job stream: spec sizes match the generated geometry, rings are
+    # closed stars around their centre, and the distribution bounds hold
+    for k in (0, 15, 20, 21, 31, 100):
+        sp = _lib.job_spec(4, k)
+        assert 256 <= sp["side"] <= 8192 and 10 <= sp["n_poly"] <= 50000 and sp["fun"] in _lib.FUNS
+        if sp["n_vertices"] > 300000:
+            continue
+        g = Geometry.synth_job(4, k)
+        assert g.n_geoms == sp["n_poly"] and g.n_vertices == sp["n_vertices"]
+        rvo = g.ring_vertex_offsets
+        n = np.diff(rvo) - 1
+        assert n.min() >= 4 and n.max() <= 4096
+        first, last = g.xy[rvo[:-1]], g.xy[rvo[1:] - 1]
+        assert np.array_equal(first, last)  # closed rings
+        f = _lib.job_field(4, k)
+        assert f.shape == (sp["n_poly"],) and f.min() >= 0 and f.max() < 1
+    # determinism across calls
+    g = Geometry.synth_job(4, 21)
+    assert np.array_equal(Geometry.synth_job(4, 21).xy, g.xy)

@@ -211,3 +211,20 @@ def test_config3_reduced():
     for fun in ["max", "sum"]:
         got, ref = run_both(g, info, "float32", fun, field=field, bg=float("nan"))
         assert_same(got, ref, f"config3 {fun}")
+
+
+@pytest.mark.parametrize("binning", [None, "always", "never"])
+def test_long_rings_edge_bands(binning, monkeypatch):
+    # many-vertex rings spanning many tile rows take the banded-edge path
+    # (edges binned by 64-row tile row); overlapping stars, holes and parts
+    if binning:
+        monkeypatch.setenv("RUSTERIZE_BINNING", binning)
+    stars = Geometry.synth(4, 51, 150, 1500.0, 1000.0, 20.0, 450.0, 65.0, 3000.0)
+    holed = Geometry.synth(2, 52, 40, 1500.0, 1000.0, 300.0, 900.0, 0.5, 1.0)
+    g = concat([stars, holed])
+    info = _lib.raster_info(None, shape=(1000, 1500), extent=(0, 0, 1500, 1000))
+    for dtype, fun, bg in [("float64", "sum", float("nan")), ("int32", "last", 0), ("uint8", "count", 0),
+                           ("float32", "min", float("nan"))]:
+        field = field_for(dtype, g.n_geoms, seed=7)
+        got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+        assert_same(got, ref, f"bands {binning} {dtype} {fun}")

@@ -100,6 +100,15 @@ __device__ inline int find_job(P ends, int n, int64_t x) {
   return lo;
 }
 
+// find_job for one index per lane of a wave whose indices ascend with the
+// lane: one (scalar) search for the wave's first active lane, and a search of
+// its own only in a lane past that job's end (rare: jobs are long).
+__device__ inline int find_job_wave(const int64_t* ends, int n, int64_t x) {
+  const int64_t x0 = __builtin_amdgcn_readfirstlane(x);
+  const int j0 = find_job(ends, n, x0);
+  return x < ends[j0] ? j0 : find_job(ends, n, x);
+}
+
 struct PrepArgs {
   const JobDev* jobs;
   const int64_t* job_vend;  // [n_jobs] prefix ends of the vertex ranges
@@ -110,6 +119,7 @@ struct PrepArgs {
   uint8_t* vflags;          // [n_vertices]
   ItemBox* item_box;        // [n_items]
   unsigned long long* item_last;  // [n_items] last kept line segment + 1 (global; 0: none)
+  uint32_t* vert_item;      // [n_vertices] global item of every vertex, or null
 };
 
 // Per-call item descriptor (64 B, one scalar load): everything the tile
@@ -128,13 +138,15 @@ struct __align__(64) ItemDesc {
 };
 
 // Edge bands: a polygon item with more than kBandMinEdges edges that spans
-// two or more 64-row tile rows gets its edges binned by tile row (an edge
-// goes to every tile row its rows [ystart, yend) meet), so a strip visits
-// only the edges of its own tile row instead of the whole ring set.  Band
-// entry band - 1 + (ty - r0 / TILE_H) of such an item holds the edges of
-// tile row ty in band_edges[band_off[e] .. band_off[e + 1]) (any order: the
-// crossing toggles and the maximum commute).
-constexpr int kBandMinEdges = 256;
+// two or more 16-row strips gets its edges binned by strip (an edge goes to
+// every strip its rows [ystart, yend) meet), so a strip visits only its own
+// edges instead of the whole ring set.  Band entry
+// band - 1 + (Rw / kBandRows - r0 / kBandRows) of such an item holds the
+// edges of the strip starting at row Rw in
+// band_edges[band_off[e] .. band_off[e + 1]) (any order: the crossing
+// toggles and the maximum commute).
+constexpr int kBandMinEdges = 128;
+constexpr int kBandRows = WAVE_ROWS;
 
 struct DescArgs {
   const JobDev* jobs;
@@ -149,9 +161,7 @@ struct DescArgs {
 };
 
 struct BandArgs {
-  const JobDev* jobs;
-  const int64_t* job_vend;
-  int n_jobs;
+  const uint32_t* vert_item;  // [n_vertices] global item of every vertex
   int64_t n_vertices;
   const ItemDesc* desc;     // [n_items], band fields assigned
   const PolyEdge* edges;    // [n_vertices]
@@ -188,6 +198,11 @@ struct TileArgs {
   const ItemDesc* pair_desc;  // descriptors in pair order
   const uint64_t* band_off;   // edge bands of banded polygon items
   const PolyEdge* band_edges;
+  // heavy tiles: heavy_base[t] .. heavy_base[t + 1] index the tile's items in
+  // the pre-pass masks (null: no heavy tiles); slot_words u64 per strip mask
+  const uint64_t* heavy_base;
+  const uint64_t* masks;
+  uint32_t slot_words;
   // scan jobs: every item of the job is tested against the tile
   const ItemDesc* desc;
   const PolyEdge* edges;
@@ -198,6 +213,26 @@ struct TileArgs {
 };
 
 constexpr int kBurnSlots = 1024;
+
+// Items per tile from which a tile's polygon coverage is precomputed by the
+// k_cover pre-pass (see rz_tile.hip).
+constexpr uint32_t kHeavyItems = 32;
+
+struct CoverArgs {
+  const JobDev* jobs;
+  const int64_t* job_tend;
+  int n_jobs;
+  const uint32_t* pair_tile;    // [n_pairs] tile of every pair (sorted keys)
+  const ItemDesc* pair_desc;    // [n_pairs]
+  const uint64_t* heavy_base;   // [n_tiles + 1]
+  const uint32_t* heavy_pairs;  // [heavy pairs] pair index
+  uint32_t n_tiles;
+  const PolyEdge* edges;
+  const uint64_t* band_off;
+  const PolyEdge* band_edges;
+  uint64_t* masks;              // [heavy pairs][4 strips][slot_words]
+  uint32_t slot_words;
+};
 
 // ------------------------------------------------ polygon row coverage --
 //

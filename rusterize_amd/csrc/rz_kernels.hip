@@ -124,11 +124,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   int64_t item = -1;
   ItemBox box{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
   if (active) {
-    const JobDev& J = a.jobs[find_job(a.job_vend, a.n_jobs, v)];
+    const JobDev& J = a.jobs[find_job_wave(a.job_vend, a.n_jobs, v)];
     const int64_t lv = v - J.v_base;
     const int32_t ring = J.vert_ring[lv];
     const int32_t litem = J.ring_item[ring];
     item = J.i_base + litem;
+    if (a.vert_item) a.vert_item[v] = (uint32_t)item;
     const int32_t kind = J.item_kind[litem];
     const int64_t rbeg = J.ring_voff[ring], rend = J.ring_voff[ring + 1];
     const bool has_next = lv + 1 < rend;
@@ -224,7 +225,7 @@ __device__ inline bool tile_rect(int32_t r0, int32_t r1, int32_t c0, int32_t c1,
 __global__ void k_item_desc(DescArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_items) return;
-  const int j = find_job(a.job_iend, a.n_jobs, i);
+  const int j = find_job_wave(a.job_iend, a.n_jobs, i);
   const JobDev& J = a.jobs[j];
   const int64_t li = i - J.i_base;
   const ItemBox b = a.item_box[i];
@@ -252,8 +253,8 @@ __global__ void k_item_desc(DescArgs a) {
   a.desc[i] = d;
   if (a.band_n) {
     const bool banded = J.item_kind[li] == 0 && d.nv > kBandMinEdges && d.r0 < d.r1 &&
-                        (d.r1 - 1) / TILE_H > d.r0 / TILE_H;
-    a.band_n[i] = banded ? (uint32_t)((d.r1 - 1) / TILE_H - d.r0 / TILE_H + 1) : 0u;
+                        (d.r1 - 1) / kBandRows > d.r0 / kBandRows;
+    a.band_n[i] = banded ? (uint32_t)((d.r1 - 1) / kBandRows - d.r0 / kBandRows + 1) : 0u;
   }
   if (a.counts) {
     int32_t tx0, tx1, ty0, ty1;
@@ -275,16 +276,15 @@ __global__ void k_band_assign(ItemDesc* desc, const uint32_t* band_n, const uint
 // The band entry of edge v's first tile row and the number of tile rows its
 // rows meet (0: the edge is not in a banded item or is empty).
 __device__ inline uint32_t edge_bands(const BandArgs& a, int64_t v, uint32_t& first) {
-  const JobDev& J = a.jobs[find_job(a.job_vend, a.n_jobs, v)];
-  const int64_t lv = v - J.v_base;
-  const ItemDesc& d = a.desc[J.i_base + J.ring_item[J.vert_ring[lv]]];
+  const ItemDesc& d = a.desc[a.vert_item[v]];
   if (!d.band) return 0u;
   const PolyEdge& e = a.edges[v];
   if (!(e.ystart < e.yend)) return 0u;
-  const int32_t ty0 = d.r0 / TILE_H;
-  const int32_t b0 = max(e.ystart / TILE_H, ty0), b1 = min((e.yend - 1) / TILE_H, (d.r1 - 1) / TILE_H);
+  const int32_t sr0 = d.r0 / kBandRows;
+  const int32_t b0 = max(e.ystart / kBandRows, sr0),
+                b1 = min((e.yend - 1) / kBandRows, (d.r1 - 1) / kBandRows);
   if (b1 < b0) return 0u;
-  first = d.band - 1u + (uint32_t)(b0 - ty0);
+  first = d.band - 1u + (uint32_t)(b0 - sr0);
   return (uint32_t)(b1 - b0 + 1);
 }
 

@@ -33,8 +33,8 @@ using namespace rzk;
 
 namespace {
 
-constexpr int kStages = 4;
-const char* kStageNames[kStages] = {"prep", "bin", "sort", "tile"};
+constexpr int kStages = 5;
+const char* kStageNames[kStages] = {"prep", "bin", "sort", "cover", "tile"};
 
 struct Buf {
   void* p = nullptr;
@@ -70,7 +70,8 @@ struct rz_context {
   hipStream_t stream = nullptr;
   // workspace
   Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
-      vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges;
+      vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
+      heavy_cnt, heavy_base, heavy_pairs, masks, vert_item;
   uint64_t* host_misc = nullptr;  // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
   // the end of its last upload); tab_dev / tab_dev_bytes say which device
@@ -93,7 +94,8 @@ struct rz_context {
   std::vector<Buf*> all() {
     return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
             &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue,
-            &band_n, &band_base, &band_cnt, &band_off, &band_edges};
+            &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
+            &heavy_pairs, &masks, &vert_item};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -585,6 +587,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   int64_t V = 0, I = 0;
   uint64_t T = 0;
   bool any_binned = false, any_poly = false, any_lp = false, any_band = false;
+  bool any_tw64 = false, any_tw256 = false;
+  int max_tile_w = 64;
   for (size_t k = 0; k < n; ++k) {
     const rz_job& j = jobs_in[order[k]];
     const rz_device_geometry* dg = j.geometry;
@@ -628,7 +632,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     // about eight small launches, so a single call scans up to ~4M
     // item-tile tests; in a batch that cost is shared by every job.
     const uint64_t scan_limit = n_in == 1 ? ((uint64_t)1 << 22) : ((uint64_t)1 << 16);
-    bool scan = ni <= 64 || ni * std::max<uint64_t>(jt, 1) <= scan_limit;
+    bool scan = ni <= 64 || (ni <= 2048 && ni * std::max<uint64_t>(jt, 1) <= scan_limit);
     if (binning == 1) scan = ni == 0;
     else if (binning == -1) scan = true;
     J.scan = scan ? 1 : 0;
@@ -639,6 +643,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     J.layer_of_geom = j.layer_dev;
     J.out = j.out_dev;
     const bool lines = has_lines(j);
+    any_tw64 |= TW == 64;
+    any_tw256 |= TW == 256;
+    max_tile_w = std::max(max_tile_w, TW);
     any_poly |= dg->has_poly;
     any_band |= dg->has_poly && dg->max_poly_nv > kBandMinEdges;
     any_lp |= lines;
@@ -673,6 +680,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   HIP_TRY(ensure(c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
   HIP_TRY(ensure(c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
   if (any_band) {
+    HIP_TRY(ensure(c->vert_item, (size_t)std::max<int64_t>(V, 1) * 4));
     HIP_TRY(ensure(c->band_n, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->band_base, (size_t)(I + 1) * 4));
   }
@@ -727,6 +735,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     pa.vflags = (uint8_t*)c->vflags.p;
     pa.item_box = (ItemBox*)c->item_box.p;
     pa.item_last = (unsigned long long*)c->item_last.p;
+    pa.vert_item = any_band ? (uint32_t*)c->vert_item.p : nullptr;
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
   }
   if (any_binned) zero_words((uint32_t*)c->counts.p + I, 1, s);
@@ -788,9 +797,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
       hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I)), dim3(256), 0, s, (ItemDesc*)c->desc.p,
                          (const uint32_t*)c->band_n.p, (const uint32_t*)c->band_base.p, I);
       BandArgs bd;
-      bd.jobs = d_jobs;
-      bd.job_vend = d_vend;
-      bd.n_jobs = (int)n;
+      bd.vert_item = (const uint32_t*)c->vert_item.p;
       bd.n_vertices = V;
       bd.desc = (const ItemDesc*)c->desc.p;
       bd.edges = (const PolyEdge*)c->edges.p;
@@ -863,10 +870,58 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
                          n_pairs, (ItemDesc*)c->pair_desc.p);
     ta.tile_off = (const uint32_t*)c->tile_off.p;
     ta.pair_desc = (const ItemDesc*)c->pair_desc.p;
+    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
+
+    // heavy tiles (kHeavyItems items or more): coverage masks precomputed in
+    // parallel, one wave per (item, strip); the buffers are sized for every
+    // pair so no host round trip is needed
+    if (n_pairs >= kHeavyItems) {
+      HIP_TRY(ensure(c->heavy_cnt, ((size_t)n_tiles + 1) * 4));
+      HIP_TRY(ensure(c->heavy_base, ((size_t)n_tiles + 1) * 8));
+      hipLaunchKernelGGL(k_heavy_count, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
+                         (const uint32_t*)c->tile_off.p, n_tiles, kHeavyItems,
+                         (uint32_t*)c->heavy_cnt.p);
+      tb = 0;
+      HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->heavy_cnt.p,
+                                      (uint64_t*)c->heavy_base.p, (uint64_t)0, (size_t)n_tiles + 1,
+                                      rocprim::plus<uint64_t>(), s));
+      HIP_TRY(ensure(c->temp, tb));
+      HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->heavy_cnt.p,
+                                      (uint64_t*)c->heavy_base.p, (uint64_t)0, (size_t)n_tiles + 1,
+                                      rocprim::plus<uint64_t>(), s));
+      const uint32_t slot_words = 16u * (uint32_t)(max_tile_w / 64);
+      HIP_TRY(ensure(c->heavy_pairs, (size_t)n_pairs * 4));
+      HIP_TRY(ensure(c->masks, (size_t)n_pairs * 4 * slot_words * 8));
+      hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
+                         (const uint32_t*)c->keys_b.p, d_total, n_pairs,
+                         (const uint32_t*)c->tile_off.p, (const uint64_t*)c->heavy_base.p,
+                         (uint32_t*)c->heavy_pairs.p);
+      CoverArgs ca;
+      ca.jobs = d_jobs;
+      ca.job_tend = d_tend;
+      ca.n_jobs = (int)n;
+      ca.pair_tile = (const uint32_t*)c->keys_b.p;
+      ca.pair_desc = (const ItemDesc*)c->pair_desc.p;
+      ca.heavy_base = (const uint64_t*)c->heavy_base.p;
+      ca.heavy_pairs = (const uint32_t*)c->heavy_pairs.p;
+      ca.n_tiles = n_tiles;
+      ca.edges = (const PolyEdge*)c->edges.p;
+      ca.band_off = ta.band_off;
+      ca.band_edges = ta.band_edges;
+      ca.masks = (uint64_t*)c->masks.p;
+      ca.slot_words = slot_words;
+      const uint32_t cgrid = (uint32_t)std::min<uint64_t>(n_pairs, 2048);
+      if (any_tw64) hipLaunchKernelGGL(k_cover<1>, dim3(cgrid), dim3(256), 0, s, ca);
+      if (any_tw256) hipLaunchKernelGGL(k_cover<4>, dim3(cgrid), dim3(256), 0, s, ca);
+      ta.heavy_base = (const uint64_t*)c->heavy_base.p;
+      ta.masks = (const uint64_t*)c->masks.p;
+      ta.slot_words = slot_words;
+    }
   } else if (c->profiling) {
     HIP_TRY(hipEventRecord(c->ev[2], s));
+    HIP_TRY(hipEventRecord(c->ev[3], s));
   }
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
 
   // 3. tiles: one persistent work-queue launch per group
   zero_words(c->queue.p, queue_words, s);
@@ -887,7 +942,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     launch_tile_dispatch(groups[g].dtype, groups[g].fun, ta, groups[g].lines, s);
   }
   HIP_TRY(hipGetLastError());
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[5], s));
   if (c->profiling) c->accumulate_times();
   c->burns_pending = c->profiling > 1;
   c->last_pairs = n_pairs;

@@ -45,6 +45,62 @@ __device__ inline uint32_t xcd_unit_lo(uint32_t x, uint32_t n) {
   return x * per + min(x, rem);
 }
 
+// Coverage of one polygon item over the 16 rows [Rw, Rw + 16) (rows from
+// strip_r1 on excluded) and the columns [C0, C0 + 64 * NWORD): lane
+// (rl = lane / 4, q = lane % 4) returns word q of row Rw + rl's mask (lanes
+// with q >= NWORD return 0).  Lane layout: the four lanes of a row take every
+// 4th active edge; their crossing toggles are XOR-combined with DPP and the
+// row mask follows from the note above row_coverage_word in rz_kernels.h.
+// `stage` is 64 edges of wave-private LDS.
+template <int NWORD>
+__device__ inline uint64_t strip_coverage(const PolyEdge* ebase, uint32_t n_e, int32_t Rw,
+                                          int32_t strip_r1, int32_t C0, double ncols_f,
+                                          PolyEdge* stage, int lane) {
+  const int rl = lane >> 2, q = lane & 3;
+  const int32_t r = Rw + rl;
+  const bool row_ok = r < strip_r1;
+  uint64_t tog[NWORD];
+#pragma unroll
+  for (int w = 0; w < NWORD; ++w) tog[w] = 0ull;
+  uint32_t lp = 0;
+  int32_t amax = -1;
+  for (uint32_t cb = 0; cb < n_e; cb += 64) {
+    // stage the edges of this chunk that reach the strip, compacted
+    // (two 16-byte vector loads and stores: no struct copy through scratch)
+    const uint32_t e = cb + (uint32_t)lane;
+    uint4 e_lo = make_uint4(0u, 0u, 0u, 0u), e_hi = make_uint4(0u, 0u, 0u, 0u);
+    bool act = false;
+    if (e < n_e) {
+      const uint4* src = reinterpret_cast<const uint4*>(ebase + e);
+      e_lo = src[0];
+      e_hi = src[1];  // dxdy, ystart (z), yend (w)
+      const int32_t ys = (int32_t)e_hi.z, ye = (int32_t)e_hi.w;
+      act = ys < strip_r1 && ye > Rw && ys < ye;
+    }
+    const uint64_t m = __ballot(act);
+    if (act) {
+      uint4* dst = reinterpret_cast<uint4*>(stage + __popcll(m & ((1ull << lane) - 1ull)));
+      dst[0] = e_lo;
+      dst[1] = e_hi;
+    }
+    wave_sync();
+    const int n = __popcll(m);
+    for (int k0 = 0; k0 < n; k0 += 4) {  // uniform trip count, predicated lanes
+      const int k = k0 + q;
+      const PolyEdge ee = stage[k < n ? k : n - 1];
+      const bool on = k < n && row_ok && ee.ystart <= r && r < ee.yend;
+      const double x = crossing(ee, r);
+      toggle_crossing_if<NWORD>(on && !is_pos_nan(x), span_col(x, ncols_f), C0, tog, lp, amax);
+    }
+    wave_sync();
+  }
+  lp = quad_xor_all(lp);
+  amax = quad_max_all(amax);
+#pragma unroll
+  for (int w = 0; w < NWORD; ++w) tog[w] = quad_xor_all64(tog[w]);
+  return q < NWORD ? row_coverage_word<NWORD>(tog, lp, amax, C0, q) : 0ull;
+}
+
 // Persistent work-queue tile kernel, one launch per pixel-function group of
 // a batch (a single call is a batch of one job).
 //
@@ -160,6 +216,8 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
   const RZ_CONST ItemDesc* pdesc_c = as_const(a.pair_desc);
   const RZ_CONST ItemDesc* desc_c = as_const(a.desc);
   const RZ_CONST uint64_t* band_c = as_const(a.band_off);
+  const RZ_CONST uint64_t* heavy_c = as_const(a.heavy_base);
+  const RZ_CONST uint64_t* mask_c = as_const(a.masks);
   uint32_t pb = 0, pe = 0;
   auto fetch_range = [&](uint32_t tl) {
     if (a.tile_off) {
@@ -193,72 +251,43 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     const int32_t tile_c1 = min(C0 + TW, ncols);
     const bool row_ok = r < strip_r1;
     const bool dedup_job = J.dedup_lines != 0;
+    // heavy tile (many items): every polygon mask comes from the pre-pass
+    uint64_t hbase = 0;
+    bool hb_ok = false;
+    if (a.heavy_base && !J.scan) {
+      hbase = heavy_c[t];
+      hb_ok = heavy_c[t + 1u] > hbase;
+    }
 
     T acc[NPIX];  // acc[k * PPC + j]: row srow + k*RSTEP, column ccol + j
 #pragma unroll
     for (int j = 0; j < NPIX; ++j) acc[j] = bg;
 
-    auto process_item = [&](const ItemDesc d) {
+    auto process_item = [&](const ItemDesc d, uint32_t pi) {
       // wave-uniform skip: the item box misses this strip
       if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) return;
       const int32_t kind = (int32_t)(d.kind_layer & 3u);
       const T v = from_bits<T>(d.value);
       const int64_t vb = d.vbeg, ve = d.vbeg + d.nv;
       if (kind == 0) {
-        // the item's edges, or for a banded item only those of this tile row
-        const PolyEdge* ebase = a.edges + vb;
-        uint32_t n_e = (uint32_t)d.nv;
-        if (d.band) {
-          const uint32_t bi = d.band - 1u + (uint32_t)(ty - d.r0 / TILE_H);
-          const uint64_t b0 = band_c[bi];
-          ebase = a.band_edges + b0;
-          n_e = (uint32_t)(band_c[bi + 1u] - b0);
+        uint64_t cov = 0;
+        if (hb_ok) {  // heavy tile: the coverage pre-pass wrote this mask
+          if (q < NWORD)
+            cov = mask_c[(hbase + (pi - pb)) * (4u * a.slot_words) + (uint32_t)strip * a.slot_words +
+                         (uint32_t)(rl * NWORD + q)];
+        } else {
+          // the item's edges, or for a banded item only those of this tile row
+          const PolyEdge* ebase = a.edges + vb;
+          uint32_t n_e = (uint32_t)d.nv;
+          if (d.band) {
+            const uint32_t bi = d.band - 1u + (uint32_t)(Rw / kBandRows - d.r0 / kBandRows);
+            const uint64_t b0 = band_c[bi];
+            ebase = a.band_edges + b0;
+            n_e = (uint32_t)(band_c[bi + 1u] - b0);
+          }
+          cov = strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, ncols_f, stage, lane);
         }
-        // ---- polygon: even-odd coverage of every row from crossing counts
-        uint64_t tog[NWORD];
-#pragma unroll
-        for (int w = 0; w < NWORD; ++w) tog[w] = 0ull;
-        uint32_t lp = 0;
-        int32_t amax = -1;
-        for (uint32_t cb = 0; cb < n_e; cb += 64) {
-          // stage the edges of this chunk that reach the strip, compacted
-          // (two 16-byte vector loads and stores: no struct copy through scratch)
-          const uint32_t e = cb + (uint32_t)lane;
-          uint4 e_lo = make_uint4(0u, 0u, 0u, 0u), e_hi = make_uint4(0u, 0u, 0u, 0u);
-          bool act = false;
-          if (e < n_e) {
-            const uint4* src = reinterpret_cast<const uint4*>(ebase + e);
-            e_lo = src[0];
-            e_hi = src[1];  // dxdy, ystart (z), yend (w)
-            const int32_t ys = (int32_t)e_hi.z, ye = (int32_t)e_hi.w;
-            act = ys < strip_r1 && ye > Rw && ys < ye;
-          }
-          const uint64_t m = __ballot(act);
-          if (act) {
-            uint4* dst = reinterpret_cast<uint4*>(stage + __popcll(m & ((1ull << lane) - 1ull)));
-            dst[0] = e_lo;
-            dst[1] = e_hi;
-          }
-          wave_sync();
-          const int n = __popcll(m);
-          for (int k0 = 0; k0 < n; k0 += 4) {  // uniform trip count, predicated lanes
-            const int k = k0 + q;
-            const PolyEdge ee = stage[k < n ? k : n - 1];
-            const bool on = k < n && row_ok && ee.ystart <= r && r < ee.yend;
-            const double x = crossing(ee, r);
-            toggle_crossing_if<NWORD>(on && !is_pos_nan(x), span_col(x, ncols_f), C0, tog, lp,
-                                      amax);
-          }
-          wave_sync();
-        }
-        // combine the four lanes of this row, then publish the row's coverage
-        // mask (the note above row_coverage_word in rz_kernels.h states why it
-        // equals the reference's spans) for the store layout
-        lp = quad_xor_all(lp);
-        amax = quad_max_all(amax);
-#pragma unroll
-        for (int w = 0; w < NWORD; ++w) tog[w] = quad_xor_all64(tog[w]);
-        if (q < NWORD) s_mask[wave][mbuf][rl][q] = row_coverage_word<NWORD>(tog, lp, amax, C0, q);
+        if (q < NWORD) s_mask[wave][mbuf][rl][q] = cov;
         wave_sync();
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
@@ -348,7 +377,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     };
 
     if (!J.scan) {
-      for (uint32_t i = pb; i < pe; ++i) process_item(load_desc(pdesc_c + i));
+      for (uint32_t i = pb; i < pe; ++i) process_item(load_desc(pdesc_c + i), i);
     } else {
       // scan job: test every item's box in order (small item counts)
       const int64_t ib = J.i_base, ie = J.i_base + J.ni;
@@ -364,7 +393,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         while (m) {
           const int bit = __builtin_ctzll(m);
           m &= m - 1;
-          process_item(load_desc(desc_c + base + bit));
+          process_item(load_desc(desc_c + base + bit), 0u);
         }
       }
     }
@@ -400,6 +429,75 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) burns += __shfl_xor(burns, off, 64);
     if (lane == 0 && burns) atomicAdd(&a.burns[blockIdx.x % kBurnSlots], burns);
+  }
+}
+
+// ---------------------------------------------------------- heavy tiles --
+//
+// A tile with at least kHeavyItems items would serialise all their coverage
+// work in its four strip waves.  For such tiles the coverage masks of every
+// (polygon item, strip) are computed beforehand by k_cover, one wave per
+// (item, strip), in parallel; the tile kernel then only folds the masks in
+// geometry order (every pixel function, ordered sum included, stays exact).
+
+__global__ void k_heavy_count(const uint32_t* tile_off, uint32_t n_tiles, uint32_t min_items,
+                              uint32_t* heavy_cnt) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_tiles) return;
+  uint32_t c = 0;
+  if (t < n_tiles) {
+    c = tile_off[t + 1] - tile_off[t];
+    if (c < min_items) c = 0;
+  }
+  heavy_cnt[t] = c;
+}
+
+__global__ void k_heavy_pairs(const uint32_t* pair_tile, const uint64_t* n_pairs, uint64_t cap,
+                              const uint32_t* tile_off, const uint64_t* heavy_base,
+                              uint32_t* heavy_pairs) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= min(*n_pairs, cap)) return;
+  const uint32_t t = pair_tile[i];
+  const uint64_t hb = heavy_base[t];
+  if (heavy_base[t + 1] > hb) heavy_pairs[hb + (i - tile_off[t])] = (uint32_t)i;
+}
+
+template <int NWORD>
+__global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
+  constexpr int TW = 64 * NWORD;
+  __shared__ PolyEdge s_edge[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
+  const RZ_CONST uint64_t* band_c = as_const(a.band_off);
+  const uint64_t units = 4ull * a.heavy_base[a.n_tiles];
+  for (uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave; u < units; u += (uint64_t)gridDim.x * 4u) {
+    const uint64_t hp = u >> 2;
+    const int32_t strip = (int32_t)(u & 3u);
+    const uint32_t i = a.heavy_pairs[hp];
+    const uint32_t t = a.pair_tile[i];
+    const ItemDesc d = load_desc(as_const(a.pair_desc) + i);
+    const RZ_CONST JobDev& J = jobs_c[d.job];
+    if (J.tile_w != TW) continue;
+    if ((d.kind_layer & 3u) != 0u) continue;
+    const uint32_t tj = t - J.tile_base;
+    const uint32_t layer = tj / J.tiles_per_layer;
+    const uint32_t tl = tj - layer * J.tiles_per_layer;
+    const int32_t ty = (int32_t)(tl / (uint32_t)J.ntx), tx = (int32_t)(tl % (uint32_t)J.ntx);
+    const int32_t C0 = tx * TW, Rw = ty * TILE_H + strip * WAVE_ROWS;
+    const int32_t strip_r1 = min(Rw + WAVE_ROWS, J.nrows), tile_c1 = min(C0 + TW, J.ncols);
+    if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) continue;
+    const PolyEdge* ebase = a.edges + d.vbeg;
+    uint32_t n_e = (uint32_t)d.nv;
+    if (d.band) {
+      const uint32_t bi = d.band - 1u + (uint32_t)(Rw / kBandRows - d.r0 / kBandRows);
+      const uint64_t b0 = band_c[bi];
+      ebase = a.band_edges + b0;
+      n_e = (uint32_t)(band_c[bi + 1u] - b0);
+    }
+    const uint64_t cov =
+        strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, (double)J.ncols, s_edge[wave], lane);
+    if ((lane & 3) < NWORD)
+      a.masks[hp * 4u * a.slot_words + (uint64_t)strip * a.slot_words + (uint64_t)((lane >> 2) * NWORD + (lane & 3))] = cov;
   }
 }
 

@@ -228,3 +228,17 @@ def test_long_rings_edge_bands(binning, monkeypatch):
         field = field_for(dtype, g.n_geoms, seed=7)
         got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
         assert_same(got, ref, f"bands {binning} {dtype} {fun}")
+
+
+def test_heavy_tiles_precomputed_coverage():
+    # thousands of overlapping polygons on a small raster: every tile holds
+    # more than 32 items, so polygon coverage comes from the parallel pre-pass
+    dense = Geometry.synth(4, 61, 3000, 320.0, 300.0, 2.0, 40.0, 4.0, 600.0)
+    lines = Geometry.synth(3, 62, 0, 320.0, 300.0, 200, 300, 20)
+    g = concat([dense, lines])
+    info = _lib.raster_info(None, shape=(300, 320), extent=(0, 0, 320, 300))
+    for dtype, fun, bg in [("float64", "sum", float("nan")), ("int32", "last", 0), ("uint8", "count", 0),
+                           ("float32", "max", float("nan")), ("int16", "first", 0), ("uint32", "min", 0)]:
+        field = field_for(dtype, g.n_geoms, seed=8)
+        got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+        assert_same(got, ref, f"heavy {dtype} {fun}")

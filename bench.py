@@ -182,7 +182,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines", "batch"])
+    ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines", "batch", "serve"])
+    ap.add_argument("--rates", default="50,200,500,1000,2000", help="serve workload: arrival rates (requests/s)")
+    ap.add_argument("--requests", type=int, default=200, help="serve workload: requests per rate")
+    ap.add_argument("--pool", type=int, default=48, help="serve workload: distinct geometry sets (stream jobs)")
     ap.add_argument("--jobs", type=int, default=256, help="batch workload: jobs of the seeded job stream")
     ap.add_argument("--budget-gb", type=float, default=150.0,
                     help="batch workload: device memory the jobs' inputs, outputs and workspace may use")
@@ -203,6 +206,8 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return batch_bench(args)
+    if args.workload == "serve":
+        return serve_bench(args)
     steps = args.steps if args.steps is not None else {"polygons": 3, "stars": 200, "lines": 5}[args.workload]
 
     L = _lib.lib()
@@ -475,6 +480,140 @@ def batch_bench(args) -> int:
     }
     print(json.dumps(result), flush=True)
     return 0 if same else 1
+
+
+# ------------------------------------------------------------------ serving
+
+
+def serve_bench(args) -> int:
+    """The served workload: Poisson arrivals at each rate, requests drawn from
+    the seeded job stream (a pool of geometry sets generated on the device);
+    30 % of requests reuse a cached geometry set with a new extent, field and
+    function.  Served = rz_server with continuous batching over 2 streams
+    under a 200 GB budget; plain = the same arrivals served one call at a time
+    (1 stream, batches of 1).  Latency is submission to completion, measured
+    by the server."""
+    import threading
+
+    from rusterize_amd.device import Context, DeviceBuffer, DeviceGeometry, Job, Server, grid, rasterize_device
+
+    rates = [float(x) for x in args.rates.split(",")]
+    ctx = Context(0)
+    pool = []
+    for k in range(args.pool):
+        sp = _lib.job_spec(BATCH_SEED, k)
+        dg = DeviceGeometry.synth_job(ctx, BATCH_SEED, k)
+        f0 = _lib.job_field(BATCH_SEED, k)
+        alt = np.random.default_rng(k).random(f0.size)
+        pool.append((sp, dg, DeviceBuffer.from_array(ctx, f0), DeviceBuffer.from_array(ctx, alt)))
+    ctx.synchronize()
+    log(f"serve: pool of {len(pool)} geometry sets "
+        f"({sum(p[0]['n_vertices'] for p in pool)} vertices), rates {rates}, {args.requests} requests each")
+
+    def schedule(rate, seed):
+        rng = np.random.default_rng(seed)
+        t, nxt, used, reqs = 0.0, 0, [], []
+        for _ in range(args.requests):
+            t += rng.exponential(1.0 / rate)
+            if used and rng.random() < 0.3:  # cached geometry, new extent / field / function
+                sp, dg, _, alt = pool[used[int(rng.integers(0, len(used)))]]
+                w = max(16, int(sp["side"] * rng.uniform(0.5, 1.0)))
+                x0 = float(rng.integers(0, sp["side"] - w + 1))
+                y0 = float(rng.integers(0, sp["side"] - w + 1))
+                job = Job(dg, grid(w, w, x0, y0), "float64", _lib.FUNS[int(rng.integers(0, 7))], None, field=alt,
+                          background=float("nan"))
+            else:
+                i = nxt % len(pool)
+                nxt += 1
+                used.append(i)
+                sp, dg, f0, _ = pool[i]
+                job = Job(dg, grid(sp["side"], sp["side"]), "float64", sp["fun"], None, field=f0,
+                          background=float("nan"))
+            reqs.append((t, job))
+        return reqs
+
+    # exact burns of every request (untimed, profiling level 2)
+    ctx.set_profiling(2)
+    scheds, burns = {}, {}
+    for rate in rates:
+        scheds[rate] = schedule(rate, int(rate))
+        tot = 0
+        for _, job in scheds[rate]:
+            out = DeviceBuffer(ctx, job.out_bytes)
+            j = Job(job.geometry, job.info, job.dtype, job.fun, out, field=job.field, background=job.background)
+            rasterize_device(ctx, j)
+            tot += ctx.last_counters()[0]
+            out.free()
+        burns[rate] = tot
+    ctx.set_profiling(0)
+
+    def run(rate, n_streams, max_batch):
+        srv = Server(0, budget_bytes=200 * 10**9, n_streams=n_streams, max_batch=max_batch)
+        reqs = scheds[rate]
+        tickets = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def producer():
+            for k, (ta, job) in enumerate(reqs):
+                d = t0 + ta - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[k] = srv.submit(job)
+
+        th = threading.Thread(target=producer)
+        th.start()
+        lat = []
+        for k in range(len(reqs)):
+            while tickets[k] is None:
+                time.sleep(0.0001)
+            _, ms = srv.wait(tickets[k])
+            lat.append(ms)
+            srv.release(tickets[k])
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        lat = np.array(lat)
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "jobs_per_s": round(len(reqs) / wall, 1), "burns_per_s": burns[rate] / wall,
+                "wall_s": round(wall, 3), "batches": st["batches"], "peak_memory_GB": round(st["memory_peak"] / 1e9, 2)}
+
+    table = {}
+    for rate in rates:
+        served = run(rate, 2, 64)
+        plain = run(rate, 1, 1)
+        table[str(int(rate))] = {"served": served, "plain": plain,
+                                 "served_over_plain_throughput": round(served["burns_per_s"] / plain["burns_per_s"], 3)}
+        log(f"rate {rate:.0f}/s: served p50 {served['p50_ms']} p99 {served['p99_ms']} ms, "
+            f"{served['jobs_per_s']} jobs/s; plain p50 {plain['p50_ms']} p99 {plain['p99_ms']} ms, "
+            f"{plain['jobs_per_s']} jobs/s")
+    for sp, dg, f0, alt in pool:
+        f0.free()
+        alt.free()
+        dg.free()
+    ctx.close()
+    top = table[str(int(rates[-1]))]
+    result = {
+        "metric": METRIC,
+        "value": top["served"]["burns_per_s"],
+        "unit": "pixel-burns/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * top["served"]["wall_s"] / args.requests, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 crossings; float64 folds",
+        "data": "synthetic (seeded job stream, generated on the device)",
+        "config": {"workload": f"served: Poisson arrivals over the seeded job stream (pool of {len(pool)} geometry "
+                               f"sets, 30% reuse with a new extent/field/function), 200 GB budget, "
+                               f"{args.requests} requests per rate",
+                   "rates": rates},
+        "by_rate": table,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

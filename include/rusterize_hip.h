@@ -271,6 +271,37 @@ typedef struct rz_job {
  * Asynchronous on the context's stream. */
 rz_status rz_batch_rasterize(rz_context* ctx, const rz_job* jobs, int64_t n_jobs);
 
+/* --------------------------------------------------------------- serving -- */
+
+/* No reference counterpart: a serving loop for streams of rasterise
+ * requests (continuous batching).  A worker thread owns n_streams contexts;
+ * whenever one is idle it admits queued requests in arrival order while their
+ * outputs and estimated workspace fit memory_budget bytes (at most max_batch
+ * per batch) and runs them as one rz_batch_rasterize.  Batches on different
+ * contexts overlap on the device.  Outputs are allocated by the server and
+ * stay on the device until released.  Every output equals the request's own
+ * rz_rasterize_device output. */
+typedef struct rz_server rz_server;
+typedef struct rz_server_stats {
+  uint64_t queued, in_flight_batches, batches, completed;
+  uint64_t memory_used, memory_peak, memory_budget;
+} rz_server_stats;
+
+rz_status rz_server_create(int device, uint64_t memory_budget, int n_streams, int max_batch,
+                           rz_server** out);
+/* Waits for every submitted request, then frees everything. */
+rz_status rz_server_destroy(rz_server* server);
+/* Queues a request (job->out_dev is ignored); the geometry, field and band
+ * map must stay valid until the request completes. */
+rz_status rz_server_submit(rz_server* server, const rz_job* job, uint64_t* ticket);
+/* Waits for a request (timeout_ms < 0: no limit) and gives its device output
+ * and its latency from submission to completion. */
+rz_status rz_server_wait(rz_server* server, uint64_t ticket, int timeout_ms, void** out_dev,
+                         double* latency_ms);
+/* Returns a completed request's output to the budget. */
+rz_status rz_server_release(rz_server* server, uint64_t ticket);
+rz_status rz_server_get_stats(rz_server* server, rz_server_stats* out);
+
 /* Profiling level: 0 off; 1 records HIP events around every stage of every
  * rz_rasterize_device call on the context's stream (no added
  * synchronisation); 2 also counts exact pixel-burns.

@@ -48,6 +48,11 @@ class RzJob(ctypes.Structure):
     ]
 
 
+class RzServerStats(ctypes.Structure):
+    _fields_ = [(name, c_uint64) for name in ("queued", "in_flight_batches", "batches", "completed",
+                                              "memory_used", "memory_peak", "memory_budget")]
+
+
 class RusterizeNativeError(RuntimeError):
     """A failing call into librusterize_hip; `status` is the rz_status code."""
 
@@ -98,6 +103,12 @@ _SIGNATURES = [
     ("rz_rasterize_device", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_batch_rasterize", c_int, [c_void_p, POINTER(RzJob), c_int64]),
+    ("rz_server_create", c_int, [c_int, c_uint64, c_int, c_int, POINTER(c_void_p)]),
+    ("rz_server_destroy", c_int, [c_void_p]),
+    ("rz_server_submit", c_int, [c_void_p, POINTER(RzJob), POINTER(c_uint64)]),
+    ("rz_server_wait", c_int, [c_void_p, c_uint64, c_int, POINTER(c_void_p), POINTER(c_double)]),
+    ("rz_server_release", c_int, [c_void_p, c_uint64]),
+    ("rz_server_get_stats", c_int, [c_void_p, POINTER(RzServerStats)]),
     ("rz_context_set_profiling", c_int, [c_void_p, c_int]),
     ("rz_context_stage_times", c_int, [c_void_p, POINTER(c_double), c_int]),
     ("rz_context_reset_stage_times", None, [c_void_p]),

@@ -533,8 +533,9 @@ struct Group {
   uint32_t t_lo, t_hi;
 };
 
-rz_status check_job(const rz_job& j) {
-  if (!j.geometry || !j.out_dev) return rz::fail(RZ_VALUE_ERROR, "NULL geometry or output of a job");
+rz_status check_job_common(const rz_job& j, bool need_out) {
+  if (!j.geometry || (need_out && !j.out_dev))
+    return rz::fail(RZ_VALUE_ERROR, "NULL geometry or output of a job");
   if ((unsigned)j.dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
   if ((unsigned)j.fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
   const uint32_t nl = j.layer_dev ? j.n_layers : 1u;
@@ -553,7 +554,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   if (n_in <= 0) return RZ_OK;
   if (n_in >= (int64_t)1 << 24) return rz::fail(RZ_CAPACITY_ERROR, "more than 2^24 jobs in one batch");
   for (int64_t k = 0; k < n_in; ++k) {
-    rz_status st = check_job(jobs_in[k]);
+    rz_status st = check_job_common(jobs_in[k], true);
     if (st) return st;
   }
   HIP_TRY(hipSetDevice(c->device));
@@ -985,6 +986,10 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
   j.out_dev = out_dev;
   return run_batch(c, &j, 1);
 }
+
+// --------------------------------------------------------------- serving --
+
+#include "rz_server.hip"
 
 // ------------------------------------------------- host-in / host-out ----
 

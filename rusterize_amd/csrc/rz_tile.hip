@@ -246,10 +246,8 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     const int32_t ty = (int32_t)(tl / (uint32_t)J.ntx), tx = (int32_t)(tl % (uint32_t)J.ntx);
     const int32_t C0 = tx * TW;
     const int32_t Rw = ty * TILE_H + strip * WAVE_ROWS;  // first row of the strip
-    const int32_t r = Rw + rl;                          // coverage-layout row of this lane
     const int32_t strip_r1 = min(Rw + WAVE_ROWS, nrows);
     const int32_t tile_c1 = min(C0 + TW, ncols);
-    const bool row_ok = r < strip_r1;
     const bool dedup_job = J.dedup_lines != 0;
     // heavy tile (many items): every polygon mask comes from the pre-pass
     uint64_t hbase = 0;

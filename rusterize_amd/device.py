@@ -136,7 +136,7 @@ class Job:
     info: RzRasterInfo
     dtype: str
     fun: str
-    out: DeviceBuffer
+    out: DeviceBuffer | None  # None for server requests (the server allocates it)
     field: DeviceBuffer | None = None
     scalar: float = 1
     layers: DeviceBuffer | None = None
@@ -163,7 +163,7 @@ class Job:
         j.layer_dev = self.layers.ptr if self.layers is not None else None
         j.n_layers = self.n_layers if self.layers is not None else 1
         j.background_bits = value_bits(self.background, self.dtype)
-        j.out_dev = self.out.ptr
+        j.out_dev = self.out.ptr if self.out is not None else None
         return j
 
     def result(self) -> np.ndarray:
@@ -190,10 +190,60 @@ def batch_rasterize(ctx: Context, jobs: list[Job]) -> None:
     check(ctx.L.rz_batch_rasterize(ctx.ptr, arr, len(jobs)))
 
 
+class Server:
+    """rz_server: admission by a device-memory budget and continuous batching
+    over n_streams contexts.  Requests are Jobs whose `out` is None; the
+    server allocates every output on the device until `release`."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 200 * 10**9, n_streams: int = 2,
+                 max_batch: int = 64):
+        self.L = _lib.lib()
+        self.ptr = c_void_p()
+        check(self.L.rz_server_create(int(device), int(budget_bytes), int(n_streams), int(max_batch),
+                                      byref(self.ptr)))
+        self._jobs = {}  # ticket -> Job (keeps its buffers alive until released)
+
+    def submit(self, job: Job) -> int:
+        t = c_uint64()
+        s = job.as_struct()
+        check(self.L.rz_server_submit(self.ptr, byref(s), byref(t)))
+        self._jobs[t.value] = job
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> tuple[int, float]:
+        """(device address of the output, latency in ms from submission to completion)."""
+        p, lat = c_void_p(), c_double()
+        check(self.L.rz_server_wait(self.ptr, ticket, int(timeout_ms), byref(p), byref(lat)))
+        return p.value, lat.value
+
+    def result(self, ctx: Context, ticket: int) -> np.ndarray:
+        """Waits for a request and downloads its output through `ctx`."""
+        p, _ = self.wait(ticket)
+        job = self._jobs[ticket]
+        out = np.empty(job.shape, dtype=_lib.np_dtype(job.dtype))
+        check(ctx.L.rz_memcpy_d2h(ctx.ptr, out.ctypes.data, c_void_p(p), out.nbytes))
+        return out
+
+    def release(self, ticket: int) -> None:
+        check(self.L.rz_server_release(self.ptr, ticket))
+        self._jobs.pop(ticket, None)
+
+    def stats(self) -> dict:
+        st = _lib.RzServerStats()
+        check(self.L.rz_server_get_stats(self.ptr, byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def close(self) -> None:
+        if self.ptr:
+            self.L.rz_server_destroy(self.ptr)
+            self.ptr = c_void_p()
+            self._jobs.clear()
+
+
 def grid(ncols: int, nrows: int, xmin: float = 0.0, ymin: float = 0.0, res: float = 1.0) -> RzRasterInfo:
     """An explicit grid: ncols x nrows pixels of `res` from (xmin, ymin)."""
     return _lib.raster_info(None, resolution=(res, res), extent=(xmin, ymin, xmin + ncols * res, ymin + nrows * res))
 
 
-__all__ = ["Context", "DeviceBuffer", "DeviceGeometry", "Job", "rasterize_device", "batch_rasterize", "grid",
-           "value_bits"]
+__all__ = ["Context", "DeviceBuffer", "DeviceGeometry", "Job", "Server", "rasterize_device", "batch_rasterize",
+           "grid", "value_bits"]

@@ -271,6 +271,21 @@ typedef struct rz_job {
  * Asynchronous on the context's stream. */
 rz_status rz_batch_rasterize(rz_context* ctx, const rz_job* jobs, int64_t n_jobs);
 
+/* No reference counterpart: a batch captured in a HIP graph on a context of
+ * its own (own stream and workspace).  One eager pass sizes the bins (the
+ * captured capacities are its totals x headroom); the captured pipeline then
+ * never returns to the host.  Each launch rasterises the same jobs again:
+ * field and band buffers may be refilled between launches (their values are
+ * read at launch time); geometry, grids, pixel functions and output pointers
+ * are fixed.  A launch whose bins outgrow the capacities fails in
+ * rz_batch_graph_synchronize (RZ_CAPACITY_ERROR): capture again. */
+typedef struct rz_batch_graph rz_batch_graph;
+rz_status rz_batch_graph_create(int device, const rz_job* jobs, int64_t n_jobs, double headroom,
+                                rz_batch_graph** out);
+rz_status rz_batch_graph_launch(rz_batch_graph* graph);
+rz_status rz_batch_graph_synchronize(rz_batch_graph* graph);
+rz_status rz_batch_graph_destroy(rz_batch_graph* graph);
+
 /* --------------------------------------------------------------- serving -- */
 
 /* No reference counterpart: a serving loop for streams of rasterise

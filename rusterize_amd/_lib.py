@@ -103,6 +103,10 @@ _SIGNATURES = [
     ("rz_rasterize_device", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_batch_rasterize", c_int, [c_void_p, POINTER(RzJob), c_int64]),
+    ("rz_batch_graph_create", c_int, [c_int, POINTER(RzJob), c_int64, c_double, POINTER(c_void_p)]),
+    ("rz_batch_graph_launch", c_int, [c_void_p]),
+    ("rz_batch_graph_synchronize", c_int, [c_void_p]),
+    ("rz_batch_graph_destroy", c_int, [c_void_p]),
     ("rz_server_create", c_int, [c_int, c_uint64, c_int, c_int, POINTER(c_void_p)]),
     ("rz_server_destroy", c_int, [c_void_p]),
     ("rz_server_submit", c_int, [c_void_p, POINTER(RzJob), POINTER(c_uint64)]),

@@ -167,7 +167,9 @@ struct BandArgs {
   const PolyEdge* edges;    // [n_vertices]
   uint32_t* band_cnt;       // [n_bands] counts (count pass) / remaining slots (scatter pass)
   const uint64_t* band_off; // [n_bands + 1] (scatter pass)
-  PolyEdge* band_edges;     // (scatter pass)
+  PolyEdge* band_edges;     // [edge_cap] (scatter pass)
+  uint64_t edge_cap;
+  uint32_t* status;         // bit 3 set when the banded edges exceed edge_cap
 };
 
 struct BinArgs {

@@ -267,10 +267,13 @@ __global__ void k_item_desc(DescArgs a) {
 
 // ------------------------------------------------------------ edge bands --
 
-__global__ void k_band_assign(ItemDesc* desc, const uint32_t* band_n, const uint32_t* band_base,
-                              int64_t n_items) {
+// Items whose band entries would pass the capacity stay unbanded (their
+// strips then scan the whole edge list: slower, same result).
+__global__ void k_band_assign(ItemDesc* desc, const uint32_t* band_n, const uint64_t* band_base,
+                              int64_t n_items, uint64_t cap_entries) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_items && band_n[i]) desc[i].band = band_base[i] + 1u;
+  if (i < n_items && band_n[i] && band_base[i] + band_n[i] <= cap_entries)
+    desc[i].band = (uint32_t)band_base[i] + 1u;
 }
 
 // The band entry of edge v's first tile row and the number of tile rows its
@@ -325,13 +328,17 @@ __global__ __launch_bounds__(256) void k_band_scatter(BandArgs a) {
     old = (uint32_t)__shfl((int)old, leader, 64);
     if ((m >> lane) & 1ull) {
       const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-      a.band_edges[a.band_off[key] + (old - (uint32_t)__popcll(m) + rank)] = e;
+      const uint64_t pos = a.band_off[key] + (old - (uint32_t)__popcll(m) + rank);
+      if (pos < a.edge_cap) a.band_edges[pos] = e;
+      else atomicOr(a.status, 8u);
     }
     todo &= ~m;
   }
   for (uint32_t b = 1; b < nb; ++b) {
     const uint32_t old = atomicSub(&a.band_cnt[first + b], 1u);
-    a.band_edges[a.band_off[first + b] + old - 1u] = e;
+    const uint64_t pos = a.band_off[first + b] + old - 1u;
+    if (pos < a.edge_cap) a.band_edges[pos] = e;
+    else atomicOr(a.status, 8u);
   }
 }
 

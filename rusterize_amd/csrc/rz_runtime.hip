@@ -8,7 +8,6 @@
 #include <cstring>
 #include <mutex>
 #include <new>
-#include <rocprim/rocprim.hpp>
 #include <string>
 #include <vector>
 
@@ -19,6 +18,7 @@
 // The kernels are compiled in this translation unit so every k_tile<T, FUN>
 // instantiation used by the dispatch below is emitted here.
 #include "rz_kernels.hip"
+#include "rz_sort.hip"
 #include "rz_tile.hip"
 
 using namespace rzk;
@@ -71,7 +71,8 @@ struct rz_context {
   // workspace
   Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
-      heavy_cnt, heavy_base, heavy_pairs, masks, vert_item;
+      heavy_cnt, heavy_base, heavy_pairs, masks, vert_item, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
+      sort_hist, sort_off, sort_tmp;
   uint64_t* host_misc = nullptr;  // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
   // the end of its last upload); tab_dev / tab_dev_bytes say which device
@@ -89,13 +90,14 @@ struct rz_context {
   std::vector<EventSet> ev_pool, ev_pending;
   EventSet cur;
   double acc_ms[kStages] = {};
-  uint64_t last_pairs = 0, last_tiles = 0;
+  uint64_t last_pairs = 0, last_tiles = 0, last_band_entries = 0, last_band_edges = 0;
   bool burns_pending = false;
   std::vector<Buf*> all() {
     return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
             &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue,
             &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
-            &heavy_pairs, &masks, &vert_item};
+            &heavy_pairs, &masks, &vert_item, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
+            &sort_hist, &sort_off, &sort_tmp};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -205,9 +207,9 @@ static rz_status check_status_word(rz_context* c) {
   if (flags & 1u)
     return rz::fail(RZ_CAPACITY_ERROR,
                     "line coordinates beyond +-2^60 pixels are outside the supported range");
-  if (flags & 4u)
+  if (flags & (4u | 8u))
     return rz::fail(RZ_CAPACITY_ERROR,
-                    "tile/item pairs exceeded the capacity of the captured batch; capture it again");
+                    "the batch outgrew the capacities it was captured with; capture it again");
   return RZ_OK;
 }
 
@@ -550,7 +552,14 @@ rz_status check_job_common(const rz_job& j, bool need_out) {
 // The pipeline of a batch.  Jobs are ordered by (dtype, function, lines) so
 // every group's tiles form one contiguous range; vertex, item and tile
 // indices are global over the batch.
-static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
+// Fixed capacities of a captured batch (see rz_batch_graph_create).
+struct BatchCaps {
+  uint64_t pairs, band_entries, band_edges;
+};
+
+static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
+                           const BatchCaps* caps = nullptr) {
+  const bool capture = caps != nullptr;
   if (n_in <= 0) return RZ_OK;
   if (n_in >= (int64_t)1 << 24) return rz::fail(RZ_CAPACITY_ERROR, "more than 2^24 jobs in one batch");
   for (int64_t k = 0; k < n_in; ++k) {
@@ -683,7 +692,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   if (any_band) {
     HIP_TRY(ensure(c->vert_item, (size_t)std::max<int64_t>(V, 1) * 4));
     HIP_TRY(ensure(c->band_n, (size_t)(I + 1) * 4));
-    HIP_TRY(ensure(c->band_base, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->band_base, (size_t)(I + 1) * 8));
   }
   if (any_binned) {
     HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
@@ -714,8 +723,10 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     std::memcpy(c->host_tab, c->tab_build.data(), tab_bytes);
     c->tab_dev = nullptr;
     HIP_TRY(hipMemcpyAsync(c->jobtab.p, c->host_tab, tab_bytes, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipEventRecord(c->tab_ev, s));
-    c->tab_pending = true;
+    if (!capture) {
+      HIP_TRY(hipEventRecord(c->tab_ev, s));
+      c->tab_pending = true;
+    }
     c->tab_dev = c->jobtab.p;
     c->tab_dev_bytes = tab_bytes;
   }
@@ -756,47 +767,42 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
 
-  // 2. binning: stable sort-middle item lists per tile for binned jobs
+  // 2. binning: stable sort-middle item lists per tile for binned jobs.
+  // Eager calls read the pair, band-entry and band-edge totals back (one or
+  // two host round trips) and size the buffers to them; a captured batch
+  // (caps != null) uses fixed capacities instead and never leaves the device.
   TileArgs ta;
   std::memset(&ta, 0, sizeof(ta));
   uint64_t n_pairs = 0;
-  // prefix sums of the tile counts (pairs) and band counts, then one host
-  // round trip for both totals
   if (any_binned) {
-    size_t tb = 0;
-    HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->counts.p,
-                                    (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
-                                    rocprim::plus<uint64_t>(), s));
-    HIP_TRY(ensure(c->temp, tb));
-    HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->counts.p,
-                                    (uint64_t*)c->offsets.p, (uint64_t)0, (size_t)(I + 1),
-                                    rocprim::plus<uint64_t>(), s));
-    HIP_TRY(hipMemcpyAsync(&c->host_misc[0], (const uint64_t*)c->offsets.p + I, 8,
-                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(ensure(c->scan_tmp, scan_blocks((uint64_t)I + 1) * 8));
+    exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->counts.p, (uint64_t*)c->offsets.p,
+                                       (uint64_t)I + 1, (uint64_t*)c->scan_tmp.p, s);
+    if (!capture)
+      HIP_TRY(hipMemcpyAsync(&c->host_misc[0], (const uint64_t*)c->offsets.p + I, 8,
+                             hipMemcpyDeviceToHost, s));
   }
   if (any_band) {
-    size_t tb = 0;
-    HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->band_n.p,
-                                    (uint32_t*)c->band_base.p, 0u, (size_t)(I + 1),
-                                    rocprim::plus<uint32_t>(), s));
-    HIP_TRY(ensure(c->temp, tb));
-    HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->band_n.p,
-                                    (uint32_t*)c->band_base.p, 0u, (size_t)(I + 1),
-                                    rocprim::plus<uint32_t>(), s));
-    c->host_misc[2] = 0;  // the copy below fills its low 4 bytes
-    HIP_TRY(hipMemcpyAsync(&c->host_misc[2], (const uint32_t*)c->band_base.p + I, 4,
-                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(ensure(c->scan_tmp2, scan_blocks((uint64_t)I + 1) * 8));
+    exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->band_n.p, (uint64_t*)c->band_base.p,
+                                       (uint64_t)I + 1, (uint64_t*)c->scan_tmp2.p, s);
+    if (!capture)
+      HIP_TRY(hipMemcpyAsync(&c->host_misc[2], (const uint64_t*)c->band_base.p + I, 8,
+                             hipMemcpyDeviceToHost, s));
   }
-  if (any_binned || any_band) HIP_TRY(hipStreamSynchronize(s));
+  if (!capture && (any_binned || any_band)) HIP_TRY(hipStreamSynchronize(s));
+  c->last_band_entries = c->last_band_edges = 0;
   if (any_band) {
     // edge bands of long polygon rings (see kBandMinEdges)
-    const uint64_t nb = (uint32_t)c->host_misc[2];
+    const uint64_t nb = capture ? caps->band_entries : c->host_misc[2];
+    c->last_band_entries = nb;
     if (nb > 0) {
       HIP_TRY(ensure(c->band_cnt, (nb + 1) * 4));
       HIP_TRY(ensure(c->band_off, (nb + 1) * 8));
+      HIP_TRY(ensure(c->scan_tmp3, scan_blocks(nb + 1) * 8));
       zero_words(c->band_cnt.p, nb + 1, s);
       hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I)), dim3(256), 0, s, (ItemDesc*)c->desc.p,
-                         (const uint32_t*)c->band_n.p, (const uint32_t*)c->band_base.p, I);
+                         (const uint32_t*)c->band_n.p, (const uint64_t*)c->band_base.p, I, nb);
       BandArgs bd;
       bd.vert_item = (const uint32_t*)c->vert_item.p;
       bd.n_vertices = V;
@@ -805,21 +811,24 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
       bd.band_cnt = (uint32_t*)c->band_cnt.p;
       bd.band_off = (const uint64_t*)c->band_off.p;
       bd.band_edges = nullptr;
+      bd.edge_cap = 0;
+      bd.status = d_status;
       hipLaunchKernelGGL(k_band_count, dim3(blocks_for(V)), dim3(256), 0, s, bd);
-      size_t tb = 0;
-      HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->band_cnt.p,
-                                      (uint64_t*)c->band_off.p, (uint64_t)0, (size_t)(nb + 1),
-                                      rocprim::plus<uint64_t>(), s));
-      HIP_TRY(ensure(c->temp, tb));
-      HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->band_cnt.p,
-                                      (uint64_t*)c->band_off.p, (uint64_t)0, (size_t)(nb + 1),
-                                      rocprim::plus<uint64_t>(), s));
-      HIP_TRY(hipMemcpyAsync(&c->host_misc[3], (const uint64_t*)c->band_off.p + nb, 8,
-                             hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      const uint64_t ne = c->host_misc[3];
+      exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->band_cnt.p, (uint64_t*)c->band_off.p,
+                                         nb + 1, (uint64_t*)c->scan_tmp3.p, s);
+      uint64_t ne = 0;
+      if (capture) {
+        ne = caps->band_edges;
+      } else {
+        HIP_TRY(hipMemcpyAsync(&c->host_misc[3], (const uint64_t*)c->band_off.p + nb, 8,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        ne = c->host_misc[3];
+      }
+      c->last_band_edges = ne;
       HIP_TRY(ensure(c->band_edges, std::max<uint64_t>(ne, 1) * sizeof(PolyEdge)));
       bd.band_edges = (PolyEdge*)c->band_edges.p;
+      bd.edge_cap = ne;
       if (ne > 0) hipLaunchKernelGGL(k_band_scatter, dim3(blocks_for(V)), dim3(256), 0, s, bd);
       ta.band_off = (const uint64_t*)c->band_off.p;
       ta.band_edges = (const PolyEdge*)c->band_edges.p;
@@ -827,8 +836,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
   }
   if (any_binned) {
     const uint64_t* d_total = (const uint64_t*)c->offsets.p + I;
-    size_t tb = 0;
-    n_pairs = c->host_misc[0];
+    n_pairs = capture ? caps->pairs : c->host_misc[0];
     if (n_pairs >= (uint64_t)UINT32_MAX)
       return rz::fail(RZ_CAPACITY_ERROR, "more than 2^32 tile/item pairs in one call");
     const uint64_t cap = std::max<uint64_t>(n_pairs, 1);
@@ -848,27 +856,38 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     ba.vals = (uint32_t*)c->vals_a.p;
     ba.status = d_status;
     hipLaunchKernelGGL(k_bin_emit, dim3(blocks_for(I)), dim3(256), 0, s, ba);
+    // a captured batch sorts its whole capacity: the slots past the actual
+    // pairs get a key above every tile, so they sort last and are never read
+    if (capture && n_pairs > 0)
+      hipLaunchKernelGGL(k_fill_keys, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
+                         (uint32_t*)c->keys_a.p, d_total, n_pairs);
     if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], s));
     // stable LSD radix sort by tile key keeps geometry order inside each tile
-    unsigned end_bit = 1;
-    while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)n_tiles) ++end_bit;
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) <= (uint64_t)n_tiles) ++end_bit;
+    const uint32_t* sorted_keys = (const uint32_t*)c->keys_a.p;
+    const uint32_t* sorted_vals = (const uint32_t*)c->vals_a.p;
     if (n_pairs > 0) {
-      tb = 0;
-      HIP_TRY(rocprim::radix_sort_pairs(nullptr, tb, (const uint32_t*)c->keys_a.p,
-                                        (uint32_t*)c->keys_b.p, (const uint32_t*)c->vals_a.p,
-                                        (uint32_t*)c->vals_b.p, (size_t)n_pairs, 0, end_bit, s));
-      HIP_TRY(ensure(c->temp, tb));
-      HIP_TRY(rocprim::radix_sort_pairs(c->temp.p, tb, (const uint32_t*)c->keys_a.p,
-                                        (uint32_t*)c->keys_b.p, (const uint32_t*)c->vals_a.p,
-                                        (uint32_t*)c->vals_b.p, (size_t)n_pairs, 0, end_bit, s));
+      const uint64_t nbk = sort_blocks(n_pairs);
+      HIP_TRY(ensure(c->sort_hist, nbk * kRadix * 4));
+      HIP_TRY(ensure(c->sort_off, nbk * kRadix * 4));
+      HIP_TRY(ensure(c->sort_tmp, scan_blocks(nbk * kRadix) * 4));
+      const bool in_b = radix_sort_pairs((uint32_t*)c->keys_a.p, (uint32_t*)c->vals_a.p,
+                                         (uint32_t*)c->keys_b.p, (uint32_t*)c->vals_b.p, n_pairs, end_bit,
+                                         (uint32_t*)c->sort_hist.p, (uint32_t*)c->sort_off.p,
+                                         (uint32_t*)c->sort_tmp.p, s);
+      if (in_b) {
+        sorted_keys = (const uint32_t*)c->keys_b.p;
+        sorted_vals = (const uint32_t*)c->vals_b.p;
+      }
     }
     hipLaunchKernelGGL(k_tile_offsets, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
-                       (const uint32_t*)c->keys_b.p, d_total, n_pairs, (const uint32_t*)d_status,
-                       n_tiles, (uint32_t*)c->tile_off.p);
+                       sorted_keys, d_total, n_pairs, (const uint32_t*)d_status, n_tiles,
+                       (uint32_t*)c->tile_off.p);
     if (n_pairs > 0)
       hipLaunchKernelGGL(k_gather_desc, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
-                         (const ItemDesc*)c->desc.p, I, (const uint32_t*)c->vals_b.p, d_total,
-                         n_pairs, (ItemDesc*)c->pair_desc.p);
+                         (const ItemDesc*)c->desc.p, I, sorted_vals, d_total, n_pairs,
+                         (ItemDesc*)c->pair_desc.p);
     ta.tile_off = (const uint32_t*)c->tile_off.p;
     ta.pair_desc = (const ItemDesc*)c->pair_desc.p;
     if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
@@ -879,29 +898,23 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
     if (n_pairs >= kHeavyItems) {
       HIP_TRY(ensure(c->heavy_cnt, ((size_t)n_tiles + 1) * 4));
       HIP_TRY(ensure(c->heavy_base, ((size_t)n_tiles + 1) * 8));
+      HIP_TRY(ensure(c->scan_tmp4, scan_blocks((uint64_t)n_tiles + 1) * 8));
       hipLaunchKernelGGL(k_heavy_count, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
                          (const uint32_t*)c->tile_off.p, n_tiles, kHeavyItems,
                          (uint32_t*)c->heavy_cnt.p);
-      tb = 0;
-      HIP_TRY(rocprim::exclusive_scan(nullptr, tb, (const uint32_t*)c->heavy_cnt.p,
-                                      (uint64_t*)c->heavy_base.p, (uint64_t)0, (size_t)n_tiles + 1,
-                                      rocprim::plus<uint64_t>(), s));
-      HIP_TRY(ensure(c->temp, tb));
-      HIP_TRY(rocprim::exclusive_scan(c->temp.p, tb, (const uint32_t*)c->heavy_cnt.p,
-                                      (uint64_t*)c->heavy_base.p, (uint64_t)0, (size_t)n_tiles + 1,
-                                      rocprim::plus<uint64_t>(), s));
+      exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->heavy_cnt.p, (uint64_t*)c->heavy_base.p,
+                                         (uint64_t)n_tiles + 1, (uint64_t*)c->scan_tmp4.p, s);
       const uint32_t slot_words = 16u * (uint32_t)(max_tile_w / 64);
       HIP_TRY(ensure(c->heavy_pairs, (size_t)n_pairs * 4));
       HIP_TRY(ensure(c->masks, (size_t)n_pairs * 4 * slot_words * 8));
       hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
-                         (const uint32_t*)c->keys_b.p, d_total, n_pairs,
-                         (const uint32_t*)c->tile_off.p, (const uint64_t*)c->heavy_base.p,
-                         (uint32_t*)c->heavy_pairs.p);
+                         sorted_keys, d_total, n_pairs, (const uint32_t*)c->tile_off.p,
+                         (const uint64_t*)c->heavy_base.p, (uint32_t*)c->heavy_pairs.p);
       CoverArgs ca;
       ca.jobs = d_jobs;
       ca.job_tend = d_tend;
       ca.n_jobs = (int)n;
-      ca.pair_tile = (const uint32_t*)c->keys_b.p;
+      ca.pair_tile = sorted_keys;
       ca.pair_desc = (const ItemDesc*)c->pair_desc.p;
       ca.heavy_base = (const uint64_t*)c->heavy_base.p;
       ca.heavy_pairs = (const uint32_t*)c->heavy_pairs.p;
@@ -954,6 +967,83 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in) {
 extern "C" rz_status rz_batch_rasterize(rz_context* c, const rz_job* jobs, int64_t n_jobs) {
   if (!c || (!jobs && n_jobs > 0)) return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_batch_rasterize");
   return run_batch(c, jobs, n_jobs);
+}
+
+// ------------------------------------------------------- captured batches --
+
+struct rz_batch_graph {
+  rz_context* ctx = nullptr;  // the graph's own workspace and stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" rz_status rz_batch_graph_destroy(rz_batch_graph* g) {
+  if (!g) return RZ_OK;
+  if (g->ctx) {
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+  }
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->ctx) rz_context_destroy(g->ctx);
+  delete g;
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_batch_graph_create(int device, const rz_job* jobs, int64_t n_jobs,
+                                           double headroom, rz_batch_graph** out) {
+  if (!out || (!jobs && n_jobs > 0)) return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_batch_graph_create");
+  if (!(headroom >= 1.0) || headroom > 16.0) return rz::fail(RZ_VALUE_ERROR, "headroom must be in [1, 16]");
+  auto* g = new (std::nothrow) rz_batch_graph();
+  if (!g) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
+  rz_status st = rz_context_create(device, &g->ctx);
+  if (st) {
+    delete g;
+    return st;
+  }
+  rz_context* c = g->ctx;
+  // 1. one eager pass: its totals size the capacities
+  st = run_batch(c, jobs, n_jobs);
+  if (st == RZ_OK) st = rz_context_synchronize(c);
+  if (st) {
+    rz_batch_graph_destroy(g);
+    return st;
+  }
+  auto grow = [&](uint64_t v) { return (uint64_t)((double)v * headroom) + (v ? 64 : 0); };
+  const BatchCaps caps{grow(c->last_pairs), grow(c->last_band_entries), grow(c->last_band_edges)};
+  // 2. a pass at those capacities allocates every buffer, so the captured
+  // pass allocates nothing; 3. capture
+  st = run_batch(c, jobs, n_jobs, &caps);
+  if (st == RZ_OK) st = rz_context_synchronize(c);
+  if (st) {
+    rz_batch_graph_destroy(g);
+    return st;
+  }
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    st = run_batch(c, jobs, n_jobs, &caps);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g->graph);
+    if (st == RZ_OK && e2 != hipSuccess) e = e2;
+  }
+  if (st == RZ_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (st || e != hipSuccess) {
+    rz_batch_graph_destroy(g);
+    return st ? st : rz::fail(RZ_DEVICE_ERROR, std::string("graph capture failed: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_batch_graph_launch(rz_batch_graph* g) {
+  if (!g || !g->exec) return rz::fail(RZ_VALUE_ERROR, "NULL graph");
+  HIP_TRY(hipSetDevice(g->ctx->device));
+  HIP_TRY(hipGraphLaunch(g->exec, g->ctx->stream));
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_batch_graph_synchronize(rz_batch_graph* g) {
+  if (!g) return rz::fail(RZ_VALUE_ERROR, "NULL graph");
+  return rz_context_synchronize(g->ctx);
 }
 
 extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry* dg,

@@ -190,6 +190,31 @@ def batch_rasterize(ctx: Context, jobs: list[Job]) -> None:
     check(ctx.L.rz_batch_rasterize(ctx.ptr, arr, len(jobs)))
 
 
+class BatchGraph:
+    """rz_batch_graph: a batch captured in a HIP graph (own context and
+    stream); launch() replays it, synchronize() waits and checks it."""
+
+    def __init__(self, jobs: list[Job], device: int = 0, headroom: float = 1.0):
+        self.L = _lib.lib()
+        self.ptr = c_void_p()
+        self.jobs = list(jobs)  # keeps the buffers alive
+        arr = (RzJob * max(len(jobs), 1))()
+        for i, j in enumerate(jobs):
+            arr[i] = j.as_struct()
+        check(self.L.rz_batch_graph_create(int(device), arr, len(jobs), float(headroom), byref(self.ptr)))
+
+    def launch(self) -> None:
+        check(self.L.rz_batch_graph_launch(self.ptr))
+
+    def synchronize(self) -> None:
+        check(self.L.rz_batch_graph_synchronize(self.ptr))
+
+    def close(self) -> None:
+        if self.ptr:
+            self.L.rz_batch_graph_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+
 class Server:
     """rz_server: admission by a device-memory budget and continuous batching
     over n_streams contexts.  Requests are Jobs whose `out` is None; the
@@ -245,5 +270,5 @@ def grid(ncols: int, nrows: int, xmin: float = 0.0, ymin: float = 0.0, res: floa
     return _lib.raster_info(None, resolution=(res, res), extent=(xmin, ymin, xmin + ncols * res, ymin + nrows * res))
 
 
-__all__ = ["Context", "DeviceBuffer", "DeviceGeometry", "Job", "Server", "rasterize_device", "batch_rasterize",
+__all__ = ["BatchGraph", "Context", "DeviceBuffer", "DeviceGeometry", "Job", "Server", "rasterize_device", "batch_rasterize",
            "grid", "value_bits"]

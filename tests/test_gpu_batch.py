@@ -199,3 +199,47 @@ def test_job_stream_device_generator_matches_oracle(ctx):
             b.free()
         for g in geoms:
             g.free()
+
+
+def test_captured_batch_graph_replays(ctx):
+    # a captured batch replayed several times, with eager batches between the
+    # replays and new field values before one of them; every replay equals the
+    # eager result of the same inputs
+    from rusterize_amd.device import BatchGraph
+    b = Batch(ctx, SPECS, seed=2)
+    g = None
+    try:
+        batch_rasterize(ctx, b.jobs)
+        ctx.synchronize()
+        eager = [j.result() for j in b.jobs]
+        g = BatchGraph(b.jobs, headroom=1.0)
+        for rep in range(3):
+            for j in b.jobs:  # clobber the outputs
+                j.out.upload(np.full(j.out_bytes, 0x5A, np.uint8))
+            g.launch()
+            g.synchronize()
+            for k, j in enumerate(b.jobs):
+                assert_same(j.result(), eager[k], f"replay {rep} job {k}")
+            batch_rasterize(ctx, b.jobs[::-1])  # eager work between replays
+            ctx.synchronize()
+        # refill one job's field: the next replay reads the new values
+        k = 2
+        f2 = field_for(b.jobs[k].dtype, b.host_geoms[SPECS[k][0]].n_geoms, seed=4242)
+        b.jobs[k].field.upload(f2)
+        g.launch()
+        g.synchronize()
+        ref = oracle.rasterize(b.host_geoms[SPECS[k][0]], oracle.copy_info(b.jobs[k].info), b.jobs[k].dtype,
+                               b.jobs[k].fun, field=f2, layer_of_geom=None if SPECS[k][7] == 1 else None,
+                               n_layers=1, background=b.jobs[k].background) if SPECS[k][7] == 1 else None
+        batch_rasterize(ctx, [b.jobs[k]])
+        ctx.synchronize()
+        eager_new = b.jobs[k].result()
+        g.launch()
+        g.synchronize()
+        assert_same(b.jobs[k].result(), eager_new, "replay after a field refill")
+        if ref is not None:
+            assert_same(eager_new, ref, "refilled field vs oracle")
+    finally:
+        if g is not None:
+            g.close()
+        b.free()

@@ -187,6 +187,9 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=200, help="serve workload: requests per rate")
     ap.add_argument("--pool", type=int, default=48, help="serve workload: distinct geometry sets (stream jobs)")
     ap.add_argument("--jobs", type=int, default=256, help="batch workload: jobs of the seeded job stream")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="batch workload: autotune the launch knobs on this many jobs first")
+    ap.add_argument("--no-graph", action="store_true", help="batch workload: skip the captured-graph leg")
     ap.add_argument("--budget-gb", type=float, default=150.0,
                     help="batch workload: device memory the jobs' inputs, outputs and workspace may use")
     ap.add_argument("--polygons", type=int, default=1_000_000, help="polygon count of the default workload")
@@ -381,8 +384,8 @@ def batch_bench(args) -> int:
     per job and (b) as one rz_batch_rasterize pipeline.  Outputs stay
     resident; both legs are timed from device synchronisation to device
     synchronisation over the same jobs."""
-    from rusterize_amd.device import (Context, DeviceBuffer, DeviceGeometry, Job, batch_rasterize, grid,
-                                      rasterize_device)
+    from rusterize_amd.device import (BatchGraph, Context, DeviceBuffer, DeviceGeometry, Job, autotune,
+                                      autotune_table, batch_rasterize, grid, rasterize_device)
 
     L = _lib.lib()
     specs, need = [], 0.0
@@ -410,6 +413,13 @@ def batch_bench(args) -> int:
                         background=float("nan")))
     ctx.synchronize()
 
+    tuned = {}
+    if args.autotune:
+        t0 = time.perf_counter()
+        autotune(ctx, jobs[:args.autotune], reps=1)
+        tuned = {str(k): v for k, v in autotune_table().items()}
+        log(f"autotune on {args.autotune} jobs: {len(tuned)} classes in {time.perf_counter() - t0:.1f} s")
+
     def plain():
         for j in jobs:
             rasterize_device(ctx, j)
@@ -432,6 +442,20 @@ def batch_bench(args) -> int:
         stages[name] = {k: round(v / reps, 3) for k, v in ctx.stage_times().items()}
         ctx.set_profiling(0)
         log(f"{name}: {times[name] * 1e3:.2f} ms for {n} jobs; stages (ms, summed over calls) {stages[name]}")
+    # the same batch captured in a HIP graph and replayed (own context)
+    graph = None
+    if not args.no_graph:
+        g = BatchGraph(jobs)
+        g.launch()
+        g.synchronize()
+        reps = max(1, args.steps or 1)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.launch()
+        g.synchronize()
+        graph = (time.perf_counter() - t0) / reps
+        g.close()
+        log(f"graph replay: {graph * 1e3:.2f} ms for {n} jobs")
     # exact burns of the whole set (untimed): one batched pass at level 2
     ctx.set_profiling(2)
     batched()
@@ -475,7 +499,11 @@ def batch_bench(args) -> int:
                     "jobs_per_s": n / times["batched"], "out_GB_per_s": px * 8 / times["batched"] / 1e9},
         "plain": {"ms": round(times["plain"] * 1e3, 3), "burns_per_s": burns / times["plain"],
                   "jobs_per_s": n / times["plain"], "out_GB_per_s": px * 8 / times["plain"] / 1e9},
+        "graph": ({"ms": round(graph * 1e3, 3), "burns_per_s": burns / graph, "jobs_per_s": n / graph}
+                  if graph else None),
         "batched_over_plain": round(times["plain"] / times["batched"], 3),
+        "graph_over_plain": round(times["plain"] / graph, 3) if graph else None,
+        "autotune_table": tuned,
         "outputs_identical_on_spot_checks": same,
     }
     print(json.dumps(result), flush=True)

@@ -286,6 +286,21 @@ rz_status rz_batch_graph_launch(rz_batch_graph* graph);
 rz_status rz_batch_graph_synchronize(rz_batch_graph* graph);
 rz_status rz_batch_graph_destroy(rz_batch_graph* graph);
 
+/* ------------------------------------------------------------- autotuner -- */
+
+/* No reference counterpart: per size class (tile width, tiles of a launch,
+ * items per tile) the tiles a work-queue grab takes and the item count from
+ * which a tile's polygon coverage is precomputed in parallel.  rz_autotune
+ * times every candidate on each job (reps timed runs each) and stores the
+ * fastest for that job's class in a process-wide table that every later call
+ * and batch uses; run it while no other rasterisation runs in the process.
+ * rz_autotune_table lists the table (returns the entry count, fills up to
+ * cap); rz_autotune_set adds or (0, 0) removes an entry. */
+rz_status rz_autotune(rz_context* ctx, const rz_job* jobs, int64_t n_jobs, int reps,
+                      int64_t* n_classes);
+int64_t rz_autotune_table(uint32_t* size_class, uint32_t* chunk, uint32_t* heavy_min, int64_t cap);
+rz_status rz_autotune_set(uint32_t size_class, uint32_t chunk, uint32_t heavy_min);
+
 /* --------------------------------------------------------------- serving -- */
 
 /* No reference counterpart: a serving loop for streams of rasterise

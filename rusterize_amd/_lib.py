@@ -107,6 +107,9 @@ _SIGNATURES = [
     ("rz_batch_graph_launch", c_int, [c_void_p]),
     ("rz_batch_graph_synchronize", c_int, [c_void_p]),
     ("rz_batch_graph_destroy", c_int, [c_void_p]),
+    ("rz_autotune", c_int, [c_void_p, POINTER(RzJob), c_int64, c_int, POINTER(c_int64)]),
+    ("rz_autotune_table", c_int64, [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32), c_int64]),
+    ("rz_autotune_set", c_int, [c_uint32, c_uint32, c_uint32]),
     ("rz_server_create", c_int, [c_int, c_uint64, c_int, c_int, POINTER(c_void_p)]),
     ("rz_server_destroy", c_int, [c_void_p]),
     ("rz_server_submit", c_int, [c_void_p, POINTER(RzJob), POINTER(c_uint64)]),

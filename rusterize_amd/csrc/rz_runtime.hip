@@ -9,6 +9,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "rz_internal.h"
@@ -471,13 +472,61 @@ uint32_t tile_grid(uint64_t n_tiles) {
   return std::max<uint32_t>(NUM_XCD, (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
 }
 
-// Tiles per queue grab: 4 once every wave has many tiles (measured best of
-// 1, 2, 4 on the 40,000^2 workload), else 1.  RUSTERIZE_CHUNK overrides it.
-inline uint32_t queue_chunk(uint64_t n_tiles, uint32_t grid) {
+// ---- autotuned launch knobs
+//
+// Per size class (tile width, launch tiles, items per tile) the autotuner
+// (rz_autotune) stores the tiles per queue grab and the item count from which
+// a tile's coverage is precomputed by the k_cover pre-pass.  Classes without
+// an entry use the defaults below.
+struct Tuning {
+  uint32_t chunk = 0;      // 0: default
+  uint32_t heavy_min = 0;  // 0: default (kHeavyItems)
+};
+
+struct TuneState {
+  std::mutex mu;
+  std::unordered_map<uint32_t, Tuning> table;
+  bool force = false;  // the autotuner's candidate under measurement
+  Tuning forced;
+};
+
+TuneState& tune_state() {
+  static TuneState s;
+  return s;
+}
+
+inline int floor_log2_u64(uint64_t v) {
+  int l = 0;
+  while (l < 62 && ((uint64_t)2 << l) <= v) ++l;
+  return l;
+}
+
+// Size class: bit 8 tile width (0: 64 px, 1: 256 px), bits 4-7 log4 of the
+// launch's tiles, bits 0-3 log2 of (items per tile + 1).
+uint32_t size_class(int tile_w, uint64_t tiles, uint64_t pairs) {
+  const uint64_t t = std::max<uint64_t>(tiles, 1);
+  const uint32_t tb = (uint32_t)std::min(15, floor_log2_u64(t) / 2);
+  const uint32_t db = (uint32_t)std::min(15, floor_log2_u64(pairs / t + 1));
+  return (tile_w == 256 ? 1u << 8 : 0u) | tb << 4 | db;
+}
+
+Tuning tuning_for(uint32_t cls) {
+  TuneState& ts = tune_state();
+  std::lock_guard<std::mutex> lk(ts.mu);
+  if (ts.force) return ts.forced;
+  auto it = ts.table.find(cls);
+  return it == ts.table.end() ? Tuning{} : it->second;
+}
+
+// Tiles per queue grab: the tuned value, else 4 once every wave has many
+// tiles (measured best of 1, 2, 4 on the 40,000^2 workload), else 1.
+// RUSTERIZE_CHUNK overrides both.
+inline uint32_t queue_chunk(uint64_t n_tiles, uint32_t grid, uint32_t tuned) {
   if (const char* e = std::getenv("RUSTERIZE_CHUNK")) {
     const int v = std::atoi(e);
     if (v >= 1 && v <= 4096) return (uint32_t)v;
   }
+  if (tuned) return tuned;
   return n_tiles >= (uint64_t)grid * 16 ? 4u : 1u;
 }
 
@@ -486,11 +535,11 @@ void launch_tile(TileArgs a, bool lines, hipStream_t s) {
   const uint64_t tiles = (uint64_t)(a.t_hi - a.t_lo);
   if (lines) {
     const uint32_t g = tile_grid<T, FUN, true>(tiles);
-    a.chunk = queue_chunk(tiles, g);
+    a.chunk = queue_chunk(tiles, g, a.chunk);
     hipLaunchKernelGGL((k_tile<T, FUN, true>), dim3(g), dim3(256), 0, s, a);
   } else {
     const uint32_t g = tile_grid<T, FUN, false>(tiles);
-    a.chunk = queue_chunk(tiles, g);
+    a.chunk = queue_chunk(tiles, g, a.chunk);
     hipLaunchKernelGGL((k_tile<T, FUN, false>), dim3(g), dim3(256), 0, s, a);
   }
 }
@@ -895,12 +944,15 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     // heavy tiles (kHeavyItems items or more): coverage masks precomputed in
     // parallel, one wave per (item, strip); the buffers are sized for every
     // pair so no host round trip is needed
-    if (n_pairs >= kHeavyItems) {
+    // the batch's class: that of its widest-tile / most-tiles group
+    const Tuning bt = tuning_for(size_class(max_tile_w, n_tiles, n_pairs));
+    const uint32_t heavy_min = bt.heavy_min ? bt.heavy_min : kHeavyItems;
+    if (n_pairs >= heavy_min) {
       HIP_TRY(ensure(c->heavy_cnt, ((size_t)n_tiles + 1) * 4));
       HIP_TRY(ensure(c->heavy_base, ((size_t)n_tiles + 1) * 8));
       HIP_TRY(ensure(c->scan_tmp4, scan_blocks((uint64_t)n_tiles + 1) * 8));
       hipLaunchKernelGGL(k_heavy_count, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
-                         (const uint32_t*)c->tile_off.p, n_tiles, kHeavyItems,
+                         (const uint32_t*)c->tile_off.p, n_tiles, heavy_min,
                          (uint32_t*)c->heavy_cnt.p);
       exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->heavy_cnt.p, (uint64_t*)c->heavy_base.p,
                                          (uint64_t)n_tiles + 1, (uint64_t*)c->scan_tmp4.p, s);
@@ -953,6 +1005,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     ta.t_lo = groups[g].t_lo;
     ta.t_hi = groups[g].t_hi;
     ta.queue = (uint32_t*)c->queue.p + g * kQueueWords;
+    const uint64_t gt = groups[g].t_hi - groups[g].t_lo;
+    ta.chunk = tuning_for(size_class(tile_w((int)rz_dtype_size(groups[g].dtype)), gt,
+                                     n_tiles ? n_pairs * gt / n_tiles : 0)).chunk;
     launch_tile_dispatch(groups[g].dtype, groups[g].fun, ta, groups[g].lines, s);
   }
   HIP_TRY(hipGetLastError());
@@ -1044,6 +1099,86 @@ extern "C" rz_status rz_batch_graph_launch(rz_batch_graph* g) {
 extern "C" rz_status rz_batch_graph_synchronize(rz_batch_graph* g) {
   if (!g) return rz::fail(RZ_VALUE_ERROR, "NULL graph");
   return rz_context_synchronize(g->ctx);
+}
+
+// -------------------------------------------------------------- autotuner --
+
+extern "C" rz_status rz_autotune(rz_context* c, const rz_job* jobs, int64_t n_jobs, int reps,
+                                 int64_t* n_classes) {
+  if (!c || (!jobs && n_jobs > 0) || reps < 1) return rz::fail(RZ_VALUE_ERROR, "bad argument to rz_autotune");
+  HIP_TRY(hipSetDevice(c->device));
+  static const uint32_t kChunks[] = {1, 2, 4, 8};
+  static const uint32_t kHeavy[] = {16, 32, 64, 1u << 30};
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  TuneState& ts = tune_state();
+  int64_t tuned = 0;
+  rz_status st = RZ_OK;
+  for (int64_t k = 0; k < n_jobs && st == RZ_OK; ++k) {
+    Tuning best;
+    float best_ms = 1e30f;
+    uint32_t cls = 0;
+    for (uint32_t ch : kChunks)
+      for (uint32_t hv : kHeavy) {
+        {
+          std::lock_guard<std::mutex> lk(ts.mu);
+          ts.force = true;
+          ts.forced = Tuning{ch, hv};
+        }
+        st = run_batch(c, jobs + k, 1);  // warm-up (workspace)
+        if (st == RZ_OK) st = rz_context_synchronize(c);
+        float ms = 0.f;
+        if (st == RZ_OK && hipEventRecord(e0, c->stream) != hipSuccess) st = RZ_DEVICE_ERROR;
+        for (int r = 0; r < reps && st == RZ_OK; ++r) st = run_batch(c, jobs + k, 1);
+        if (st == RZ_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          st = rz::fail(RZ_DEVICE_ERROR, "autotune timing failed");
+        if (st) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = Tuning{ch, hv};
+        }
+        cls = size_class(tile_w((int)rz_dtype_size(jobs[k].dtype)), c->last_tiles, c->last_pairs);
+      }
+    if (st == RZ_OK) {
+      std::lock_guard<std::mutex> lk(ts.mu);
+      if (!ts.table.count(cls)) ++tuned;
+      ts.table[cls] = best;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(ts.mu);
+    ts.force = false;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (n_classes) *n_classes = tuned;
+  return st;
+}
+
+extern "C" int64_t rz_autotune_table(uint32_t* cls, uint32_t* chunk, uint32_t* heavy_min, int64_t cap) {
+  TuneState& ts = tune_state();
+  std::lock_guard<std::mutex> lk(ts.mu);
+  int64_t i = 0;
+  for (const auto& kv : ts.table) {
+    if (i < cap) {
+      if (cls) cls[i] = kv.first;
+      if (chunk) chunk[i] = kv.second.chunk;
+      if (heavy_min) heavy_min[i] = kv.second.heavy_min;
+    }
+    ++i;
+  }
+  return i;
+}
+
+extern "C" rz_status rz_autotune_set(uint32_t cls, uint32_t chunk, uint32_t heavy_min) {
+  if (chunk > 4096) return rz::fail(RZ_VALUE_ERROR, "chunk must be <= 4096");
+  TuneState& ts = tune_state();
+  std::lock_guard<std::mutex> lk(ts.mu);
+  if (chunk == 0 && heavy_min == 0) ts.table.erase(cls);
+  else ts.table[cls] = Tuning{chunk, heavy_min};
+  return RZ_OK;
 }
 
 extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry* dg,

@@ -9,7 +9,7 @@ RusterizeNativeError from rz_context_create).
 """
 from __future__ import annotations
 
-from ctypes import byref, c_double, c_uint64, c_void_p
+from ctypes import byref, c_double, c_int64, c_uint32, c_uint64, c_void_p
 from dataclasses import dataclass
 
 import numpy as np
@@ -265,10 +265,36 @@ class Server:
             self._jobs.clear()
 
 
+def autotune(ctx: Context, jobs: list[Job], reps: int = 2) -> int:
+    """Times every launch-knob candidate on each job (rz_autotune) and stores
+    the fastest per size class; returns the number of new classes."""
+    arr = (RzJob * max(len(jobs), 1))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.as_struct()
+    n = c_int64()
+    check(ctx.L.rz_autotune(ctx.ptr, arr, len(jobs), int(reps), byref(n)))
+    return n.value
+
+
+def autotune_table() -> dict:
+    """The process-wide table: size class -> (tiles per grab, heavy-tile items)."""
+    L = _lib.lib()
+    n = L.rz_autotune_table(None, None, None, 0)
+    cls, ch, hv = (c_uint32 * max(n, 1))(), (c_uint32 * max(n, 1))(), (c_uint32 * max(n, 1))()
+    n = min(n, L.rz_autotune_table(cls, ch, hv, n))
+    return {int(cls[i]): (int(ch[i]), int(hv[i])) for i in range(n)}
+
+
+def autotune_clear() -> None:
+    L = _lib.lib()
+    for c in autotune_table():
+        check(L.rz_autotune_set(c, 0, 0))
+
+
 def grid(ncols: int, nrows: int, xmin: float = 0.0, ymin: float = 0.0, res: float = 1.0) -> RzRasterInfo:
     """An explicit grid: ncols x nrows pixels of `res` from (xmin, ymin)."""
     return _lib.raster_info(None, resolution=(res, res), extent=(xmin, ymin, xmin + ncols * res, ymin + nrows * res))
 
 
-__all__ = ["BatchGraph", "Context", "DeviceBuffer", "DeviceGeometry", "Job", "Server", "rasterize_device", "batch_rasterize",
+__all__ = ["autotune", "autotune_clear", "autotune_table", "BatchGraph", "Context", "DeviceBuffer", "DeviceGeometry", "Job", "Server", "rasterize_device", "batch_rasterize",
            "grid", "value_bits"]

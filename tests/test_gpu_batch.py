@@ -243,3 +243,22 @@ def test_captured_batch_graph_replays(ctx):
         if g is not None:
             g.close()
         b.free()
+
+
+def test_autotuner_fills_table_and_keeps_results(ctx):
+    from rusterize_amd.device import autotune, autotune_clear, autotune_table
+    b = Batch(ctx, SPECS[:6], seed=5)
+    try:
+        autotune_clear()
+        n = autotune(ctx, b.jobs[:4], reps=1)
+        table = autotune_table()
+        assert n >= 1 and len(table) == n
+        for cls, (chunk, heavy) in table.items():
+            assert chunk in (1, 2, 4, 8) and heavy in (16, 32, 64, 1 << 30)
+        batch_rasterize(ctx, b.jobs)  # tuned launches give the same rasters
+        ctx.synchronize()
+        for k, j in enumerate(b.jobs):
+            assert_same(j.result(), b.refs[k], f"tuned job {k}")
+    finally:
+        autotune_clear()
+        b.free()

@@ -205,6 +205,7 @@ struct TileArgs {
   const uint64_t* heavy_base;
   const uint64_t* masks;
   uint32_t slot_words;
+  uint32_t heavy_min;       // items from which a tile is heavy
   // scan jobs: every item of the job is tested against the tile
   const ItemDesc* desc;
   const PolyEdge* edges;

@@ -982,6 +982,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ta.heavy_base = (const uint64_t*)c->heavy_base.p;
       ta.masks = (const uint64_t*)c->masks.p;
       ta.slot_words = slot_words;
+      ta.heavy_min = heavy_min;
     }
   } else if (c->profiling) {
     HIP_TRY(hipEventRecord(c->ev[2], s));

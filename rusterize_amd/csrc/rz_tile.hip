@@ -126,8 +126,18 @@ __device__ inline uint64_t strip_coverage(const PolyEdge* ebase, uint32_t n_e, i
 //    l / LPR + k * RSTEP, so every store instruction writes whole tile rows
 //    (full 128-byte lines, contiguous across lanes).
 // The polygon path is branch-free per lane (predicated crossings and folds).
+// Waves per SIMD the register allocation targets: 7 for 4-byte outputs, 6
+// for 1- and 2-byte outputs (measured: the latency-bound strip loop gains more
+// from the extra wave than the few spilled registers cost), the compiler's
+// choice for 8-byte outputs and the line / point variants.
+template <typename T, bool LINES>
+struct TileOccupancy {
+  static constexpr int value = LINES ? 1 : (sizeof(T) == 4 ? 7 : (sizeof(T) == 8 ? 1 : 6));
+};
+
 template <typename T, int FUN, bool LINES>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile(TileArgs a) {
   constexpr int TW = tile_w(sizeof(T));
   constexpr int NWORD = TW / 64;                    // u64 words of a row mask
   constexpr int PPC = 16 / (int)sizeof(T);          // pixels per 16-byte chunk
@@ -250,12 +260,11 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     const int32_t tile_c1 = min(C0 + TW, ncols);
     const bool dedup_job = J.dedup_lines != 0;
     // heavy tile (many items): every polygon mask comes from the pre-pass
+    // (a tile is heavy iff its pair range holds at least heavy_min items, so
+    // the slot base is only loaded for heavy tiles)
     uint64_t hbase = 0;
-    bool hb_ok = false;
-    if (a.heavy_base && !J.scan) {
-      hbase = heavy_c[t];
-      hb_ok = heavy_c[t + 1u] > hbase;
-    }
+    const bool hb_ok = a.heavy_base && !J.scan && pe - pb >= a.heavy_min;
+    if (hb_ok) hbase = heavy_c[t];
 
     T acc[NPIX];  // acc[k * PPC + j]: row srow + k*RSTEP, column ccol + j
 #pragma unroll

@@ -229,6 +229,8 @@ struct CoverArgs {
   const ItemDesc* pair_desc;    // [n_pairs]
   const uint64_t* heavy_base;   // [n_tiles + 1]
   const uint32_t* heavy_pairs;  // [heavy pairs] pair index
+  const uint32_t* cover_list;   // [*cover_count] slots of heavy polygon pairs
+  const uint32_t* cover_count;
   uint32_t n_tiles;
   const PolyEdge* edges;
   const uint64_t* band_off;

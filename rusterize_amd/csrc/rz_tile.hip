@@ -459,14 +459,33 @@ __global__ void k_heavy_count(const uint32_t* tile_off, uint32_t n_tiles, uint32
   heavy_cnt[t] = c;
 }
 
-__global__ void k_heavy_pairs(const uint32_t* pair_tile, const uint64_t* n_pairs, uint64_t cap,
-                              const uint32_t* tile_off, const uint64_t* heavy_base,
-                              uint32_t* heavy_pairs) {
+// Slot of every pair of a heavy tile, and the compact list of the slots that
+// hold polygon items (the only ones k_cover computes), appended with one
+// atomic per wave.
+__global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_desc,
+                              const uint64_t* n_pairs, uint64_t cap, const uint32_t* tile_off,
+                              const uint64_t* heavy_base, uint32_t* heavy_pairs, uint32_t* cover_list,
+                              uint32_t* cover_count) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= min(*n_pairs, cap)) return;
-  const uint32_t t = pair_tile[i];
-  const uint64_t hb = heavy_base[t];
-  if (heavy_base[t + 1] > hb) heavy_pairs[hb + (i - tile_off[t])] = (uint32_t)i;
+  const int lane = threadIdx.x & 63;
+  uint64_t slot = 0;
+  bool poly = false;
+  if (i < min(*n_pairs, cap)) {
+    const uint32_t t = pair_tile[i];
+    const uint64_t hb = heavy_base[t];
+    if (heavy_base[t + 1] > hb) {
+      slot = hb + (i - tile_off[t]);
+      heavy_pairs[slot] = (uint32_t)i;
+      poly = (pair_desc[i].kind_layer & 3u) == 0u;
+    }
+  }
+  const uint64_t m = __ballot(poly);
+  if (m == 0) return;
+  uint32_t base = 0;
+  const int leader = __builtin_ctzll(m);
+  if (lane == leader) base = atomicAdd(cover_count, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, leader, 64);
+  if (poly) cover_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)slot;
 }
 
 template <int NWORD>
@@ -476,16 +495,15 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
   const RZ_CONST uint64_t* band_c = as_const(a.band_off);
-  const uint64_t units = 4ull * a.heavy_base[a.n_tiles];
+  const uint64_t units = 4ull * *a.cover_count;
   for (uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave; u < units; u += (uint64_t)gridDim.x * 4u) {
-    const uint64_t hp = u >> 2;
+    const uint64_t hp = a.cover_list[u >> 2];
     const int32_t strip = (int32_t)(u & 3u);
     const uint32_t i = a.heavy_pairs[hp];
     const uint32_t t = a.pair_tile[i];
     const ItemDesc d = load_desc(as_const(a.pair_desc) + i);
     const RZ_CONST JobDev& J = jobs_c[d.job];
     if (J.tile_w != TW) continue;
-    if ((d.kind_layer & 3u) != 0u) continue;
     const uint32_t tj = t - J.tile_base;
     const uint32_t layer = tj / J.tiles_per_layer;
     const uint32_t tl = tj - layer * J.tiles_per_layer;

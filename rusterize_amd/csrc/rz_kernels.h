@@ -118,7 +118,8 @@ struct PrepArgs {
   double2* px;              // [n_vertices] pixel coordinates (lines / points)
   uint8_t* vflags;          // [n_vertices]
   ItemBox* item_box;        // [n_items]
-  unsigned long long* item_last;  // [n_items] last kept line segment + 1 (global; 0: none)
+  unsigned long long* item_last;  // [n_items] lines: last kept segment + 1 (global; 0: none);
+                                  // polygons: the sum over edges of the rows each crosses
   uint32_t* vert_item;      // [n_vertices] global item of every vertex, or null
 };
 
@@ -134,8 +135,26 @@ struct __align__(64) ItemDesc {
   uint64_t last;            // lines: last kept segment + 1 (0: none)
   uint32_t job;
   uint32_t band;            // polygons with banded edges: first band entry + 1 (0: not banded)
-  uint64_t pad1;
+  uint32_t flags;           // kItem* bits
+  uint32_t wide;            // kItemWide: first word of the item's row masks / 16
 };
+
+// ItemDesc::flags.  kItemWide: the polygon's coverage is precomputed once
+// per 16-row strip over its whole column range (k_wide_cover) instead of in
+// every output tile it meets: for items several tiles wide with many edges
+// per strip, and for items of many short edges (which the pre-pass visits
+// edge by edge).
+constexpr uint32_t kItemWide = 1u;
+
+// Row masks of a kItemWide item: strip k (rows from (r0 / 16 + k) * 16) holds
+// 16 rows of nw = (c1 - 1) / 64 - c0 / 64 + 1 words each (word w covers
+// columns (c0 / 64 + w) * 64 .. + 63), at wide_masks[wide * 16 + (16 k + row)
+// * nw + w].  A unit of k_wide_cover is one strip of one item and at most
+// kWideWords words of it (a window).
+constexpr int kWideWords = 32;
+
+__host__ __device__ inline int32_t wide_row_words(int32_t c0, int32_t c1) { return ((c1 - 1) >> 6) - (c0 >> 6) + 1; }
+__host__ __device__ inline int32_t wide_strip_count(int32_t r0, int32_t r1) { return (r1 - 1) / 16 - r0 / 16 + 1; }
 
 // Edge bands: a polygon item with more than kBandMinEdges edges that spans
 // two or more 16-row strips gets its edges binned by strip (an edge goes to
@@ -154,10 +173,39 @@ struct DescArgs {
   int n_jobs;
   int64_t n_items;
   const ItemBox* item_box;
-  const unsigned long long* item_last;
+  const unsigned long long* item_last;  // lines: last kept segment + 1; polygons: rows their edges cross
   ItemDesc* desc;           // [n_items]
   uint32_t* counts;         // [n_items] tiles per item, or null (no binned job)
   uint32_t* band_n;         // [n_items] tile rows of a banded polygon item (0: not banded)
+  int wide_mode;            // wide pre-pass: 0 chosen per item, 1 never, 2 every polygon item
+  uint32_t* wide_units;     // [n_items] k_wide_cover units of a wide item (0: not wide), or null
+  uint32_t* wide_words;     // [n_items] row-mask words of a wide item
+};
+
+// Wide items within the capacities get their flag, mask base and unit list.
+struct WideAssignArgs {
+  ItemDesc* desc;
+  int64_t n_items;
+  const uint32_t* wide_units;
+  const uint64_t* unit_base;  // [n_items + 1] exclusive scan of wide_units
+  const uint64_t* word_base;  // [n_items + 1] exclusive scan of wide_words
+  uint64_t unit_cap, word_cap;
+  uint32_t* unit_item;        // [unit_cap] item of every unit
+};
+
+struct WideCoverArgs {
+  const JobDev* jobs;
+  const ItemDesc* desc;       // [n_items], wide fields assigned
+  int64_t n_items;
+  const uint32_t* unit_item;  // [units]
+  const uint64_t* unit_base;  // [n_items + 1]
+  const uint64_t* n_units;    // unit total (device side), clamped to unit_cap
+  uint64_t unit_cap;
+  const PolyEdge* edges;
+  const uint64_t* band_off;
+  const PolyEdge* band_edges;
+  uint64_t* wide_masks;
+  int cov_mode;               // layout per chunk of edges: 0 the cheaper, 1 row-major, 2 edge-major
 };
 
 struct BandArgs {
@@ -206,6 +254,7 @@ struct TileArgs {
   const uint64_t* masks;
   uint32_t slot_words;
   uint32_t heavy_min;       // items from which a tile is heavy
+  const uint64_t* wide_masks;  // row masks of kItemWide items (null: none)
   // scan jobs: every item of the job is tested against the tile
   const ItemDesc* desc;
   const PolyEdge* edges;

@@ -26,16 +26,26 @@ __device__ inline int64_t sat_isize(double v) {
   return (int64_t)v;
 }
 
+// NaN results as the reference's x86 build produces them (SSE): an operation
+// with a NaN operand returns that NaN, sign kept (the first one when both
+// are NaN); an invalid operation (0 * inf, inf - inf, inf / inf) returns the
+// default NaN, which is negative.  The sign decides where f64::total_cmp
+// sorts a NaN crossing.  The GPU's default NaN is positive, and a compiled
+// a - b may negate b first, NaN included, or swap the operands of a + b, so
+// every NaN is chosen explicitly here.
+__device__ inline double x86_nan(double r, double a, double b) {
+  return isnan(a) ? a : (isnan(b) ? b : (isnan(r) ? -__builtin_nan("") : r));
+}
+
 // PolyEdge::intersect_at (rust/src/geo/edges.rs:50-55): x0 + ((r+0.5) - y0) * dxdy.
-// A NaN produced by an invalid operation (0 * inf, inf - inf) is given the
-// sign x86 SSE gives its default NaN (negative), so total_cmp places it like
-// the reference does; propagated NaNs keep their operand's sign.
+// y0 is never NaN for a kept edge (PolyEdge::new is reached only when
+// |y0 - y1| >= eps, false for NaN), so t is not NaN.
 __device__ inline double crossing(const PolyEdge& e, int32_t r) {
-  double t = ((double)r + 0.5) - e.y0;
+  const double t = ((double)r + 0.5) - e.y0;
   double m = t * e.dxdy;
-  if (isnan(m) && !isnan(t) && !isnan(e.dxdy)) m = -__builtin_nan("");
+  if (isnan(m)) m = isnan(e.dxdy) ? e.dxdy : -__builtin_nan("");
   double x = e.x0 + m;
-  if (isnan(x) && !isnan(e.x0) && !isnan(m)) x = -__builtin_nan("");
+  if (isnan(x)) x = isnan(e.x0) ? e.x0 : (isnan(m) ? m : -__builtin_nan(""));
   return x;
 }
 
@@ -89,7 +99,8 @@ __global__ void k_fill_keys(uint32_t* keys, const uint64_t* n_pairs, uint64_t ca
 // Segmented (by item) min/max reduction across the wave, then one set of
 // atomics per item segment.  Items own contiguous vertex ranges, so lanes of
 // one item are contiguous.
-__device__ inline void reduce_box(int64_t item, ItemBox b, ItemBox* out, bool active) {
+__device__ inline void reduce_box(int64_t item, ItemBox b, uint32_t rows, ItemBox* out,
+                                  unsigned long long* rows_out, bool active) {
   const int lane = threadIdx.x & 63;
   int64_t it = active ? item : -1;
 #pragma unroll
@@ -97,11 +108,13 @@ __device__ inline void reduce_box(int64_t item, ItemBox b, ItemBox* out, bool ac
     int64_t oit = __shfl_up(it, off, 64);
     int32_t r0 = __shfl_up(b.r0, off, 64), r1 = __shfl_up(b.r1, off, 64);
     int32_t c0 = __shfl_up(b.c0, off, 64), c1 = __shfl_up(b.c1, off, 64);
+    uint32_t orows = (uint32_t)__shfl_up((int)rows, off, 64);
     if (lane >= off && oit == it) {
       b.r0 = min(b.r0, r0);
       b.r1 = max(b.r1, r1);
       b.c0 = min(b.c0, c0);
       b.c1 = max(b.c1, c1);
+      rows = min(rows + orows, 0x7FFFFFFFu);  // saturating: a ranking statistic only
     }
   }
   int64_t nit = __shfl_down(it, 1, 64);
@@ -112,6 +125,7 @@ __device__ inline void reduce_box(int64_t item, ItemBox b, ItemBox* out, bool ac
     atomicMin(&out[item].c0, b.c0);
     atomicMax(&out[item].c1, b.c1);
   }
+  if (active && tail && rows) atomicAdd(&rows_out[item], (unsigned long long)rows);
 }
 
 // One thread per vertex of the batch: world -> pixel (edges.rs:94-97),
@@ -123,6 +137,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   const bool active = v < a.n_vertices;
   int64_t item = -1;
   ItemBox box{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  uint32_t crossed = 0;  // polygon edges: rows [ystart, yend) the edge crosses
   if (active) {
     const JobDev& J = a.jobs[find_job_wave(a.job_vend, a.n_jobs, v)];
     const int64_t lv = v - J.v_base;
@@ -134,13 +149,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     const int64_t rbeg = J.ring_voff[ring], rend = J.ring_voff[ring + 1];
     const bool has_next = lv + 1 < rend;
     const double rows = (double)J.nrows, cols = (double)J.ncols;
-    const double x0 = (J.xy[2 * lv] - J.xmin) / J.xres;
-    const double y0 = (J.ymax - J.xy[2 * lv + 1]) / J.yres;
+    // world -> pixel; a NaN coordinate stays that NaN (x86 semantics, x86_nan)
+    auto px_x = [&](double X) { return isnan(X) ? X : (X - J.xmin) / J.xres; };
+    auto px_y = [&](double Y) { return isnan(Y) ? Y : (J.ymax - Y) / J.yres; };
+    const double x0 = px_x(J.xy[2 * lv]);
+    const double y0 = px_y(J.xy[2 * lv + 1]);
     if (kind == 0) {  // polygon rings: extract_ring + PolyEdge::new
       PolyEdge e{0.0, 0.0, 0.0, 0, 0};
       if (has_next) {
-        const double x1 = (J.xy[2 * lv + 2] - J.xmin) / J.xres;
-        const double y1 = (J.ymax - J.xy[2 * lv + 3]) / J.yres;
+        const double x1 = px_x(J.xy[2 * lv + 2]);
+        const double y1 = px_y(J.xy[2 * lv + 3]);
         if (fabs(y0 - y1) >= 2.220446049250313e-16) {  // edges.rs:100
           const double min_y = fmin(y0, y1), max_y = fmax(y0, y1);
           if (min_y < rows && max_y >= 0.0) {  // edges.rs:105
@@ -149,12 +167,15 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             else { xt = x1; yt = y1; xb = x0; yb = y0; }
             e.ystart = sat_usize_cap(ceil(yt - 0.5), J.nrows);
             e.yend = sat_usize_cap(ceil(yb - 0.5), J.nrows);
-            e.dxdy = (xb - xt) / (yb - yt);
+            // (yb - yt is neither NaN nor zero here)
+            const double dx = x86_nan(xb - xt, xb, xt);
+            e.dxdy = x86_nan(dx / (yb - yt), dx, 0.0);
             e.x0 = xt;
             e.y0 = yt;
             if (e.ystart < e.yend) {
               box.r0 = e.ystart;
               box.r1 = e.yend;
+              crossed = (uint32_t)(e.yend - e.ystart);
               // Crossings of this edge lie between its end points up to a few
               // ulps, so their span columns are within one pixel of these.
               const double ax = fabs(xt), bx = fabs(xb);
@@ -180,8 +201,8 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         f |= VF_CLOSED;
       a.vflags[v] = f;
       if (has_next) {
-        const double x1 = (J.xy[2 * lv + 2] - J.xmin) / J.xres;
-        const double y1 = (J.ymax - J.xy[2 * lv + 3]) / J.yres;
+        const double x1 = px_x(J.xy[2 * lv + 2]);
+        const double y1 = px_y(J.xy[2 * lv + 3]);
         const double mnx = fmin(x0, x1), mxx = fmax(x0, x1);
         const double mny = fmin(y0, y1), mxy = fmax(y0, y1);
         if (mnx < cols && mxx >= 0.0 && mny < rows && mxy >= 0.0) {  // edges.rs:130
@@ -204,7 +225,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       }
     }
   }
-  reduce_box(item, box, a.item_box, active);
+  reduce_box(item, box, crossed, a.item_box, a.item_last, active);
 }
 
 // ------------------------------------------------------- item descriptors --
@@ -246,16 +267,45 @@ __global__ void k_item_desc(DescArgs a) {
     for (int k = 0; k < J.esz; ++k) v |= (uint64_t)f[k] << (8 * k);
   }
   d.value = v;
-  d.last = a.item_last[i];
+  const int32_t kind = J.item_kind[li];
+  const bool banded = a.band_n && kind == 0 && d.nv > kBandMinEdges && d.r0 < d.r1 &&
+                      (d.r1 - 1) / kBandRows > d.r0 / kBandRows;
+  d.last = kind == 1 ? a.item_last[i] : 0ull;
   d.job = (uint32_t)j;
   d.band = 0;
-  d.pad1 = 0;
+  d.flags = 0;
+  d.wide = 0;
   a.desc[i] = d;
-  if (a.band_n) {
-    const bool banded = J.item_kind[li] == 0 && d.nv > kBandMinEdges && d.r0 < d.r1 &&
-                        (d.r1 - 1) / kBandRows > d.r0 / kBandRows;
-    a.band_n[i] = banded ? (uint32_t)((d.r1 - 1) / kBandRows - d.r0 / kBandRows + 1) : 0u;
+  if (a.wide_units) {
+    // Work of one strip, from the rows the edges cross (S) over the item's
+    // height (H): an edge crossing s rows meets about (s + 15) / H of the
+    // item's strips, so a strip sees n ~ (S + 15 nv) / H edges.  In a tile,
+    // row-major coverage takes ~n / 4 steps, repeated in each of the tw tiles
+    // of the item's width.  The wide pre-pass does a strip once, and visits a
+    // chunk of 64 loaded edges (all nv, or the n of the strip's band) in the
+    // longest crossing run of its edges when that is shorter (edge-major,
+    // taken as 2 S / nv + 1 rows, at most 16); it writes 16 x nw mask words
+    // per strip.  Wide: a strip takes more than a couple of row-major steps
+    // (n >= 8) and the item is two or more tiles wide or edge-major is
+    // cheaper.
+    uint32_t units = 0, words = 0;
+    if (kind == 0 && d.nv > 0 && d.r0 < d.r1 && d.c0 < d.c1 && a.wide_mode != 1) {
+      const float S = (float)a.item_last[i], nv = (float)d.nv, H = (float)(d.r1 - d.r0);
+      const float n = fminf(nv, (S + 15.0f * nv) / H);
+      const float chunks = ceilf((banded ? n : nv) / 64.0f);
+      const float run = fminf(16.0f, 2.0f * S / nv + 1.0f);
+      const int32_t tw = (d.c1 - 1) / J.tile_w - d.c0 / J.tile_w + 1;
+      if (a.wide_mode == 2 || (n >= 8.0f && (tw >= 2 || chunks * run < n / 4.0f))) {
+        const int32_t ns = wide_strip_count(d.r0, d.r1), nw = wide_row_words(d.c0, d.c1);
+        units = (uint32_t)ns * (uint32_t)((nw + kWideWords - 1) / kWideWords);
+        words = (uint32_t)ns * 16u * (uint32_t)nw;
+      }
+    }
+    a.wide_units[i] = units;
+    a.wide_words[i] = words;
   }
+  if (a.band_n)
+    a.band_n[i] = banded ? (uint32_t)((d.r1 - 1) / kBandRows - d.r0 / kBandRows + 1) : 0u;
   if (a.counts) {
     int32_t tx0, tx1, ty0, ty1;
     uint32_t c = 0;
@@ -274,6 +324,21 @@ __global__ void k_band_assign(ItemDesc* desc, const uint32_t* band_n, const uint
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_items && band_n[i] && band_base[i] + band_n[i] <= cap_entries)
     desc[i].band = (uint32_t)band_base[i] + 1u;
+}
+
+// Wide items whose units and mask words fit the capacities (every item
+// below them in the scans does too) take the wide pre-pass: flag, mask base
+// and one unit-list entry per (strip, window) unit.
+__global__ void k_wide_assign(WideAssignArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_items) return;
+  const uint32_t nu = a.wide_units[i];
+  if (!nu) return;
+  const uint64_t ub = a.unit_base[i], wb = a.word_base[i];
+  if (ub + nu > a.unit_cap || a.word_base[i + 1] > a.word_cap) return;  // stays per tile
+  a.desc[i].flags |= kItemWide;
+  a.desc[i].wide = (uint32_t)(wb >> 4);  // every item's words are a multiple of 16
+  for (uint32_t k = 0; k < nu; ++k) a.unit_item[ub + k] = (uint32_t)i;
 }
 
 // The band entry of edge v's first tile row and the number of tile rows its

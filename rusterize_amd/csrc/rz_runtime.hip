@@ -73,8 +73,10 @@ struct rz_context {
   Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
       heavy_cnt, heavy_base, heavy_pairs, masks, vert_item, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
-      sort_hist, sort_off, sort_tmp, cover_list;
-  uint64_t* host_misc = nullptr;  // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges
+      sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
+      wide_item, wide_masks, scan_tmp5, scan_tmp6;
+  // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words
+  uint64_t* host_misc = nullptr;
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
   // the end of its last upload); tab_dev / tab_dev_bytes say which device
   // buffer holds the staged bytes
@@ -92,13 +94,15 @@ struct rz_context {
   EventSet cur;
   double acc_ms[kStages] = {};
   uint64_t last_pairs = 0, last_tiles = 0, last_band_entries = 0, last_band_edges = 0;
+  uint64_t last_wide_units = 0, last_wide_words = 0;
   bool burns_pending = false;
   std::vector<Buf*> all() {
     return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
             &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue,
             &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
             &heavy_pairs, &masks, &vert_item, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
-            &sort_hist, &sort_off, &sort_tmp, &cover_list};
+            &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
+            &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -171,7 +175,7 @@ extern "C" rz_status rz_context_create(int device, rz_context** out) {
   if (!c) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
   c->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIP_TRY(hipHostMalloc((void**)&c->host_misc, 4 * sizeof(uint64_t), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&c->host_misc, 6 * sizeof(uint64_t), hipHostMallocDefault));
   HIP_TRY(hipEventCreateWithFlags(&c->tab_ev, hipEventDisableTiming));
   *out = c;
   return RZ_OK;
@@ -603,7 +607,7 @@ rz_status check_job_common(const rz_job& j, bool need_out) {
 // indices are global over the batch.
 // Fixed capacities of a captured batch (see rz_batch_graph_create).
 struct BatchCaps {
-  uint64_t pairs, band_entries, band_edges;
+  uint64_t pairs, band_entries, band_edges, wide_units, wide_words;
 };
 
 static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
@@ -623,6 +627,20 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   if (const char* bm = std::getenv("RUSTERIZE_BINNING")) {
     if (std::strcmp(bm, "always") == 0) binning = 1;
     else if (std::strcmp(bm, "never") == 0) binning = -1;
+  }
+  // RUSTERIZE_COVERAGE=rows|edges forces one coverage layout in the wide
+  // pre-pass (tests run both on the same inputs).
+  int cov_mode = 0;
+  if (const char* cm = std::getenv("RUSTERIZE_COVERAGE")) {
+    if (std::strcmp(cm, "rows") == 0) cov_mode = 1;
+    else if (std::strcmp(cm, "edges") == 0) cov_mode = 2;
+  }
+  // RUSTERIZE_WIDE=never|always: no item / every polygon item takes the wide
+  // coverage pre-pass (tests run both on the same inputs).
+  int wide_mode = 0;
+  if (const char* wm = std::getenv("RUSTERIZE_WIDE")) {
+    if (std::strcmp(wm, "never") == 0) wide_mode = 1;
+    else if (std::strcmp(wm, "always") == 0) wide_mode = 2;
   }
   auto has_lines = [](const rz_job& j) { return j.geometry->has_line || j.geometry->has_point; };
   std::vector<int64_t> order((size_t)n_in);
@@ -644,7 +662,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   int64_t* tend = iend + n;
   std::vector<Group> groups;
   int64_t V = 0, I = 0;
-  uint64_t T = 0;
+  uint64_t T = 0, out_px = 0;
   bool any_binned = false, any_poly = false, any_lp = false, any_band = false;
   bool any_tw64 = false, any_tw256 = false;
   int max_tile_w = 64;
@@ -714,6 +732,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     V += dg->V;
     I += dg->I;
     T += jt;
+    out_px += (uint64_t)nrows * (uint64_t)ncols * nl;
     if (T >= ((uint64_t)1 << 29))
       return rz::fail(RZ_CAPACITY_ERROR, "more than 2^29 output tiles in one call or batch");
     groups.back().t_hi = (uint32_t)T;
@@ -746,6 +765,13 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   if (any_binned) {
     HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
+  }
+  const bool any_wide = any_poly && wide_mode != 1 && I > 0;
+  if (any_wide) {
+    HIP_TRY(ensure(c->wide_units, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->wide_words, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->wide_ubase, (size_t)(I + 1) * 8));
+    HIP_TRY(ensure(c->wide_wbase, (size_t)(I + 1) * 8));
   }
   const JobDev* d_jobs = (const JobDev*)c->jobtab.p;
   const int64_t* d_vend = (const int64_t*)(d_jobs + n);
@@ -801,6 +827,10 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   }
   if (any_binned) zero_words((uint32_t*)c->counts.p + I, 1, s);
   if (any_band) zero_words((uint32_t*)c->band_n.p + I, 1, s);
+  if (any_wide) {
+    zero_words((uint32_t*)c->wide_units.p + I, 1, s);
+    zero_words((uint32_t*)c->wide_words.p + I, 1, s);
+  }
   if (I > 0) {
     DescArgs da;
     da.jobs = d_jobs;
@@ -812,6 +842,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     da.desc = (ItemDesc*)c->desc.p;
     da.counts = any_binned ? (uint32_t*)c->counts.p : nullptr;
     da.band_n = any_band ? (uint32_t*)c->band_n.p : nullptr;
+    da.wide_mode = wide_mode;
+    da.wide_units = any_wide ? (uint32_t*)c->wide_units.p : nullptr;
+    da.wide_words = any_wide ? (uint32_t*)c->wide_words.p : nullptr;
     hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
@@ -839,7 +872,51 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       HIP_TRY(hipMemcpyAsync(&c->host_misc[2], (const uint64_t*)c->band_base.p + I, 8,
                              hipMemcpyDeviceToHost, s));
   }
-  if (!capture && (any_binned || any_band)) HIP_TRY(hipStreamSynchronize(s));
+  if (any_wide) {
+    HIP_TRY(ensure(c->scan_tmp5, scan_blocks((uint64_t)I + 1) * 8));
+    HIP_TRY(ensure(c->scan_tmp6, scan_blocks((uint64_t)I + 1) * 8));
+    exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->wide_units.p, (uint64_t*)c->wide_ubase.p,
+                                       (uint64_t)I + 1, (uint64_t*)c->scan_tmp5.p, s);
+    exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->wide_words.p, (uint64_t*)c->wide_wbase.p,
+                                       (uint64_t)I + 1, (uint64_t*)c->scan_tmp6.p, s);
+    if (!capture) {
+      HIP_TRY(hipMemcpyAsync(&c->host_misc[4], (const uint64_t*)c->wide_ubase.p + I, 8,
+                             hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(&c->host_misc[5], (const uint64_t*)c->wide_wbase.p + I, 8,
+                             hipMemcpyDeviceToHost, s));
+    }
+  }
+  if (!capture && (any_binned || any_band || any_wide)) HIP_TRY(hipStreamSynchronize(s));
+  // wide pre-pass: unit list, flags and mask bases (before the descriptors
+  // are copied per pair).  The masks take at most 2 bytes per output pixel
+  // (16 overlapping wide items per pixel at one bit); items past that stay
+  // per tile.
+  uint64_t wide_units = 0, wide_words = 0;
+  c->last_wide_units = c->last_wide_words = 0;
+  if (any_wide) {
+    wide_units = capture ? caps->wide_units : c->host_misc[4];
+    wide_words = capture ? caps->wide_words : c->host_misc[5];
+    c->last_wide_units = wide_units;
+    c->last_wide_words = wide_words;
+    wide_words = std::min<uint64_t>(wide_words, std::max<uint64_t>(out_px / 4, (uint64_t)1 << 20));
+    wide_words = std::min<uint64_t>(wide_words, (uint64_t)1 << 36);  // ItemDesc::wide holds words / 16
+    if (wide_units > 0 && wide_words > 0) {
+      HIP_TRY(ensure(c->wide_item, wide_units * 4));
+      HIP_TRY(ensure(c->wide_masks, wide_words * 8));
+      WideAssignArgs wa;
+      wa.desc = (ItemDesc*)c->desc.p;
+      wa.n_items = I;
+      wa.wide_units = (const uint32_t*)c->wide_units.p;
+      wa.unit_base = (const uint64_t*)c->wide_ubase.p;
+      wa.word_base = (const uint64_t*)c->wide_wbase.p;
+      wa.unit_cap = wide_units;
+      wa.word_cap = wide_words;
+      wa.unit_item = (uint32_t*)c->wide_item.p;
+      hipLaunchKernelGGL(k_wide_assign, dim3(blocks_for(I)), dim3(256), 0, s, wa);
+    } else {
+      wide_units = 0;
+    }
+  }
   c->last_band_entries = c->last_band_edges = 0;
   if (any_band) {
     // edge bands of long polygon rings (see kBandMinEdges)
@@ -994,6 +1071,24 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     HIP_TRY(hipEventRecord(c->ev[2], s));
     HIP_TRY(hipEventRecord(c->ev[3], s));
   }
+  if (wide_units > 0) {
+    WideCoverArgs wc;
+    wc.jobs = d_jobs;
+    wc.desc = (const ItemDesc*)c->desc.p;
+    wc.n_items = I;
+    wc.unit_item = (const uint32_t*)c->wide_item.p;
+    wc.unit_base = (const uint64_t*)c->wide_ubase.p;
+    wc.n_units = (const uint64_t*)c->wide_ubase.p + I;
+    wc.unit_cap = wide_units;
+    wc.edges = (const PolyEdge*)c->edges.p;
+    wc.band_off = ta.band_off;
+    wc.band_edges = ta.band_edges;
+    wc.wide_masks = (uint64_t*)c->wide_masks.p;
+    wc.cov_mode = cov_mode;
+    const uint32_t wgrid = (uint32_t)std::min<uint64_t>((wide_units + 3) / 4, 4096);
+    hipLaunchKernelGGL(k_wide_cover, dim3(wgrid), dim3(256), 0, s, wc);
+    ta.wide_masks = (const uint64_t*)c->wide_masks.p;
+  }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
 
   // 3. tiles: one persistent work-queue launch per group
@@ -1072,7 +1167,8 @@ extern "C" rz_status rz_batch_graph_create(int device, const rz_job* jobs, int64
     return st;
   }
   auto grow = [&](uint64_t v) { return (uint64_t)((double)v * headroom) + (v ? 64 : 0); };
-  const BatchCaps caps{grow(c->last_pairs), grow(c->last_band_entries), grow(c->last_band_edges)};
+  const BatchCaps caps{grow(c->last_pairs), grow(c->last_band_entries), grow(c->last_band_edges),
+                       grow(c->last_wide_units), grow(c->last_wide_words)};
   // 2. a pass at those capacities allocates every buffer, so the captured
   // pass allocates nothing; 3. capture
   st = run_batch(c, jobs, n_jobs, &caps);

@@ -46,10 +46,12 @@ struct InFlight {
 };
 
 // Device memory a request may need besides its output: edge records and
-// coordinates per vertex, descriptors and bins per item.
+// coordinates per vertex, descriptors and bins per item, and the wide
+// pre-pass masks (capped at 2 bytes per output pixel).
 size_t workspace_estimate(const rz_job& j) {
   const rz_device_geometry* g = j.geometry;
-  return (size_t)g->V * 64 + (size_t)g->I * 512;
+  const size_t px = (size_t)j.info.nrows * (size_t)j.info.ncols * (j.layer_dev ? j.n_layers : 1u);
+  return (size_t)g->V * 64 + (size_t)g->I * 512 + 2 * px;
 }
 
 }  // namespace

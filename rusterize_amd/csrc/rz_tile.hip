@@ -101,6 +101,15 @@ __device__ inline uint64_t strip_coverage(const PolyEdge* ebase, uint32_t n_e, i
   return q < NWORD ? row_coverage_word<NWORD>(tog, lp, amax, C0, q) : 0ull;
 }
 
+// Largest value of v (0..16) over the lanes of the wave, from five ballots.
+__device__ inline int32_t wave_max16(int32_t v) {
+  int32_t m = 0;
+#pragma unroll
+  for (int step = 16; step >= 1; step >>= 1)
+    if (__any(v >= m + step)) m += step;
+  return m;
+}
+
 // Persistent work-queue tile kernel, one launch per pixel-function group of
 // a batch (a single call is a batch of one job).
 //
@@ -132,7 +141,7 @@ __device__ inline uint64_t strip_coverage(const PolyEdge* ebase, uint32_t n_e, i
 // choice for 8-byte outputs and the line / point variants.
 template <typename T, bool LINES>
 struct TileOccupancy {
-  static constexpr int value = LINES ? 1 : (sizeof(T) == 4 ? 7 : (sizeof(T) == 8 ? 1 : 6));
+  static constexpr int value = LINES ? 1 : (sizeof(T) == 4 ? 7 : (sizeof(T) == 8 ? 5 : 6));
 };
 
 template <typename T, int FUN, bool LINES>
@@ -228,6 +237,7 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
   const RZ_CONST uint64_t* band_c = as_const(a.band_off);
   const RZ_CONST uint64_t* heavy_c = as_const(a.heavy_base);
   const RZ_CONST uint64_t* mask_c = as_const(a.masks);
+  const RZ_CONST uint64_t* wide_c = as_const(a.wide_masks);
   uint32_t pb = 0, pe = 0;
   auto fetch_range = [&](uint32_t tl) {
     if (a.tile_off) {
@@ -278,7 +288,12 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
       const int64_t vb = d.vbeg, ve = d.vbeg + d.nv;
       if (kind == 0) {
         uint64_t cov = 0;
-        if (hb_ok) {  // heavy tile: the coverage pre-pass wrote this mask
+        if (d.flags & kItemWide) {  // the wide pre-pass wrote the item's masks
+          const int32_t nw = wide_row_words(d.c0, d.c1), wi = (C0 >> 6) + q - (d.c0 >> 6);
+          if (q < NWORD && wi >= 0 && wi < nw)
+            cov = wide_c[(uint64_t)d.wide * 16u +
+                         (uint64_t)(((Rw >> 4) - (d.r0 >> 4)) * 16 + rl) * (uint64_t)nw + (uint64_t)wi];
+        } else if (hb_ok) {  // heavy tile: the coverage pre-pass wrote this mask
           if (q < NWORD)
             cov = mask_c[(hbase + (pi - pb)) * (4u * a.slot_words) + (uint32_t)strip * a.slot_words +
                          (uint32_t)(rl * NWORD + q)];
@@ -460,8 +475,8 @@ __global__ void k_heavy_count(const uint32_t* tile_off, uint32_t n_tiles, uint32
 }
 
 // Slot of every pair of a heavy tile, and the compact list of the slots that
-// hold polygon items (the only ones k_cover computes), appended with one
-// atomic per wave.
+// hold polygon items other than wide ones (the only ones k_cover computes),
+// appended with one atomic per wave.
 __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_desc,
                               const uint64_t* n_pairs, uint64_t cap, const uint32_t* tile_off,
                               const uint64_t* heavy_base, uint32_t* heavy_pairs, uint32_t* cover_list,
@@ -476,7 +491,8 @@ __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_de
     if (heavy_base[t + 1] > hb) {
       slot = hb + (i - tile_off[t]);
       heavy_pairs[slot] = (uint32_t)i;
-      poly = (pair_desc[i].kind_layer & 3u) == 0u;
+      const ItemDesc& d = pair_desc[i];
+      poly = (d.kind_layer & 3u) == 0u && !(d.flags & kItemWide);  // wide items have their masks
     }
   }
   const uint64_t m = __ballot(poly);
@@ -523,6 +539,131 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
         strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, (double)J.ncols, s_edge[wave], lane);
     if ((lane & 3) < NWORD)
       a.masks[hp * 4u * a.slot_words + (uint64_t)strip * a.slot_words + (uint64_t)((lane >> 2) * NWORD + (lane & 3))] = cov;
+  }
+}
+
+// The wide pre-pass: one wave per unit = (kItemWide item, 16-row strip,
+// window of at most kWideWords mask words), grid-stride over the unit list.
+// The strip's edges (its band, or the whole edge list) are visited once for
+// the window, in the cheaper of the two layouts per chunk of 64 edges
+// (row-major: four lanes per row over the staged active edges; edge-major:
+// one lane per edge over the rows it crosses), and every crossing updates
+// the row's LDS state with an atomic (XOR of its toggle bit or of the left
+// parity, max of the column; order-free).  Then one row at a time, lane w
+// turns word w of the toggles into coverage (prefix parity across the words
+// from one ballot, right limit from the rightmost crossing) and stores it.
+__global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
+  __shared__ PolyEdge s_edge[4][64];
+  __shared__ uint64_t s_tog[4][WAVE_ROWS][kWideWords];
+  __shared__ uint32_t s_lp[4][WAVE_ROWS];
+  __shared__ int32_t s_amax[4][WAVE_ROWS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rl = lane >> 2, q = lane & 3;
+  const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
+  const RZ_CONST uint64_t* band_c = as_const(a.band_off);
+  const uint64_t units = min(*a.n_units, a.unit_cap);
+  PolyEdge* stage = s_edge[wave];
+  uint64_t(*tog)[kWideWords] = s_tog[wave];
+  uint32_t* lp = s_lp[wave];
+  int32_t* amax = s_amax[wave];
+  for (uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave; u < units; u += (uint64_t)gridDim.x * 4u) {
+    const uint32_t it = a.unit_item[u];
+    if ((int64_t)it >= a.n_items) continue;  // a unit of an item past the capacities
+    const uint64_t ub = a.unit_base[it];
+    if (u < ub || u >= a.unit_base[it + 1]) continue;
+    const ItemDesc d = load_desc(as_const(a.desc) + it);
+    if (!(d.flags & kItemWide)) continue;
+    const RZ_CONST JobDev& J = jobs_c[d.job];
+    const int32_t nw = wide_row_words(d.c0, d.c1);
+    const int32_t nwin = (nw + kWideWords - 1) / kWideWords;
+    const int32_t local = (int32_t)(u - ub);
+    const int32_t k = local / nwin, w0 = (local - k * nwin) * kWideWords;
+    const int32_t ww = min(kWideWords, nw - w0);
+    const int32_t Rw = ((d.r0 >> 4) + k) * WAVE_ROWS, strip_r1 = min(Rw + WAVE_ROWS, J.nrows);
+    const int32_t W0 = ((d.c0 >> 6) + w0) * 64, WEND = W0 + ww * 64;
+    const double ncols_f = (double)J.ncols;
+    const PolyEdge* ebase = a.edges + d.vbeg;
+    uint32_t n_e = (uint32_t)d.nv;
+    if (d.band) {
+      const uint32_t bi = d.band - 1u + (uint32_t)k;
+      const uint64_t b0 = band_c[bi];
+      ebase = a.band_edges + b0;
+      n_e = (uint32_t)(band_c[bi + 1u] - b0);
+    }
+    for (int idx = lane; idx < WAVE_ROWS * kWideWords; idx += 64) (&tog[0][0])[idx] = 0ull;
+    if (lane < WAVE_ROWS) {
+      lp[lane] = 0u;
+      amax[lane] = -1;
+    }
+    wave_sync();
+    // one crossing of row `row` (0..15) at x
+    auto put = [&](int32_t row, double x) {
+      if (is_pos_nan(x)) return;
+      const int32_t col = span_col(x, ncols_f);
+      const int32_t rel = col - W0;
+      if (rel <= 0) {
+        __hip_atomic_fetch_xor(&lp[row], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        return;
+      }
+      __hip_atomic_fetch_max(&amax[row], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (col < WEND)
+        __hip_atomic_fetch_xor(&tog[row][rel >> 6], 1ull << (rel & 63), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+    };
+    for (uint32_t cb = 0; cb < n_e; cb += 64) {
+      const uint32_t e = cb + (uint32_t)lane;
+      uint4 e_lo = make_uint4(0u, 0u, 0u, 0u), e_hi = make_uint4(0u, 0u, 0u, 0u);
+      int32_t a0 = 0, span = 0;  // rows [a0, a0 + span) of the strip the edge crosses
+      if (e < n_e) {
+        const uint4* src = reinterpret_cast<const uint4*>(ebase + e);
+        e_lo = src[0];
+        e_hi = src[1];  // dxdy, ystart (z), yend (w)
+        a0 = max((int32_t)e_hi.z, Rw);
+        span = max(0, min((int32_t)e_hi.w, strip_r1) - a0);
+      }
+      const uint64_t m = __ballot(span > 0);
+      const int n = __popcll(m);
+      const int32_t steps = wave_max16(span);
+      if (a.cov_mode == 2 || (a.cov_mode == 0 && steps <= (n + 3) / 4)) {  // edge-major
+        PolyEdge ee;
+        __builtin_memcpy(&ee, &e_lo, 16);
+        __builtin_memcpy(reinterpret_cast<char*>(&ee) + 16, &e_hi, 16);
+        for (int32_t j = 0; j < steps; ++j) {
+          const int32_t rr = a0 + j;
+          const double x = crossing(ee, rr);
+          if (j < span) put(rr - Rw, x);
+        }
+        continue;
+      }
+      // row-major over the staged active edges
+      if (span > 0) {
+        uint4* dst = reinterpret_cast<uint4*>(stage + __popcll(m & ((1ull << lane) - 1ull)));
+        dst[0] = e_lo;
+        dst[1] = e_hi;
+      }
+      wave_sync();
+      const int32_t r = Rw + rl;
+      for (int k0 = 0; k0 < n; k0 += 4) {
+        const int kk = k0 + q;
+        const PolyEdge ee = stage[kk < n ? kk : n - 1];
+        const double x = crossing(ee, r);
+        if (kk < n && r < strip_r1 && ee.ystart <= r && r < ee.yend) put(rl, x);
+      }
+      wave_sync();
+    }
+    wave_sync();
+    uint64_t* out = a.wide_masks + (uint64_t)d.wide * 16u + (uint64_t)k * 16u * (uint64_t)nw + (uint64_t)w0;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int row = 0; row < WAVE_ROWS; ++row) {
+      const uint64_t t = lane < ww ? tog[row][lane] : 0ull;
+      const uint64_t odd = __ballot(__popcll(t) & 1);
+      const uint32_t carry = lp[row] ^ ((uint32_t)__popcll(odd & below) & 1u);
+      uint64_t x = prefix_xor64(t) ^ (carry ? ~0ull : 0ull);
+      const int32_t lim = amax[row] - (W0 + 64 * lane);
+      x &= lim <= 0 ? 0ull : (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
+      if (lane < ww) out[(uint64_t)row * (uint64_t)nw + (uint64_t)lane] = x;
+    }
+    wave_sync();  // the state is read before the next unit clears it
   }
 }
 

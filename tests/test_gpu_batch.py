@@ -201,11 +201,15 @@ def test_job_stream_device_generator_matches_oracle(ctx):
             g.free()
 
 
-def test_captured_batch_graph_replays(ctx):
+@pytest.mark.parametrize("wide", [None, "always"])
+def test_captured_batch_graph_replays(ctx, wide, monkeypatch):
     # a captured batch replayed several times, with eager batches between the
     # replays and new field values before one of them; every replay equals the
-    # eager result of the same inputs
+    # eager result of the same inputs (also with every polygon item taking the
+    # wide coverage pre-pass inside the graph)
     from rusterize_amd.device import BatchGraph
+    if wide:
+        monkeypatch.setenv("RUSTERIZE_WIDE", wide)
     b = Batch(ctx, SPECS, seed=2)
     g = None
     try:

@@ -242,3 +242,81 @@ def test_heavy_tiles_precomputed_coverage():
         field = field_for(dtype, g.n_geoms, seed=8)
         got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
         assert_same(got, ref, f"heavy {dtype} {fun}")
+
+
+# ------------------------------------------------ polygon coverage layouts
+
+def polygon_rings(rings_per_geom):
+    """A Geometry of one polygon item per entry; each entry is a list of rings
+    (arrays of (x, y) vertices, closed by the caller)."""
+    gio, kinds, iro, rvo, xy = [0], [], [0], [0], []
+    for rings in rings_per_geom:
+        for r in rings:
+            r = np.asarray(r, dtype=np.float64).reshape(-1, 2)
+            xy.append(r)
+            rvo.append(rvo[-1] + len(r))
+        kinds.append(0)
+        iro.append(iro[-1] + len(rings))
+        gio.append(len(kinds))
+    return Geometry(gio, kinds, iro, rvo, np.concatenate(xy))
+
+
+def degenerate_geometry():
+    """Rings with NaN (of either sign), infinite and huge coordinates: NaN
+    crossings (sorted first or dropped), rows with an odd crossing count,
+    skipped segments and edges clamped to the raster, next to ordinary rings
+    they overlap."""
+    nan, inf = float("nan"), float("inf")
+    star = [(150 + (60 if k % 2 == 0 else 25) * np.cos(k * np.pi / 40),
+             100 + (60 if k % 2 == 0 else 25) * np.sin(k * np.pi / 40)) for k in range(80)]
+    star.append(star[0])
+    return polygon_rings([
+        [[(10, 10), (80, 15), (nan, 60), (20, 90), (10, 10)]],           # NaN vertex
+        [[(100, 5), (140, 5), (inf, 40), (100, 40), (100, 5)]],          # +inf vertex
+        [[(200, 20), (260, 30), (-inf, 70), (200, 20)]],                 # -inf vertex
+        [[(30, 120), (1e300, 130), (60, 190), (30, 120)]],               # huge x
+        [[(250, -1e300), (290, 150), (240, 150), (250, -1e300)]],        # huge y
+        [[(120, 150), (125, nan), (180, 190), (120, 150)]],              # NaN y
+        [[(40, 40), (inf, 45), (inf, 80), (45, 85), (40, 40)]],          # inf - inf slope
+        [[(160, 100), (-nan, 130), (190, 170), (160, 100)]],             # negative NaN x
+        [star],
+        [[(5, 5), (295, 5), (295, 195), (5, 195), (5, 5)],               # frame with a hole
+         [(100, 60), (200, 60), (200, 140), (100, 140), (100, 60)]],
+    ])
+
+
+def test_degenerate_coordinates():
+    g = degenerate_geometry()
+    info = _lib.raster_info(None, shape=(200, 300), extent=(0, 0, 300, 200))
+    for dtype, fun, bg in [("float64", "sum", float("nan")), ("int32", "count", 0), ("uint8", "last", 0)]:
+        field = field_for(dtype, g.n_geoms, seed=9)
+        got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+        assert_same(got, ref, f"degenerate {dtype} {fun}")
+
+
+@pytest.mark.parametrize("wide,layout", [("never", None), ("always", "rows"), ("always", "edges"),
+                                         (None, None)])
+def test_coverage_paths_agree(wide, layout, monkeypatch):
+    # every polygon through one coverage path: in-tile (never wide), or the
+    # wide pre-pass in one layout (row-major or edge-major chunks), on
+    # ordinary, degenerate, banded and heavy-tile inputs, in both item-list
+    # forms
+    for var, val in (("RUSTERIZE_WIDE", wide), ("RUSTERIZE_COVERAGE", layout)):
+        if val:
+            monkeypatch.setenv(var, val)
+    stars = Geometry.synth(4, 51, 60, 1500.0, 1000.0, 20.0, 450.0, 65.0, 3000.0)
+    dense = Geometry.synth(4, 61, 1500, 320.0, 300.0, 2.0, 40.0, 4.0, 600.0)
+    cases = [(mixed_geometry(), (200, 300)), (degenerate_geometry(), (200, 300)),
+             (stars, (1000, 1500)), (dense, (300, 320))]
+    for g, (h, w) in cases:
+        info = _lib.raster_info(None, shape=(h, w), extent=(0, 0, w, h))
+        for binning in [None, "always", "never"]:
+            if binning:
+                monkeypatch.setenv("RUSTERIZE_BINNING", binning)
+            else:
+                monkeypatch.delenv("RUSTERIZE_BINNING", raising=False)
+            for dtype, fun, bg in [("float64", "sum", float("nan")), ("uint8", "count", 0),
+                                   ("int32", "first", 0)]:
+                field = field_for(dtype, g.n_geoms, seed=11)
+                got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+                assert_same(got, ref, f"{wide} {layout} {binning} {g.n_geoms} geoms {dtype} {fun}")

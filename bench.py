@@ -195,7 +195,7 @@ def main() -> int:
     ap.add_argument("--polygons", type=int, default=1_000_000, help="polygon count of the default workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None,
-                    help="DTYPE:FUN -- one warm-up and one timed call of that combination only "
+                    help="DTYPE:FUN -- one warm-up and three timed calls of that combination only "
                          "(for counter collection)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run only a device-to-device copy of this many bytes (the plain copy "
@@ -263,9 +263,10 @@ def main() -> int:
         check(L.rz_context_synchronize(ctx))
         check(L.rz_context_set_profiling(ctx, 1))
         L.rz_context_reset_stage_times(ctx)
-        call(dt, fun)
+        for _ in range(3):
+            call(dt, fun)
         check(L.rz_context_synchronize(ctx))
-        print(json.dumps({"only": args.only, "stages_ms": stage_times(1),
+        print(json.dumps({"only": args.only, "stages_ms": stage_times(3),
                           "b_alg_tile": wl.b_alg_tile(np.dtype(dt).itemsize)}), flush=True)
         return 0
 

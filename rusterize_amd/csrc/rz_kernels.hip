@@ -29,24 +29,19 @@ __device__ inline int64_t sat_isize(double v) {
 // NaN results as the reference's x86 build produces them (SSE): an operation
 // with a NaN operand returns that NaN, sign kept (the first one when both
 // are NaN); an invalid operation (0 * inf, inf - inf, inf / inf) returns the
-// default NaN, which is negative.  The sign decides where f64::total_cmp
-// sorts a NaN crossing.  The GPU's default NaN is positive, and a compiled
-// a - b may negate b first, NaN included, or swap the operands of a + b, so
-// every NaN is chosen explicitly here.
+// default NaN, which is negative.  The GPU's default NaN is positive, and a
+// compiled a - b may negate b first, NaN included, so the edge set-up picks
+// every NaN explicitly.
 __device__ inline double x86_nan(double r, double a, double b) {
   return isnan(a) ? a : (isnan(b) ? b : (isnan(r) ? -__builtin_nan("") : r));
 }
 
 // PolyEdge::intersect_at (rust/src/geo/edges.rs:50-55): x0 + ((r+0.5) - y0) * dxdy.
-// y0 is never NaN for a kept edge (PolyEdge::new is reached only when
-// |y0 - y1| >= eps, false for NaN), so t is not NaN.
+// k_prep leaves no NaN x0 or dxdy in a kept edge, so a NaN here comes from an
+// invalid operation, which is negative on x86: span_col sends it to column 0
+// like any NaN, which is where f64::total_cmp sorts a negative NaN.
 __device__ inline double crossing(const PolyEdge& e, int32_t r) {
-  const double t = ((double)r + 0.5) - e.y0;
-  double m = t * e.dxdy;
-  if (isnan(m)) m = isnan(e.dxdy) ? e.dxdy : -__builtin_nan("");
-  double x = e.x0 + m;
-  if (isnan(x)) x = isnan(e.x0) ? e.x0 : (isnan(m) ? m : -__builtin_nan(""));
-  return x;
+  return e.x0 + (((double)r + 0.5) - e.y0) * e.dxdy;
 }
 
 // ------------------------------------------------------------- item maps --
@@ -172,6 +167,21 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             e.dxdy = x86_nan(dx / (yb - yt), dx, 0.0);
             e.x0 = xt;
             e.y0 = yt;
+            // With a NaN x0 or slope, every crossing of the edge is NaN with
+            // that NaN's sign (x0's when both are NaN).  The reference sorts a
+            // positive NaN crossing last, where it closes an empty span or is
+            // the odd one out: as if it were absent, so the edge is emptied.
+            // A negative one sorts first and spans from column 0, like -inf:
+            // the edge is pinned there.
+            if (isnan(xt) || isnan(e.dxdy)) {
+              if (signbit(isnan(xt) ? xt : e.dxdy)) {
+                e.x0 = -__builtin_inf();
+                e.y0 = 0.0;
+                e.dxdy = 0.0;
+              } else {
+                e.ystart = e.yend = 0;
+              }
+            }
             if (e.ystart < e.yend) {
               box.r0 = e.ystart;
               box.r1 = e.yend;
@@ -279,23 +289,25 @@ __global__ void k_item_desc(DescArgs a) {
   if (a.wide_units) {
     // Work of one strip, from the rows the edges cross (S) over the item's
     // height (H): an edge crossing s rows meets about (s + 15) / H of the
-    // item's strips, so a strip sees n ~ (S + 15 nv) / H edges.  In a tile,
-    // row-major coverage takes ~n / 4 steps, repeated in each of the tw tiles
-    // of the item's width.  The wide pre-pass does a strip once, and visits a
-    // chunk of 64 loaded edges (all nv, or the n of the strip's band) in the
-    // longest crossing run of its edges when that is shorter (edge-major,
-    // taken as 2 S / nv + 1 rows, at most 16); it writes 16 x nw mask words
-    // per strip.  Wide: a strip takes more than a couple of row-major steps
-    // (n >= 8) and the item is two or more tiles wide or edge-major is
-    // cheaper.
+    // item's strips, so a strip sees n ~ (S + 15 nv) / H edges.  In steps of
+    // 64 crossing tests: in a tile, row-major coverage takes n / 4 steps,
+    // repeated in each of the tw tiles of the item's width.  The wide
+    // pre-pass does the strip once, in n / 4 steps or, edge-major, in the
+    // longest crossing run of each chunk of 64 loaded edges (all nv, or the n
+    // of the strip's band; taken as 2 S / nv + 1 rows, at most 16), plus
+    // about 16 steps of set-up and mask stores, and the tiles then read one
+    // mask word each (about half a step).  Wide when that is at least a
+    // quarter cheaper.
     uint32_t units = 0, words = 0;
     if (kind == 0 && d.nv > 0 && d.r0 < d.r1 && d.c0 < d.c1 && a.wide_mode != 1) {
       const float S = (float)a.item_last[i], nv = (float)d.nv, H = (float)(d.r1 - d.r0);
       const float n = fminf(nv, (S + 15.0f * nv) / H);
       const float chunks = ceilf((banded ? n : nv) / 64.0f);
       const float run = fminf(16.0f, 2.0f * S / nv + 1.0f);
-      const int32_t tw = (d.c1 - 1) / J.tile_w - d.c0 / J.tile_w + 1;
-      if (a.wide_mode == 2 || (n >= 8.0f && (tw >= 2 || chunks * run < n / 4.0f))) {
+      const float tw = (float)((d.c1 - 1) / J.tile_w - d.c0 / J.tile_w + 1);
+      const float in_tile = tw * fmaxf(1.0f, n / 4.0f);
+      const float wide = fminf(n / 4.0f, chunks * run) + 16.0f + 0.5f * tw;
+      if (a.wide_mode == 2 || in_tile > 1.25f * wide) {
         const int32_t ns = wide_strip_count(d.r0, d.r1), nw = wide_row_words(d.c0, d.c1);
         units = (uint32_t)ns * (uint32_t)((nw + kWideWords - 1) / kWideWords);
         words = (uint32_t)ns * 16u * (uint32_t)nw;

@@ -90,7 +90,7 @@ __device__ inline uint64_t strip_coverage(const PolyEdge* ebase, uint32_t n_e, i
       const PolyEdge ee = stage[k < n ? k : n - 1];
       const bool on = k < n && row_ok && ee.ystart <= r && r < ee.yend;
       const double x = crossing(ee, r);
-      toggle_crossing_if<NWORD>(on && !is_pos_nan(x), span_col(x, ncols_f), C0, tog, lp, amax);
+      toggle_crossing_if<NWORD>(on, span_col(x, ncols_f), C0, tog, lp, amax);
     }
     wave_sync();
   }
@@ -549,9 +549,10 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
 // (row-major: four lanes per row over the staged active edges; edge-major:
 // one lane per edge over the rows it crosses), and every crossing updates
 // the row's LDS state with an atomic (XOR of its toggle bit or of the left
-// parity, max of the column; order-free).  Then one row at a time, lane w
-// turns word w of the toggles into coverage (prefix parity across the words
-// from one ballot, right limit from the rightmost crossing) and stores it.
+// parity, max of the column; order-free).  Then two rows at a time, lane
+// (half, w) turns word w of the toggles of its row into coverage (prefix
+// parity across the words from one ballot, right limit from the rightmost
+// crossing) and stores it.
 __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
   __shared__ PolyEdge s_edge[4][64];
   __shared__ uint64_t s_tog[4][WAVE_ROWS][kWideWords];
@@ -598,7 +599,6 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
     wave_sync();
     // one crossing of row `row` (0..15) at x
     auto put = [&](int32_t row, double x) {
-      if (is_pos_nan(x)) return;
       const int32_t col = span_col(x, ncols_f);
       const int32_t rel = col - W0;
       if (rel <= 0) {
@@ -652,16 +652,20 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
       wave_sync();
     }
     wave_sync();
+    // two rows per step: lanes 0..31 row 2i, lanes 32..63 row 2i + 1
+    static_assert(kWideWords == 32, "two rows of window words per wave step");
     uint64_t* out = a.wide_masks + (uint64_t)d.wide * 16u + (uint64_t)k * 16u * (uint64_t)nw + (uint64_t)w0;
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (int row = 0; row < WAVE_ROWS; ++row) {
-      const uint64_t t = lane < ww ? tog[row][lane] : 0ull;
+    const int half = lane >> 5, w = lane & 31;
+    const uint64_t below = ((1ull << lane) - 1ull) & (half ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull);
+    for (int r2 = 0; r2 < WAVE_ROWS; r2 += 2) {
+      const int row = r2 + half;
+      const uint64_t t = w < ww ? tog[row][w] : 0ull;
       const uint64_t odd = __ballot(__popcll(t) & 1);
       const uint32_t carry = lp[row] ^ ((uint32_t)__popcll(odd & below) & 1u);
       uint64_t x = prefix_xor64(t) ^ (carry ? ~0ull : 0ull);
-      const int32_t lim = amax[row] - (W0 + 64 * lane);
+      const int32_t lim = amax[row] - (W0 + 64 * w);
       x &= lim <= 0 ? 0ull : (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
-      if (lane < ww) out[(uint64_t)row * (uint64_t)nw + (uint64_t)lane] = x;
+      if (w < ww) out[(uint64_t)row * (uint64_t)nw + (uint64_t)w] = x;
     }
     wave_sync();  // the state is read before the next unit clears it
   }

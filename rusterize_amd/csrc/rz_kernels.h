@@ -120,7 +120,6 @@ struct PrepArgs {
   ItemBox* item_box;        // [n_items]
   unsigned long long* item_last;  // [n_items] lines: last kept segment + 1 (global; 0: none);
                                   // polygons: the sum over edges of the rows each crosses
-  uint32_t* vert_item;      // [n_vertices] global item of every vertex, or null
 };
 
 // Per-call item descriptor (64 B, one scalar load): everything the tile
@@ -208,13 +207,18 @@ struct WideCoverArgs {
   int cov_mode;               // layout per chunk of edges: 0 the cheaper, 1 row-major, 2 edge-major
 };
 
+// Per-item edge-band build: one wave per item (grid-stride), counters of
+// the item's strips in wave-private LDS, at most kBandLds strips per pass
+// (taller items take several passes over their edges).
+constexpr int kBandLds = 2048;
 struct BandArgs {
-  const uint32_t* vert_item;  // [n_vertices] global item of every vertex
-  int64_t n_vertices;
   const ItemDesc* desc;     // [n_items], band fields assigned
+  const uint32_t* band_n;   // [n_items] strips of a banded item
+  const uint32_t* items;    // [*n_list] the banded items with assigned entries
+  const uint32_t* n_list;
   const PolyEdge* edges;    // [n_vertices]
-  uint32_t* band_cnt;       // [n_bands] counts (count pass) / remaining slots (scatter pass)
-  const uint64_t* band_off; // [n_bands + 1] (scatter pass)
+  uint32_t* band_cnt;       // [n_bands] edges per band entry (count pass)
+  const uint64_t* band_off; // [n_bands + 1] exclusive scan of band_cnt (scatter pass)
   PolyEdge* band_edges;     // [edge_cap] (scatter pass)
   uint64_t edge_cap;
   uint32_t* status;         // bit 3 set when the banded edges exceed edge_cap

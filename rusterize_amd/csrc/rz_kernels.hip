@@ -44,6 +44,14 @@ __device__ inline double crossing(const PolyEdge& e, int32_t r) {
   return e.x0 + (((double)r + 0.5) - e.y0) * e.dxdy;
 }
 
+// Orders a wave's LDS accesses around this point (wave-private buffers need
+// no workgroup barrier).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------- item maps --
 
 __global__ void k_item_geom(const int64_t* geom_item_off, int64_t n_geoms, int32_t* item_geom) {
@@ -139,7 +147,6 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     const int32_t ring = J.vert_ring[lv];
     const int32_t litem = J.ring_item[ring];
     item = J.i_base + litem;
-    if (a.vert_item) a.vert_item[v] = (uint32_t)item;
     const int32_t kind = J.item_kind[litem];
     const int64_t rbeg = J.ring_voff[ring], rend = J.ring_voff[ring + 1];
     const bool has_next = lv + 1 < rend;
@@ -331,11 +338,20 @@ __global__ void k_item_desc(DescArgs a) {
 
 // Items whose band entries would pass the capacity stay unbanded (their
 // strips then scan the whole edge list: slower, same result).
+// Assigned items are also appended to the list the band passes walk (any
+// order: every item owns its entries), one atomic per wave.
 __global__ void k_band_assign(ItemDesc* desc, const uint32_t* band_n, const uint64_t* band_base,
-                              int64_t n_items, uint64_t cap_entries) {
+                              int64_t n_items, uint64_t cap_entries, uint32_t* items, uint32_t* n_list) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_items && band_n[i] && band_base[i] + band_n[i] <= cap_entries)
-    desc[i].band = (uint32_t)band_base[i] + 1u;
+  const bool ok = i < n_items && band_n[i] && band_base[i] + band_n[i] <= cap_entries;
+  if (ok) desc[i].band = (uint32_t)band_base[i] + 1u;
+  const uint64_t m = __ballot(ok);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63, leader = __builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(n_list, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, leader, 64);
+  if (ok) items[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
 }
 
 // Wide items whose units and mask words fit the capacities (every item
@@ -353,69 +369,93 @@ __global__ void k_wide_assign(WideAssignArgs a) {
   for (uint32_t k = 0; k < nu; ++k) a.unit_item[ub + k] = (uint32_t)i;
 }
 
-// The band entry of edge v's first tile row and the number of tile rows its
-// rows meet (0: the edge is not in a banded item or is empty).
-__device__ inline uint32_t edge_bands(const BandArgs& a, int64_t v, uint32_t& first) {
-  const ItemDesc& d = a.desc[a.vert_item[v]];
-  if (!d.band) return 0u;
-  const PolyEdge& e = a.edges[v];
-  if (!(e.ystart < e.yend)) return 0u;
-  const int32_t sr0 = d.r0 / kBandRows;
-  const int32_t b0 = max(e.ystart / kBandRows, sr0),
-                b1 = min((e.yend - 1) / kBandRows, (d.r1 - 1) / kBandRows);
-  if (b1 < b0) return 0u;
-  first = d.band - 1u + (uint32_t)(b0 - sr0);
-  return (uint32_t)(b1 - b0 + 1);
+// Strips [b0, b1] (relative to the item's first strip) that edge e's rows
+// [ystart, yend) meet; false for an empty edge.
+__device__ inline bool edge_strips(const PolyEdge& e, int32_t sr0, int32_t sr1, int32_t& b0, int32_t& b1) {
+  if (!(e.ystart < e.yend)) return false;
+  b0 = max(e.ystart / kBandRows, sr0) - sr0;
+  b1 = min((e.yend - 1) / kBandRows, sr1) - sr0;
+  return b0 <= b1;
 }
 
-// Count pass: one thread per vertex; lanes of a wave adding to the same
-// band entry first combine their counts (ballot per distinct key).
+// Count pass, one wave per banded item (of the list k_band_assign built): band_cnt of each of its strips,
+// from a difference array in LDS (+1 at an edge's first strip, -1 after its
+// last) and a running sum, so an edge costs two LDS atomics however many
+// strips it meets.  Plain stores: every item owns its band entries.
 __global__ __launch_bounds__(256) void k_band_count(BandArgs a) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  uint32_t first = 0, nb = 0;
-  if (v < a.n_vertices) nb = edge_bands(a, v, first);
-  uint64_t todo = __ballot(nb > 0);
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const uint32_t key = (uint32_t)__shfl((int)first, leader, 64);
-    const uint64_t m = __ballot(nb > 0 && first == key);
-    if (lane == leader) atomicAdd(&a.band_cnt[key], (uint32_t)__popcll(m));
-    todo &= ~m;
+  __shared__ uint32_t s_cnt[4][kBandLds + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t* cnt = s_cnt[wave];
+  const uint32_t n_list = *a.n_list;
+  for (uint32_t li = blockIdx.x * 4u + (uint32_t)wave; li < n_list; li += gridDim.x * 4u) {
+    const uint32_t i = a.items[li];
+    const uint32_t ns = a.band_n[i];
+    const ItemDesc d = a.desc[i];
+    const int32_t sr0 = d.r0 / kBandRows, sr1 = (d.r1 - 1) / kBandRows;
+    const PolyEdge* eb = a.edges + d.vbeg;
+    for (uint32_t s0 = 0; s0 < ns; s0 += kBandLds) {
+      const uint32_t L = min((uint32_t)kBandLds, ns - s0);
+      for (uint32_t k = lane; k <= L; k += 64) cnt[k] = 0u;
+      wave_sync();
+      for (int32_t e = lane; e < d.nv; e += 64) {
+        int32_t b0, b1;
+        if (!edge_strips(eb[e], sr0, sr1, b0, b1)) continue;
+        b0 = max(b0, (int32_t)s0);
+        b1 = min(b1, (int32_t)(s0 + L - 1));
+        if (b0 > b1) continue;
+        atomicAdd(&cnt[b0 - s0], 1u);
+        atomicSub(&cnt[b1 - s0 + 1], 1u);
+      }
+      wave_sync();
+      uint32_t run = 0;
+      for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        uint32_t x = k < L ? cnt[k] : 0u;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+          if (lane >= off) x += y;
+        }
+        if (k < L) a.band_cnt[d.band - 1u + s0 + k] = run + x;
+        run += (uint32_t)__shfl((int)x, 63, 64);
+      }
+      wave_sync();
+    }
   }
-  for (uint32_t b = 1; b < nb; ++b) atomicAdd(&a.band_cnt[first + b], 1u);
 }
 
-// Scatter pass: copies every edge of a banded item into each of its bands;
-// band_cnt counts down the free slots of every entry.
+// Scatter pass, one wave per banded item: each edge is copied into every
+// strip it meets, at a slot taken from the strip's LDS cursor.
 __global__ __launch_bounds__(256) void k_band_scatter(BandArgs a) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  uint32_t first = 0, nb = 0;
-  if (v < a.n_vertices) nb = edge_bands(a, v, first);
-  PolyEdge e;
-  if (nb) e = a.edges[v];
-  uint64_t todo = __ballot(nb > 0);
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const uint32_t key = (uint32_t)__shfl((int)first, leader, 64);
-    const uint64_t m = __ballot(nb > 0 && first == key);
-    uint32_t old = 0;
-    if (lane == leader) old = atomicSub(&a.band_cnt[key], (uint32_t)__popcll(m));
-    old = (uint32_t)__shfl((int)old, leader, 64);
-    if ((m >> lane) & 1ull) {
-      const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-      const uint64_t pos = a.band_off[key] + (old - (uint32_t)__popcll(m) + rank);
-      if (pos < a.edge_cap) a.band_edges[pos] = e;
-      else atomicOr(a.status, 8u);
+  __shared__ uint32_t s_pos[4][kBandLds];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t* pos = s_pos[wave];
+  const uint32_t n_list = *a.n_list;
+  for (uint32_t li = blockIdx.x * 4u + (uint32_t)wave; li < n_list; li += gridDim.x * 4u) {
+    const uint32_t i = a.items[li];
+    const uint32_t ns = a.band_n[i];
+    const ItemDesc d = a.desc[i];
+    const int32_t sr0 = d.r0 / kBandRows, sr1 = (d.r1 - 1) / kBandRows;
+    const PolyEdge* eb = a.edges + d.vbeg;
+    const uint64_t base = a.band_off[d.band - 1u];
+    for (uint32_t s0 = 0; s0 < ns; s0 += kBandLds) {
+      const uint32_t L = min((uint32_t)kBandLds, ns - s0);
+      for (uint32_t k = lane; k < L; k += 64) pos[k] = (uint32_t)(a.band_off[d.band - 1u + s0 + k] - base);
+      wave_sync();
+      for (int32_t e = lane; e < d.nv; e += 64) {
+        const PolyEdge ed = eb[e];
+        int32_t b0, b1;
+        if (!edge_strips(ed, sr0, sr1, b0, b1)) continue;
+        b0 = max(b0, (int32_t)s0);
+        b1 = min(b1, (int32_t)(s0 + L - 1));
+        for (int32_t b = b0; b <= b1; ++b) {
+          const uint64_t slot = base + atomicAdd(&pos[b - s0], 1u);
+          if (slot < a.edge_cap) a.band_edges[slot] = ed;
+          else atomicOr(a.status, 8u);
+        }
+      }
+      wave_sync();
     }
-    todo &= ~m;
-  }
-  for (uint32_t b = 1; b < nb; ++b) {
-    const uint32_t old = atomicSub(&a.band_cnt[first + b], 1u);
-    const uint64_t pos = a.band_off[first + b] + old - 1u;
-    if (pos < a.edge_cap) a.band_edges[pos] = e;
-    else atomicOr(a.status, 8u);
   }
 }
 

@@ -72,9 +72,9 @@ struct rz_context {
   // workspace
   Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
-      heavy_cnt, heavy_base, heavy_pairs, masks, vert_item, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
+      heavy_cnt, heavy_base, heavy_pairs, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
       sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
-      wide_item, wide_masks, scan_tmp5, scan_tmp6;
+      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items;
   // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words
   uint64_t* host_misc = nullptr;
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
@@ -100,9 +100,9 @@ struct rz_context {
     return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
             &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue,
             &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
-            &heavy_pairs, &masks, &vert_item, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
+            &heavy_pairs, &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
             &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
-            &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6};
+            &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -758,7 +758,6 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   HIP_TRY(ensure(c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
   HIP_TRY(ensure(c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
   if (any_band) {
-    HIP_TRY(ensure(c->vert_item, (size_t)std::max<int64_t>(V, 1) * 4));
     HIP_TRY(ensure(c->band_n, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->band_base, (size_t)(I + 1) * 8));
   }
@@ -822,7 +821,6 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     pa.vflags = (uint8_t*)c->vflags.p;
     pa.item_box = (ItemBox*)c->item_box.p;
     pa.item_last = (unsigned long long*)c->item_last.p;
-    pa.vert_item = any_band ? (uint32_t*)c->vert_item.p : nullptr;
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
   }
   if (any_binned) zero_words((uint32_t*)c->counts.p + I, 1, s);
@@ -927,19 +925,26 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       HIP_TRY(ensure(c->band_off, (nb + 1) * 8));
       HIP_TRY(ensure(c->scan_tmp3, scan_blocks(nb + 1) * 8));
       zero_words(c->band_cnt.p, nb + 1, s);
+      // list of the banded items (band_items), its length after them
+      HIP_TRY(ensure(c->band_items, ((size_t)I + 1) * 4));
+      uint32_t* n_list = (uint32_t*)c->band_items.p + I;
+      zero_words(n_list, 1, s);
       hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I)), dim3(256), 0, s, (ItemDesc*)c->desc.p,
-                         (const uint32_t*)c->band_n.p, (const uint64_t*)c->band_base.p, I, nb);
+                         (const uint32_t*)c->band_n.p, (const uint64_t*)c->band_base.p, I, nb,
+                         (uint32_t*)c->band_items.p, n_list);
       BandArgs bd;
-      bd.vert_item = (const uint32_t*)c->vert_item.p;
-      bd.n_vertices = V;
       bd.desc = (const ItemDesc*)c->desc.p;
+      bd.band_n = (const uint32_t*)c->band_n.p;
+      bd.items = (const uint32_t*)c->band_items.p;
+      bd.n_list = n_list;
       bd.edges = (const PolyEdge*)c->edges.p;
       bd.band_cnt = (uint32_t*)c->band_cnt.p;
       bd.band_off = (const uint64_t*)c->band_off.p;
       bd.band_edges = nullptr;
       bd.edge_cap = 0;
       bd.status = d_status;
-      hipLaunchKernelGGL(k_band_count, dim3(blocks_for(V)), dim3(256), 0, s, bd);
+      const unsigned bgrid = (unsigned)std::min<uint64_t>((nb + 3) / 4, 16384);
+      hipLaunchKernelGGL(k_band_count, dim3(bgrid), dim3(256), 0, s, bd);
       exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->band_cnt.p, (uint64_t*)c->band_off.p,
                                          nb + 1, (uint64_t*)c->scan_tmp3.p, s);
       uint64_t ne = 0;
@@ -955,7 +960,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       HIP_TRY(ensure(c->band_edges, std::max<uint64_t>(ne, 1) * sizeof(PolyEdge)));
       bd.band_edges = (PolyEdge*)c->band_edges.p;
       bd.edge_cap = ne;
-      if (ne > 0) hipLaunchKernelGGL(k_band_scatter, dim3(blocks_for(V)), dim3(256), 0, s, bd);
+      if (ne > 0) hipLaunchKernelGGL(k_band_scatter, dim3(bgrid), dim3(256), 0, s, bd);
       ta.band_off = (const uint64_t*)c->band_off.p;
       ta.band_edges = (const PolyEdge*)c->band_edges.p;
     }

@@ -6,12 +6,6 @@
 
 namespace rzk {
 
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // XOR-combine a value across the four lanes of a quad with DPP quad

@@ -83,6 +83,8 @@ struct JobDev {
 // kernel runs, so the compiler may use scalar loads for uniform indices even
 // though the kernel also stores to global memory.
 #define RZ_CONST __attribute__((address_space(4)))
+// LDS (group segment) pointers, for the direct global -> LDS loads
+#define RZ_LDS __attribute__((address_space(3)))
 template <typename T>
 __device__ inline const RZ_CONST T* as_const(const T* p) {
   return (const RZ_CONST T*)p;
@@ -210,7 +212,7 @@ struct WideCoverArgs {
 // Per-item edge-band build: one wave per item (grid-stride), counters of
 // the item's strips in wave-private LDS, at most kBandLds strips per pass
 // (taller items take several passes over their edges).
-constexpr int kBandLds = 2048;
+constexpr int kBandLds = 1024;
 struct BandArgs {
   const ItemDesc* desc;     // [n_items], band fields assigned
   const uint32_t* band_n;   // [n_items] strips of a banded item
@@ -253,12 +255,20 @@ struct TileArgs {
   const uint64_t* band_off;   // edge bands of banded polygon items
   const PolyEdge* band_edges;
   // heavy tiles: heavy_base[t] .. heavy_base[t + 1] index the tile's items in
-  // the pre-pass masks (null: no heavy tiles); slot_words u64 per strip mask
+  // the pre-pass masks (null: no heavy tiles); slot_words u64 per strip mask,
+  // item j of tile t, strip s at slot heavy_base[t] * 4 + s * (items of t) + j
   const uint64_t* heavy_base;
   const uint64_t* masks;
   uint32_t slot_words;
   uint32_t heavy_min;       // items from which a tile is heavy
   const uint64_t* wide_masks;  // row masks of kItemWide items (null: none)
+  // heavy tiles of polygon-only groups go to k_tile_heavy (heavy_split = 1):
+  // the heavy tiles in tile order, each tile's rank in that list, and the
+  // launch's work counter
+  int heavy_split;
+  const uint32_t* heavy_list;
+  const uint32_t* heavy_rank;  // [n_tiles + 1]
+  uint32_t* heavy_ctr;
   // scan jobs: every item of the job is tested against the tile
   const ItemDesc* desc;
   const PolyEdge* edges;
@@ -288,7 +298,9 @@ struct CoverArgs {
   const PolyEdge* edges;
   const uint64_t* band_off;
   const PolyEdge* band_edges;
-  uint64_t* masks;              // [heavy pairs][4 strips][slot_words]
+  const uint32_t* tile_off;     // [n_tiles + 1]
+  const uint64_t* wide_masks;   // row masks of kItemWide items (copied, not recomputed)
+  uint64_t* masks;              // [4 x heavy pairs][slot_words], strip-major per tile
   uint32_t slot_words;
 };
 

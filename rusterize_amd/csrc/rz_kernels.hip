@@ -369,15 +369,6 @@ __global__ void k_wide_assign(WideAssignArgs a) {
   for (uint32_t k = 0; k < nu; ++k) a.unit_item[ub + k] = (uint32_t)i;
 }
 
-// Strips [b0, b1] (relative to the item's first strip) that edge e's rows
-// [ystart, yend) meet; false for an empty edge.
-__device__ inline bool edge_strips(const PolyEdge& e, int32_t sr0, int32_t sr1, int32_t& b0, int32_t& b1) {
-  if (!(e.ystart < e.yend)) return false;
-  b0 = max(e.ystart / kBandRows, sr0) - sr0;
-  b1 = min((e.yend - 1) / kBandRows, sr1) - sr0;
-  return b0 <= b1;
-}
-
 // Count pass, one wave per banded item (of the list k_band_assign built): band_cnt of each of its strips,
 // from a difference array in LDS (+1 at an edge's first strip, -1 after its
 // last) and a running sum, so an edge costs two LDS atomics however many
@@ -398,10 +389,10 @@ __global__ __launch_bounds__(256) void k_band_count(BandArgs a) {
       for (uint32_t k = lane; k <= L; k += 64) cnt[k] = 0u;
       wave_sync();
       for (int32_t e = lane; e < d.nv; e += 64) {
-        int32_t b0, b1;
-        if (!edge_strips(eb[e], sr0, sr1, b0, b1)) continue;
-        b0 = max(b0, (int32_t)s0);
-        b1 = min(b1, (int32_t)(s0 + L - 1));
+        const int2 yy = *reinterpret_cast<const int2*>(&eb[e].ystart);  // ystart, yend
+        if (!(yy.x < yy.y)) continue;
+        const int32_t b0 = max(max(yy.x / kBandRows, sr0) - sr0, (int32_t)s0);
+        const int32_t b1 = min(min((yy.y - 1) / kBandRows, sr1) - sr0, (int32_t)(s0 + L - 1));
         if (b0 > b1) continue;
         atomicAdd(&cnt[b0 - s0], 1u);
         atomicSub(&cnt[b1 - s0 + 1], 1u);
@@ -443,15 +434,22 @@ __global__ __launch_bounds__(256) void k_band_scatter(BandArgs a) {
       for (uint32_t k = lane; k < L; k += 64) pos[k] = (uint32_t)(a.band_off[d.band - 1u + s0 + k] - base);
       wave_sync();
       for (int32_t e = lane; e < d.nv; e += 64) {
-        const PolyEdge ed = eb[e];
-        int32_t b0, b1;
-        if (!edge_strips(ed, sr0, sr1, b0, b1)) continue;
-        b0 = max(b0, (int32_t)s0);
-        b1 = min(b1, (int32_t)(s0 + L - 1));
+        // the record as two 16-byte vectors (no struct copy through scratch)
+        const uint4* src = reinterpret_cast<const uint4*>(eb + e);
+        const uint4 lo = src[0], hi = src[1];  // hi.z = ystart, hi.w = yend
+        const int32_t ys = (int32_t)hi.z, ye = (int32_t)hi.w;
+        if (!(ys < ye)) continue;
+        const int32_t b0 = max(max(ys / kBandRows, sr0) - sr0, (int32_t)s0);
+        const int32_t b1 = min(min((ye - 1) / kBandRows, sr1) - sr0, (int32_t)(s0 + L - 1));
         for (int32_t b = b0; b <= b1; ++b) {
           const uint64_t slot = base + atomicAdd(&pos[b - s0], 1u);
-          if (slot < a.edge_cap) a.band_edges[slot] = ed;
-          else atomicOr(a.status, 8u);
+          if (slot < a.edge_cap) {
+            uint4* dst = reinterpret_cast<uint4*>(a.band_edges + slot);
+            dst[0] = lo;
+            dst[1] = hi;
+          } else {
+            atomicOr(a.status, 8u);
+          }
         }
       }
       wave_sync();

@@ -74,7 +74,7 @@ struct rz_context {
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
       heavy_cnt, heavy_base, heavy_pairs, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
       sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
-      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items;
+      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7;
   // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words
   uint64_t* host_misc = nullptr;
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
@@ -102,7 +102,7 @@ struct rz_context {
             &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
             &heavy_pairs, &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
             &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
-            &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items};
+            &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items, &heavy_list, &scan_tmp7};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -561,6 +561,50 @@ void launch_tile_fun(int fun, const TileArgs& a, bool lines, hipStream_t s) {
   }
 }
 
+// k_tile_heavy: a persistent grid (the work counter hands out units)
+template <typename T, int FUN>
+void launch_heavy(const TileArgs& a, hipStream_t s) {
+  static uint32_t cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 256, per_cu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_heavy<T, FUN>, 256, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    cached = (uint32_t)(cus * per_cu);
+  }
+  hipLaunchKernelGGL((k_tile_heavy<T, FUN>), dim3(cached), dim3(256), 0, s, a);
+}
+
+template <typename T>
+void launch_heavy_fun(int fun, const TileArgs& a, hipStream_t s) {
+  switch (fun) {
+    case 0: launch_heavy<T, 0>(a, s); break;
+    case 1: launch_heavy<T, 1>(a, s); break;
+    case 2: launch_heavy<T, 2>(a, s); break;
+    case 3: launch_heavy<T, 3>(a, s); break;
+    case 4: launch_heavy<T, 4>(a, s); break;
+    case 5: launch_heavy<T, 5>(a, s); break;
+    default: launch_heavy<T, 6>(a, s); break;
+  }
+}
+
+void launch_heavy_dispatch(rz_dtype dt, int fun, const TileArgs& a, hipStream_t s) {
+  switch (dt) {
+    case RZ_U8: launch_heavy_fun<uint8_t>(fun, a, s); break;
+    case RZ_U16: launch_heavy_fun<uint16_t>(fun, a, s); break;
+    case RZ_U32: launch_heavy_fun<uint32_t>(fun, a, s); break;
+    case RZ_U64: launch_heavy_fun<uint64_t>(fun, a, s); break;
+    case RZ_I8: launch_heavy_fun<int8_t>(fun, a, s); break;
+    case RZ_I16: launch_heavy_fun<int16_t>(fun, a, s); break;
+    case RZ_I32: launch_heavy_fun<int32_t>(fun, a, s); break;
+    case RZ_I64: launch_heavy_fun<int64_t>(fun, a, s); break;
+    case RZ_F32: launch_heavy_fun<float>(fun, a, s); break;
+    default: launch_heavy_fun<double>(fun, a, s); break;
+  }
+}
+
 void launch_tile_dispatch(rz_dtype dt, int fun, const TileArgs& a, bool lines, hipStream_t s) {
   switch (dt) {
     case RZ_U8: launch_tile_fun<uint8_t>(fun, a, lines, s); break;
@@ -747,7 +791,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   HIP_TRY(ensure(c->jobtab, tab_bytes));
   const size_t misc_bytes = 64 + kBurnSlots * sizeof(unsigned long long);
   HIP_TRY(ensure(c->misc, misc_bytes));
-  const size_t queue_words = groups.size() * kQueueWords;
+  // per group: the tile queue, then one word: the k_tile_heavy work counter
+  const size_t queue_words = groups.size() * (kQueueWords + 1);
   HIP_TRY(ensure(c->queue, queue_words * 4));
   uint32_t* d_status = (uint32_t*)c->misc.p;
   unsigned long long* d_burns = (unsigned long long*)((char*)c->misc.p + 64);
@@ -965,6 +1010,27 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ta.band_edges = (const PolyEdge*)c->band_edges.p;
     }
   }
+  // the wide pre-pass (after the bands it reads)
+  auto launch_wide = [&]() {
+    if (wide_units > 0) {
+      WideCoverArgs wc;
+      wc.jobs = d_jobs;
+      wc.desc = (const ItemDesc*)c->desc.p;
+      wc.n_items = I;
+      wc.unit_item = (const uint32_t*)c->wide_item.p;
+      wc.unit_base = (const uint64_t*)c->wide_ubase.p;
+      wc.n_units = (const uint64_t*)c->wide_ubase.p + I;
+      wc.unit_cap = wide_units;
+      wc.edges = (const PolyEdge*)c->edges.p;
+      wc.band_off = ta.band_off;
+      wc.band_edges = ta.band_edges;
+      wc.wide_masks = (uint64_t*)c->wide_masks.p;
+      wc.cov_mode = cov_mode;
+      const uint32_t wgrid = (uint32_t)std::min<uint64_t>((wide_units + 3) / 4, 4096);
+      hipLaunchKernelGGL(k_wide_cover, dim3(wgrid), dim3(256), 0, s, wc);
+      ta.wide_masks = (const uint64_t*)c->wide_masks.p;
+    }
+  };
   if (any_binned) {
     const uint64_t* d_total = (const uint64_t*)c->offsets.p + I;
     n_pairs = capture ? caps->pairs : c->host_misc[0];
@@ -1022,6 +1088,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     ta.tile_off = (const uint32_t*)c->tile_off.p;
     ta.pair_desc = (const ItemDesc*)c->pair_desc.p;
     if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
+    launch_wide();  // before the heavy-tile pre-pass, which copies wide masks
 
     // heavy tiles (kHeavyItems items or more): coverage masks precomputed in
     // parallel, one wave per (item, strip); the buffers are sized for every
@@ -1033,9 +1100,20 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       HIP_TRY(ensure(c->heavy_cnt, ((size_t)n_tiles + 1) * 4));
       HIP_TRY(ensure(c->heavy_base, ((size_t)n_tiles + 1) * 8));
       HIP_TRY(ensure(c->scan_tmp4, scan_blocks((uint64_t)n_tiles + 1) * 8));
+      // heavy flags, their ranks and the ordered list of heavy tiles
+      HIP_TRY(ensure(c->heavy_list, ((size_t)n_tiles + 1) * 4 * 3));
+      uint32_t* h_flag = (uint32_t*)c->heavy_list.p;
+      uint32_t* h_rank = h_flag + (n_tiles + 1);
+      uint32_t* h_list = h_rank + (n_tiles + 1);
+      HIP_TRY(ensure(c->scan_tmp7, scan_blocks((uint64_t)n_tiles + 1) * 4));
       hipLaunchKernelGGL(k_heavy_count, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
                          (const uint32_t*)c->tile_off.p, n_tiles, heavy_min,
-                         (uint32_t*)c->heavy_cnt.p);
+                         (uint32_t*)c->heavy_cnt.p, h_flag);
+      exclusive_scan<uint32_t, uint32_t>(h_flag, h_rank, (uint64_t)n_tiles + 1, (uint32_t*)c->scan_tmp7.p, s);
+      hipLaunchKernelGGL(k_heavy_list, dim3(blocks_for((int64_t)n_tiles)), dim3(256), 0, s,
+                         (const uint32_t*)h_flag, (const uint32_t*)h_rank, n_tiles, h_list);
+      ta.heavy_list = h_list;
+      ta.heavy_rank = h_rank;
       exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->heavy_cnt.p, (uint64_t*)c->heavy_base.p,
                                          (uint64_t)n_tiles + 1, (uint64_t*)c->scan_tmp4.p, s);
       const uint32_t slot_words = 16u * (uint32_t)(max_tile_w / 64);
@@ -1062,6 +1140,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ca.edges = (const PolyEdge*)c->edges.p;
       ca.band_off = ta.band_off;
       ca.band_edges = ta.band_edges;
+      ca.tile_off = (const uint32_t*)c->tile_off.p;
+      ca.wide_masks = (const uint64_t*)c->wide_masks.p;
       ca.masks = (uint64_t*)c->masks.p;
       ca.slot_words = slot_words;
       const uint32_t cgrid = (uint32_t)std::min<uint64_t>(n_pairs, 2048);
@@ -1072,27 +1152,12 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ta.slot_words = slot_words;
       ta.heavy_min = heavy_min;
     }
-  } else if (c->profiling) {
-    HIP_TRY(hipEventRecord(c->ev[2], s));
-    HIP_TRY(hipEventRecord(c->ev[3], s));
-  }
-  if (wide_units > 0) {
-    WideCoverArgs wc;
-    wc.jobs = d_jobs;
-    wc.desc = (const ItemDesc*)c->desc.p;
-    wc.n_items = I;
-    wc.unit_item = (const uint32_t*)c->wide_item.p;
-    wc.unit_base = (const uint64_t*)c->wide_ubase.p;
-    wc.n_units = (const uint64_t*)c->wide_ubase.p + I;
-    wc.unit_cap = wide_units;
-    wc.edges = (const PolyEdge*)c->edges.p;
-    wc.band_off = ta.band_off;
-    wc.band_edges = ta.band_edges;
-    wc.wide_masks = (uint64_t*)c->wide_masks.p;
-    wc.cov_mode = cov_mode;
-    const uint32_t wgrid = (uint32_t)std::min<uint64_t>((wide_units + 3) / 4, 4096);
-    hipLaunchKernelGGL(k_wide_cover, dim3(wgrid), dim3(256), 0, s, wc);
-    ta.wide_masks = (const uint64_t*)c->wide_masks.p;
+  } else {
+    if (c->profiling) {
+      HIP_TRY(hipEventRecord(c->ev[2], s));
+      HIP_TRY(hipEventRecord(c->ev[3], s));
+    }
+    launch_wide();
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
 
@@ -1112,10 +1177,14 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     ta.t_lo = groups[g].t_lo;
     ta.t_hi = groups[g].t_hi;
     ta.queue = (uint32_t*)c->queue.p + g * kQueueWords;
+    ta.heavy_ctr = (uint32_t*)c->queue.p + groups.size() * kQueueWords + g;
+    // polygon-only groups leave their heavy tiles to k_tile_heavy
+    ta.heavy_split = (ta.heavy_base && !groups[g].lines) ? 1 : 0;
     const uint64_t gt = groups[g].t_hi - groups[g].t_lo;
     ta.chunk = tuning_for(size_class(tile_w((int)rz_dtype_size(groups[g].dtype)), gt,
                                      n_tiles ? n_pairs * gt / n_tiles : 0)).chunk;
     launch_tile_dispatch(groups[g].dtype, groups[g].fun, ta, groups[g].lines, s);
+    if (ta.heavy_split) launch_heavy_dispatch(groups[g].dtype, groups[g].fun, ta, s);
   }
   HIP_TRY(hipGetLastError());
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[5], s));

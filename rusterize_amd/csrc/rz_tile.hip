@@ -289,7 +289,7 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
                          (uint64_t)(((Rw >> 4) - (d.r0 >> 4)) * 16 + rl) * (uint64_t)nw + (uint64_t)wi];
         } else if (hb_ok) {  // heavy tile: the coverage pre-pass wrote this mask
           if (q < NWORD)
-            cov = mask_c[(hbase + (pi - pb)) * (4u * a.slot_words) + (uint32_t)strip * a.slot_words +
+            cov = mask_c[(hbase * 4u + (uint64_t)strip * (pe - pb) + (pi - pb)) * a.slot_words +
                          (uint32_t)(rl * NWORD + q)];
         } else {
           // the item's edges, or for a banded item only those of this tile row
@@ -393,7 +393,10 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
     };
 
     if (!J.scan) {
-      for (uint32_t i = pb; i < pe; ++i) process_item(load_desc(pdesc_c + i), i);
+      // (a heavy tile of a polygon-only group is folded by k_tile_heavy,
+      // which runs after this kernel and stores over this strip)
+      const uint32_t pend = (hb_ok && a.heavy_split) ? pb : pe;
+      for (uint32_t i = pb; i < pend; ++i) process_item(load_desc(pdesc_c + i), i);
     } else {
       // scan job: test every item's box in order (small item counts)
       const int64_t ib = J.i_base, ie = J.i_base + J.ni;
@@ -456,8 +459,9 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
 // (item, strip), in parallel; the tile kernel then only folds the masks in
 // geometry order (every pixel function, ordered sum included, stays exact).
 
+// Items of every heavy tile (0 for the others) and its heavy flag.
 __global__ void k_heavy_count(const uint32_t* tile_off, uint32_t n_tiles, uint32_t min_items,
-                              uint32_t* heavy_cnt) {
+                              uint32_t* heavy_cnt, uint32_t* heavy_flag) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > n_tiles) return;
   uint32_t c = 0;
@@ -466,11 +470,20 @@ __global__ void k_heavy_count(const uint32_t* tile_off, uint32_t n_tiles, uint32
     if (c < min_items) c = 0;
   }
   heavy_cnt[t] = c;
+  heavy_flag[t] = c > 0 ? 1u : 0u;
+}
+
+// The heavy tiles in tile order: heavy_list[heavy_rank[t]] = t, so the heavy
+// tiles of tiles [t_lo, t_hi) are heavy_list[heavy_rank[t_lo] .. heavy_rank[t_hi]).
+__global__ void k_heavy_list(const uint32_t* heavy_flag, const uint32_t* heavy_rank, uint32_t n_tiles,
+                             uint32_t* heavy_list) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_tiles && heavy_flag[t]) heavy_list[heavy_rank[t]] = t;
 }
 
 // Slot of every pair of a heavy tile, and the compact list of the slots that
-// hold polygon items other than wide ones (the only ones k_cover computes),
-// appended with one atomic per wave.
+// hold polygon items (the only ones k_cover writes), appended with one
+// atomic per wave.
 __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_desc,
                               const uint64_t* n_pairs, uint64_t cap, const uint32_t* tile_off,
                               const uint64_t* heavy_base, uint32_t* heavy_pairs, uint32_t* cover_list,
@@ -485,8 +498,7 @@ __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_de
     if (heavy_base[t + 1] > hb) {
       slot = hb + (i - tile_off[t]);
       heavy_pairs[slot] = (uint32_t)i;
-      const ItemDesc& d = pair_desc[i];
-      poly = (d.kind_layer & 3u) == 0u && !(d.flags & kItemWide);  // wide items have their masks
+      poly = (pair_desc[i].kind_layer & 3u) == 0u;
     }
   }
   const uint64_t m = __ballot(poly);
@@ -503,6 +515,7 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
   constexpr int TW = 64 * NWORD;
   __shared__ PolyEdge s_edge[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rl = lane >> 2, q = lane & 3;
   const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
   const RZ_CONST uint64_t* band_c = as_const(a.band_off);
   const uint64_t units = 4ull * *a.cover_count;
@@ -521,18 +534,159 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
     const int32_t C0 = tx * TW, Rw = ty * TILE_H + strip * WAVE_ROWS;
     const int32_t strip_r1 = min(Rw + WAVE_ROWS, J.nrows), tile_c1 = min(C0 + TW, J.ncols);
     if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) continue;
-    const PolyEdge* ebase = a.edges + d.vbeg;
-    uint32_t n_e = (uint32_t)d.nv;
-    if (d.band) {
-      const uint32_t bi = d.band - 1u + (uint32_t)(Rw / kBandRows - d.r0 / kBandRows);
-      const uint64_t b0 = band_c[bi];
-      ebase = a.band_edges + b0;
-      n_e = (uint32_t)(band_c[bi + 1u] - b0);
+    uint64_t cov = 0;
+    if (d.flags & kItemWide) {  // already computed by the wide pre-pass: copy
+      const int32_t nw = wide_row_words(d.c0, d.c1), wi = (C0 >> 6) + q - (d.c0 >> 6);
+      if (q < NWORD && wi >= 0 && wi < nw)
+        cov = a.wide_masks[(uint64_t)d.wide * 16u +
+                           (uint64_t)(((Rw >> 4) - (d.r0 >> 4)) * 16 + rl) * (uint64_t)nw + (uint64_t)wi];
+    } else {
+      const PolyEdge* ebase = a.edges + d.vbeg;
+      uint32_t n_e = (uint32_t)d.nv;
+      if (d.band) {
+        const uint32_t bi = d.band - 1u + (uint32_t)(Rw / kBandRows - d.r0 / kBandRows);
+        const uint64_t b0 = band_c[bi];
+        ebase = a.band_edges + b0;
+        n_e = (uint32_t)(band_c[bi + 1u] - b0);
+      }
+      cov = strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, (double)J.ncols, s_edge[wave], lane);
     }
-    const uint64_t cov =
-        strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, (double)J.ncols, s_edge[wave], lane);
-    if ((lane & 3) < NWORD)
-      a.masks[hp * 4u * a.slot_words + (uint64_t)strip * a.slot_words + (uint64_t)((lane >> 2) * NWORD + (lane & 3))] = cov;
+    // strip-major slots: item j of heavy tile t, strip s at heavy_base[t] * 4 + s * cnt + j
+    const uint32_t p0 = a.tile_off[t], cnt = a.tile_off[t + 1] - p0;
+    const uint64_t slot = a.heavy_base[t] * 4u + (uint64_t)strip * cnt + (i - p0);
+    if (q < NWORD) a.masks[slot * a.slot_words + (uint64_t)(rl * NWORD + q)] = cov;
+  }
+}
+
+// Heavy tiles of a polygon-only group (k_tile skips them): one wave per
+// (heavy tile, strip) unit, taken from a work counter.  Every item's strip
+// mask was written by the pre-pass (k_cover), strip-major per tile.  Masks
+// and descriptors go through LDS kBlk items at a time: the next block is
+// loaded into registers while the current block is folded from LDS, then
+// stored to LDS, so the per-item loads of a tile with thousands of items
+// overlap instead of serialising.  Same folds, geometry order and stores as
+// k_tile.
+template <typename T, int FUN>
+__global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
+  constexpr int TW = tile_w(sizeof(T));
+  constexpr int NWORD = TW / 64;
+  constexpr int PPC = 16 / (int)sizeof(T);
+  constexpr int LPR = TW / PPC;
+  constexpr int RSTEP = 64 / LPR;
+  constexpr int RPL = WAVE_ROWS / RSTEP;
+  constexpr int NPIX = RPL * PPC;
+  constexpr int SW = 16 * NWORD;          // words of one item's strip mask
+  constexpr uint32_t kBlk = 256 / SW;     // items per block (16 or 4)
+  __shared__ uint64_t s_mk[4][256];
+  __shared__ ItemDesc s_ds[4][16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int srow = lane / LPR, ch = lane % LPR, ccol = ch * PPC;
+  const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
+  const RZ_CONST int64_t* tend_c = as_const(a.job_tend);
+  // this launch's heavy tiles: heavy_list[h0 .. h1), four units (strips) each
+  const uint32_t h0 = a.heavy_rank[a.t_lo], units = 4u * (a.heavy_rank[a.t_hi] - h0);
+  // units per counter grab: a few per wave of the grid, at most 16
+  const uint32_t grab = max(1u, min(16u, units / (gridDim.x * 4u * 4u)));
+  uint64_t* mk = s_mk[wave];
+  ItemDesc* ds = s_ds[wave];
+  unsigned long long burns = 0;
+  uint32_t u = 0, u_end = 0;
+  for (;;) {
+    if (u == u_end) {
+      uint32_t g = 0;
+      if (lane == 0) g = atomicAdd(a.heavy_ctr, grab);
+      u = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
+      u_end = min(u + grab, units);
+      if (u >= units) break;
+    }
+    const uint32_t t = a.heavy_list[h0 + (u >> 2)];
+    const int32_t strip = (int32_t)(u & 3u);
+    ++u;
+    const RZ_CONST JobDev& J = jobs_c[find_job(tend_c, a.n_jobs, (int64_t)t)];
+    const int32_t nrows = J.nrows, ncols = J.ncols;
+    const T bg = from_bits<T>(J.bg_bits);
+    const uint32_t tj = t - J.tile_base;
+    const uint32_t layer = tj / J.tiles_per_layer;
+    const uint32_t tl = tj - layer * J.tiles_per_layer;
+    const int32_t ty = (int32_t)(tl / (uint32_t)J.ntx), tx = (int32_t)(tl % (uint32_t)J.ntx);
+    const int32_t C0 = tx * TW, Rw = ty * TILE_H + strip * WAVE_ROWS;
+    const int32_t strip_r1 = min(Rw + WAVE_ROWS, nrows), tile_c1 = min(C0 + TW, ncols);
+    const uint32_t pb = a.tile_off[t], cnt = a.tile_off[t + 1] - pb;
+    const uint64_t* sbase = a.masks + (a.heavy_base[t] * 4u + (uint64_t)strip * cnt) * a.slot_words;
+    const ItemDesc* dbase = a.pair_desc + pb;
+
+    T acc[NPIX];
+#pragma unroll
+    for (int j = 0; j < NPIX; ++j) acc[j] = bg;
+
+    // block j0 into registers: masks (lane L: words 4L .. 4L + 3 of the
+    // block) and descriptors (lane L: bytes 16 (L % 4) .. of descriptor L / 4)
+    const uint32_t jm = (uint32_t)(lane * 4) / SW, om = (uint32_t)(lane * 4) % SW, jd = (uint32_t)lane >> 2;
+    uint4 m_lo = make_uint4(0u, 0u, 0u, 0u), m_hi = m_lo, d_q = m_lo;
+    auto load_block = [&](uint32_t j0) {
+      if (j0 + jm < cnt) {
+        const uint4* src = reinterpret_cast<const uint4*>(sbase + (uint64_t)(j0 + jm) * a.slot_words + om);
+        m_lo = src[0];
+        m_hi = src[1];
+      }
+      if (jd < kBlk && j0 + jd < cnt)
+        d_q = reinterpret_cast<const uint4*>(dbase + j0 + jd)[lane & 3];
+    };
+    auto stage_block = [&]() {
+      wave_sync();  // the previous block's reads are done
+      uint4* dm = reinterpret_cast<uint4*>(mk + lane * 4);
+      dm[0] = m_lo;
+      dm[1] = m_hi;
+      if (jd < kBlk) reinterpret_cast<uint4*>(ds + jd)[lane & 3] = d_q;
+      wave_sync();
+    };
+    load_block(0u);
+    for (uint32_t j0 = 0; j0 < cnt; j0 += kBlk) {
+      stage_block();
+      if (j0 + kBlk < cnt) load_block(j0 + kBlk);  // in flight during the folds below
+      const uint32_t gn = min(kBlk, cnt - j0);
+      for (uint32_t g = 0; g < gn; ++g) {
+        const ItemDesc& d = ds[g];
+        if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) continue;
+        const T v = from_bits<T>(d.value);
+        const uint64_t* m = mk + g * SW;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          const uint64_t row_word = m[(srow + k * RSTEP) * NWORD + (ccol >> 6)];
+          const uint32_t bits = (uint32_t)(row_word >> (ccol & 63)) & ((1u << PPC) - 1u);
+          if (__any(bits != 0u)) {
+            burns += (unsigned long long)__popc(bits);
+#pragma unroll
+            for (int j = 0; j < PPC; ++j)
+              acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
+          }
+        }
+      }
+    }
+
+    // write the strip once, as k_tile does
+    const int32_t c = C0 + ccol;
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int32_t row = Rw + srow + k * RSTEP;
+      if (row < nrows) {
+        T* orow = (T*)J.out + ((uint64_t)layer * (uint64_t)nrows + (uint64_t)row) * (uint64_t)ncols;
+        if (c + PPC <= ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
+          u32x4 vec;
+          __builtin_memcpy(&vec, &acc[k * PPC], 16);
+          __builtin_nontemporal_store(vec, (u32x4*)(orow + c));
+        } else {
+#pragma unroll
+          for (int j = 0; j < PPC; ++j)
+            if (c + j < ncols) orow[c + j] = acc[k * PPC + j];
+        }
+      }
+    }
+  }
+  if (a.burns) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) burns += __shfl_xor(burns, off, 64);
+    if (lane == 0 && burns) atomicAdd(&a.burns[blockIdx.x % kBurnSlots], burns);
   }
 }
 

@@ -146,6 +146,15 @@ struct __align__(64) ItemDesc {
 // per strip, and for items of many short edges (which the pre-pass visits
 // edge by edge).
 constexpr uint32_t kItemWide = 1u;
+// kItemOddRows: some row of the polygon may have an odd crossing count (an
+// edge emptied for a positive-NaN slope or origin, or a segment skipped as
+// horizontal that straddles a pixel-centre row).  Only then does a row's
+// coverage need its rightmost crossing: with an even count the prefix
+// parity is already zero from there on.  (Rings are closed, so their
+// crossing counts are even otherwise.)
+constexpr uint32_t kItemOddRows = 2u;
+// k_prep marks such items in bit 63 of their item_last word.
+constexpr unsigned long long kItemOddBit = 1ull << 63;
 
 // Row masks of a kItemWide item: strip k (rows from (r0 / 16 + k) * 16) holds
 // 16 rows of nw = (c1 - 1) / 64 - c0 / 64 + 1 words each (word w covers
@@ -207,6 +216,14 @@ struct WideCoverArgs {
   const PolyEdge* band_edges;
   uint64_t* wide_masks;
   int cov_mode;               // layout per chunk of edges: 0 the cheaper, 1 row-major, 2 edge-major
+  // heavy tiles (null heavy_cnt: none): their items' masks also go to the
+  // heavy slots (see TileArgs), found through the item-ordered pair lists
+  const uint32_t* heavy_cnt;  // [n_tiles] items of a heavy tile, 0 otherwise
+  const uint64_t* heavy_base;
+  const uint32_t* tile_off;
+  const uint32_t* pair_item;  // [n_pairs] item of every pair, sorted by tile
+  uint64_t* masks;
+  uint32_t slot_words;
 };
 
 // Per-item edge-band build: one wave per item (grid-stride), counters of
@@ -299,7 +316,6 @@ struct CoverArgs {
   const uint64_t* band_off;
   const PolyEdge* band_edges;
   const uint32_t* tile_off;     // [n_tiles + 1]
-  const uint64_t* wide_masks;   // row masks of kItemWide items (copied, not recomputed)
   uint64_t* masks;              // [4 x heavy pairs][slot_words], strip-major per tile
   uint32_t slot_words;
 };

@@ -141,6 +141,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   int64_t item = -1;
   ItemBox box{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
   uint32_t crossed = 0;  // polygon edges: rows [ystart, yend) the edge crosses
+  bool odd = false;      // polygon edges: some row may end with an odd crossing count
   if (active) {
     const JobDev& J = a.jobs[find_job_wave(a.job_vend, a.n_jobs, v)];
     const int64_t lv = v - J.v_base;
@@ -158,10 +159,18 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     const double y0 = px_y(J.xy[2 * lv + 1]);
     if (kind == 0) {  // polygon rings: extract_ring + PolyEdge::new
       PolyEdge e{0.0, 0.0, 0.0, 0, 0};
+      // a ring left open by the caller (its last coordinate is not its
+      // first) can leave rows with an odd crossing count
+      if (lv == rbeg && !(J.xy[2 * rbeg] == J.xy[2 * (rend - 1)] && J.xy[2 * rbeg + 1] == J.xy[2 * (rend - 1) + 1]))
+        odd = true;
       if (has_next) {
         const double x1 = px_x(J.xy[2 * lv + 2]);
         const double y1 = px_y(J.xy[2 * lv + 3]);
-        if (fabs(y0 - y1) >= 2.220446049250313e-16) {  // edges.rs:100
+        if (!(fabs(y0 - y1) >= 2.220446049250313e-16)) {
+          // a skipped segment whose end points straddle a pixel-centre row
+          // (or are not numbers) leaves rows with an odd crossing count
+          if (isnan(y0) || isnan(y1) || ceil(fmin(y0, y1) - 0.5) != ceil(fmax(y0, y1) - 0.5)) odd = true;
+        } else {  // edges.rs:100
           const double min_y = fmin(y0, y1), max_y = fmax(y0, y1);
           if (min_y < rows && max_y >= 0.0) {  // edges.rs:105
             double xt, yt, xb, yb;
@@ -186,6 +195,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
                 e.y0 = 0.0;
                 e.dxdy = 0.0;
               } else {
+                if (e.ystart < e.yend) odd = true;  // its rows lose one crossing each
                 e.ystart = e.yend = 0;
               }
             }
@@ -243,6 +253,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     }
   }
   reduce_box(item, box, crossed, a.item_box, a.item_last, active);
+  if (odd) atomicOr(&a.item_last[item], kItemOddBit);
 }
 
 // ------------------------------------------------------- item descriptors --
@@ -292,6 +303,8 @@ __global__ void k_item_desc(DescArgs a) {
   d.band = 0;
   d.flags = 0;
   d.wide = 0;
+  const uint64_t S_raw = a.item_last[i];
+  if (kind == 0 && (S_raw & kItemOddBit)) d.flags |= kItemOddRows;
   a.desc[i] = d;
   if (a.wide_units) {
     // Work of one strip, from the rows the edges cross (S) over the item's
@@ -307,7 +320,7 @@ __global__ void k_item_desc(DescArgs a) {
     // quarter cheaper.
     uint32_t units = 0, words = 0;
     if (kind == 0 && d.nv > 0 && d.r0 < d.r1 && d.c0 < d.c1 && a.wide_mode != 1) {
-      const float S = (float)a.item_last[i], nv = (float)d.nv, H = (float)(d.r1 - d.r0);
+      const float S = (float)(S_raw & ~kItemOddBit), nv = (float)d.nv, H = (float)(d.r1 - d.r0);
       const float n = fminf(nv, (S + 15.0f * nv) / H);
       const float chunks = ceilf((banded ? n : nv) / 64.0f);
       const float run = fminf(16.0f, 2.0f * S / nv + 1.0f);

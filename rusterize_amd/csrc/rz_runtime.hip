@@ -1011,7 +1011,10 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     }
   }
   // the wide pre-pass (after the bands it reads)
-  auto launch_wide = [&]() {
+  // (with heavy tiles: their counts, slot bases, pair lists and slots, so the
+  // wide items' masks also land in the heavy slots)
+  auto launch_wide = [&](const uint32_t* h_cnt, const uint64_t* h_base, const uint32_t* t_off,
+                         const uint32_t* p_item, uint64_t* masks, uint32_t slot_words) {
     if (wide_units > 0) {
       WideCoverArgs wc;
       wc.jobs = d_jobs;
@@ -1026,6 +1029,12 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       wc.band_edges = ta.band_edges;
       wc.wide_masks = (uint64_t*)c->wide_masks.p;
       wc.cov_mode = cov_mode;
+      wc.heavy_cnt = h_cnt;
+      wc.heavy_base = h_base;
+      wc.tile_off = t_off;
+      wc.pair_item = p_item;
+      wc.masks = masks;
+      wc.slot_words = slot_words;
       const uint32_t wgrid = (uint32_t)std::min<uint64_t>((wide_units + 3) / 4, 4096);
       hipLaunchKernelGGL(k_wide_cover, dim3(wgrid), dim3(256), 0, s, wc);
       ta.wide_masks = (const uint64_t*)c->wide_masks.p;
@@ -1088,7 +1097,6 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     ta.tile_off = (const uint32_t*)c->tile_off.p;
     ta.pair_desc = (const ItemDesc*)c->pair_desc.p;
     if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
-    launch_wide();  // before the heavy-tile pre-pass, which copies wide masks
 
     // heavy tiles (kHeavyItems items or more): coverage masks precomputed in
     // parallel, one wave per (item, strip); the buffers are sized for every
@@ -1141,9 +1149,10 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ca.band_off = ta.band_off;
       ca.band_edges = ta.band_edges;
       ca.tile_off = (const uint32_t*)c->tile_off.p;
-      ca.wide_masks = (const uint64_t*)c->wide_masks.p;
       ca.masks = (uint64_t*)c->masks.p;
       ca.slot_words = slot_words;
+      launch_wide((const uint32_t*)c->heavy_cnt.p, (const uint64_t*)c->heavy_base.p,
+                  (const uint32_t*)c->tile_off.p, sorted_vals, (uint64_t*)c->masks.p, slot_words);
       const uint32_t cgrid = (uint32_t)std::min<uint64_t>(n_pairs, 2048);
       if (any_tw64) hipLaunchKernelGGL(k_cover<1>, dim3(cgrid), dim3(256), 0, s, ca);
       if (any_tw256) hipLaunchKernelGGL(k_cover<4>, dim3(cgrid), dim3(256), 0, s, ca);
@@ -1151,13 +1160,15 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ta.masks = (const uint64_t*)c->masks.p;
       ta.slot_words = slot_words;
       ta.heavy_min = heavy_min;
+    } else {
+      launch_wide(nullptr, nullptr, nullptr, nullptr, nullptr, 0);
     }
   } else {
     if (c->profiling) {
       HIP_TRY(hipEventRecord(c->ev[2], s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
     }
-    launch_wide();
+    launch_wide(nullptr, nullptr, nullptr, nullptr, nullptr, 0);
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
 

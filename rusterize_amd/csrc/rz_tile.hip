@@ -482,8 +482,8 @@ __global__ void k_heavy_list(const uint32_t* heavy_flag, const uint32_t* heavy_r
 }
 
 // Slot of every pair of a heavy tile, and the compact list of the slots that
-// hold polygon items (the only ones k_cover writes), appended with one
-// atomic per wave.
+// hold polygon items other than wide ones (the only ones k_cover writes;
+// k_wide_cover writes the wide ones), appended with one atomic per wave.
 __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_desc,
                               const uint64_t* n_pairs, uint64_t cap, const uint32_t* tile_off,
                               const uint64_t* heavy_base, uint32_t* heavy_pairs, uint32_t* cover_list,
@@ -498,7 +498,8 @@ __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_de
     if (heavy_base[t + 1] > hb) {
       slot = hb + (i - tile_off[t]);
       heavy_pairs[slot] = (uint32_t)i;
-      poly = (pair_desc[i].kind_layer & 3u) == 0u;
+      const ItemDesc& d = pair_desc[i];
+      poly = (d.kind_layer & 3u) == 0u && !(d.flags & kItemWide);  // k_wide_cover writes those
     }
   }
   const uint64_t m = __ballot(poly);
@@ -534,23 +535,15 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
     const int32_t C0 = tx * TW, Rw = ty * TILE_H + strip * WAVE_ROWS;
     const int32_t strip_r1 = min(Rw + WAVE_ROWS, J.nrows), tile_c1 = min(C0 + TW, J.ncols);
     if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) continue;
-    uint64_t cov = 0;
-    if (d.flags & kItemWide) {  // already computed by the wide pre-pass: copy
-      const int32_t nw = wide_row_words(d.c0, d.c1), wi = (C0 >> 6) + q - (d.c0 >> 6);
-      if (q < NWORD && wi >= 0 && wi < nw)
-        cov = a.wide_masks[(uint64_t)d.wide * 16u +
-                           (uint64_t)(((Rw >> 4) - (d.r0 >> 4)) * 16 + rl) * (uint64_t)nw + (uint64_t)wi];
-    } else {
-      const PolyEdge* ebase = a.edges + d.vbeg;
-      uint32_t n_e = (uint32_t)d.nv;
-      if (d.band) {
-        const uint32_t bi = d.band - 1u + (uint32_t)(Rw / kBandRows - d.r0 / kBandRows);
-        const uint64_t b0 = band_c[bi];
-        ebase = a.band_edges + b0;
-        n_e = (uint32_t)(band_c[bi + 1u] - b0);
-      }
-      cov = strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, (double)J.ncols, s_edge[wave], lane);
+    const PolyEdge* ebase = a.edges + d.vbeg;
+    uint32_t n_e = (uint32_t)d.nv;
+    if (d.band) {
+      const uint32_t bi = d.band - 1u + (uint32_t)(Rw / kBandRows - d.r0 / kBandRows);
+      const uint64_t b0 = band_c[bi];
+      ebase = a.band_edges + b0;
+      n_e = (uint32_t)(band_c[bi + 1u] - b0);
     }
+    const uint64_t cov = strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, (double)J.ncols, s_edge[wave], lane);
     // strip-major slots: item j of heavy tile t, strip s at heavy_base[t] * 4 + s * cnt + j
     const uint32_t p0 = a.tile_off[t], cnt = a.tile_off[t + 1] - p0;
     const uint64_t slot = a.heavy_base[t] * 4u + (uint64_t)strip * cnt + (i - p0);
@@ -703,7 +696,9 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
 // crossing) and stores it.
 __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
   __shared__ PolyEdge s_edge[4][64];
-  __shared__ uint64_t s_tog[4][WAVE_ROWS][kWideWords];
+  // rows padded by one word, so the same word of different rows falls in
+  // different LDS banks
+  __shared__ uint64_t s_tog[4][WAVE_ROWS][kWideWords + 1];
   __shared__ uint32_t s_lp[4][WAVE_ROWS];
   __shared__ int32_t s_amax[4][WAVE_ROWS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -712,7 +707,7 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
   const RZ_CONST uint64_t* band_c = as_const(a.band_off);
   const uint64_t units = min(*a.n_units, a.unit_cap);
   PolyEdge* stage = s_edge[wave];
-  uint64_t(*tog)[kWideWords] = s_tog[wave];
+  uint64_t(*tog)[kWideWords + 1] = s_tog[wave];
   uint32_t* lp = s_lp[wave];
   int32_t* amax = s_amax[wave];
   for (uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave; u < units; u += (uint64_t)gridDim.x * 4u) {
@@ -739,7 +734,9 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
       ebase = a.band_edges + b0;
       n_e = (uint32_t)(band_c[bi + 1u] - b0);
     }
-    for (int idx = lane; idx < WAVE_ROWS * kWideWords; idx += 64) (&tog[0][0])[idx] = 0ull;
+    for (int idx = lane; idx < WAVE_ROWS * (kWideWords + 1); idx += 64) (&tog[0][0])[idx] = 0ull;
+    // the rightmost crossing of a row matters only with an odd count
+    const bool odd_rows = (d.flags & kItemOddRows) != 0;
     if (lane < WAVE_ROWS) {
       lp[lane] = 0u;
       amax[lane] = -1;
@@ -753,7 +750,7 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
         __hip_atomic_fetch_xor(&lp[row], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         return;
       }
-      __hip_atomic_fetch_max(&amax[row], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (odd_rows) __hip_atomic_fetch_max(&amax[row], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       if (col < WEND)
         __hip_atomic_fetch_xor(&tog[row][rel >> 6], 1ull << (rel & 63), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -804,6 +801,29 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
     static_assert(kWideWords == 32, "two rows of window words per wave step");
     uint64_t* out = a.wide_masks + (uint64_t)d.wide * 16u + (uint64_t)k * 16u * (uint64_t)nw + (uint64_t)w0;
     const int half = lane >> 5, w = lane & 31;
+    // A heavy tile's fold reads its items' masks from its pre-pass slots, so
+    // each word also goes to the slot of the tile that holds it when that
+    // tile is heavy: tile column gw / (TW / 64), the item's place j in the
+    // tile's (item-ordered) pair list found by binary search.
+    uint64_t* slot = nullptr;
+    if (a.heavy_cnt && !J.scan && w < ww) {
+      const int32_t nwt = J.tile_w >> 6;  // words per tile row
+      const int32_t gw = (W0 >> 6) + w, tx = gw / nwt;
+      const uint32_t t = J.tile_base + (d.kind_layer >> 2) * J.tiles_per_layer +
+                         (uint32_t)(Rw / TILE_H) * (uint32_t)J.ntx + (uint32_t)tx;
+      const uint32_t cnt = a.heavy_cnt[t];
+      if (cnt) {
+        uint32_t lo = a.tile_off[t], hi = lo + cnt;  // pair_item[lo .. hi) ascending
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (a.pair_item[mid] < it) lo = mid + 1;
+          else hi = mid;
+        }
+        const uint32_t j = lo - a.tile_off[t];
+        const uint32_t s = (uint32_t)((Rw % TILE_H) / WAVE_ROWS);
+        slot = a.masks + (a.heavy_base[t] * 4u + (uint64_t)s * cnt + j) * a.slot_words + (uint32_t)(gw % nwt);
+      }
+    }
     const uint64_t below = ((1ull << lane) - 1ull) & (half ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull);
     for (int r2 = 0; r2 < WAVE_ROWS; r2 += 2) {
       const int row = r2 + half;
@@ -811,9 +831,20 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
       const uint64_t odd = __ballot(__popcll(t) & 1);
       const uint32_t carry = lp[row] ^ ((uint32_t)__popcll(odd & below) & 1u);
       uint64_t x = prefix_xor64(t) ^ (carry ? ~0ull : 0ull);
-      const int32_t lim = amax[row] - (W0 + 64 * w);
+      const int32_t lim = odd_rows ? amax[row] - (W0 + 64 * w) : 64;
       x &= lim <= 0 ? 0ull : (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
       if (w < ww) out[(uint64_t)row * (uint64_t)nw + (uint64_t)w] = x;
+      if (slot) {
+        // the tile's words left of the item's first word or right of its
+        // last one are empty: written as such (the fold reads every word)
+        const int32_t nwt = J.tile_w >> 6, gw = (W0 >> 6) + w, qq = gw % nwt;
+        uint64_t* srow_p = slot + row * nwt - qq;
+        srow_p[qq] = x;
+        if (gw == (d.c0 >> 6))
+          for (int z = 0; z < qq; ++z) srow_p[z] = 0ull;
+        if (gw == ((d.c1 - 1) >> 6))
+          for (int z = qq + 1; z < nwt; ++z) srow_p[z] = 0ull;
+      }
     }
     wave_sync();  // the state is read before the next unit clears it
   }

@@ -183,7 +183,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines", "batch", "serve"])
-    ap.add_argument("--rates", default="50,200,500,1000,2000", help="serve workload: arrival rates (requests/s)")
+    ap.add_argument("--rates", default="50,200,500,1000,2000,3000,4000",
+                    help="serve workload: arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=200, help="serve workload: requests per rate")
     ap.add_argument("--pool", type=int, default=48, help="serve workload: distinct geometry sets (stream jobs)")
     ap.add_argument("--jobs", type=int, default=256, help="batch workload: jobs of the seeded job stream")

@@ -107,8 +107,8 @@ __device__ inline int find_job(P ends, int n, int64_t x) {
 // its own only in a lane past that job's end (rare: jobs are long).
 __device__ inline int find_job_wave(const int64_t* ends, int n, int64_t x) {
   const int64_t x0 = __builtin_amdgcn_readfirstlane(x);
-  const int j0 = find_job(ends, n, x0);
-  return x < ends[j0] ? j0 : find_job(ends, n, x);
+  const int j0 = find_job(as_const(ends), n, x0);  // uniform: scalar loads
+  return x < as_const(ends)[j0] ? j0 : find_job(ends, n, x);
 }
 
 struct PrepArgs {

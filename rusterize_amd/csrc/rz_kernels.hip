@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   uint32_t crossed = 0;  // polygon edges: rows [ystart, yend) the edge crosses
   bool odd = false;      // polygon edges: some row may end with an odd crossing count
   if (active) {
-    const JobDev& J = a.jobs[find_job_wave(a.job_vend, a.n_jobs, v)];
+    const RZ_CONST JobDev& J = as_const(a.jobs)[find_job_wave(a.job_vend, a.n_jobs, v)];
     const int64_t lv = v - J.v_base;
     const int32_t ring = J.vert_ring[lv];
     const int32_t litem = J.ring_item[ring];
@@ -275,7 +275,7 @@ __global__ void k_item_desc(DescArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_items) return;
   const int j = find_job_wave(a.job_iend, a.n_jobs, i);
-  const JobDev& J = a.jobs[j];
+  const RZ_CONST JobDev& J = as_const(a.jobs)[j];
   const int64_t li = i - J.i_base;
   const ItemBox b = a.item_box[i];
   const int32_t g = J.item_geom[li];

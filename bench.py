@@ -186,6 +186,8 @@ def main() -> int:
     ap.add_argument("--rates", default="50,200,500,1000,2000,3000,4000",
                     help="serve workload: arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=200, help="serve workload: requests per rate")
+    ap.add_argument("--streams", type=int, default=2, help="serve workload: server worker streams")
+    ap.add_argument("--max-batch", type=int, default=64, help="serve workload: largest batch")
     ap.add_argument("--pool", type=int, default=48, help="serve workload: distinct geometry sets (stream jobs)")
     ap.add_argument("--jobs", type=int, default=256, help="batch workload: jobs of the seeded job stream")
     ap.add_argument("--autotune", type=int, default=0,
@@ -519,10 +521,11 @@ def serve_bench(args) -> int:
     """The served workload: Poisson arrivals at each rate, requests drawn from
     the seeded job stream (a pool of geometry sets generated on the device);
     30 % of requests reuse a cached geometry set with a new extent, field and
-    function.  Served = rz_server with continuous batching over 2 streams
-    under a 200 GB budget; plain = the same arrivals served one call at a time
-    (1 stream, batches of 1).  Latency is submission to completion, measured
-    by the server."""
+    function.  Served = rz_server with continuous batching over --streams
+    worker streams (batches of up to --max-batch) under a 200 GB budget; plain = the same arrivals served one call at a time
+    (1 stream, batches of 1).  Both servers are warmed up on the pool once
+    and then serve every rate in turn.  Latency is submission to completion,
+    measured by the server."""
     import threading
 
     from rusterize_amd.device import Context, DeviceBuffer, DeviceGeometry, Job, Server, grid, rasterize_device
@@ -577,9 +580,23 @@ def serve_bench(args) -> int:
         burns[rate] = tot
     ctx.set_profiling(0)
 
-    def run(rate, n_streams, max_batch):
+    def make_server(n_streams, max_batch):
+        """A server warmed up on every geometry set of the pool once (its
+        contexts' workspaces and output buffers grown), as a long-running
+        server would be."""
         srv = Server(0, budget_bytes=200 * 10**9, n_streams=n_streams, max_batch=max_batch)
+        warm = []
+        for sp, dg, f0, _ in pool:
+            warm.append(srv.submit(Job(dg, grid(sp["side"], sp["side"]), "float64", sp["fun"], None, field=f0,
+                                       background=float("nan"))))
+        for t in warm:
+            srv.wait(t)
+            srv.release(t)
+        return srv
+
+    def run(srv, rate):
         reqs = scheds[rate]
+        st0 = srv.stats()
         tickets = [None] * len(reqs)
         t0 = time.perf_counter()
 
@@ -602,21 +619,25 @@ def serve_bench(args) -> int:
         th.join()
         wall = time.perf_counter() - t0
         st = srv.stats()
-        srv.close()
         lat = np.array(lat)
         return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "jobs_per_s": round(len(reqs) / wall, 1), "burns_per_s": burns[rate] / wall,
-                "wall_s": round(wall, 3), "batches": st["batches"], "peak_memory_GB": round(st["memory_peak"] / 1e9, 2)}
+                "wall_s": round(wall, 3), "batches": st["batches"] - st0["batches"],
+                "peak_memory_GB": round(st["memory_peak"] / 1e9, 2)}
 
+    served_srv = make_server(args.streams, args.max_batch)
+    plain_srv = make_server(1, 1)
     table = {}
     for rate in rates:
-        served = run(rate, 2, 64)
-        plain = run(rate, 1, 1)
+        served = run(served_srv, rate)
+        plain = run(plain_srv, rate)
         table[str(int(rate))] = {"served": served, "plain": plain,
                                  "served_over_plain_throughput": round(served["burns_per_s"] / plain["burns_per_s"], 3)}
         log(f"rate {rate:.0f}/s: served p50 {served['p50_ms']} p99 {served['p99_ms']} ms, "
             f"{served['jobs_per_s']} jobs/s; plain p50 {plain['p50_ms']} p99 {plain['p99_ms']} ms, "
             f"{plain['jobs_per_s']} jobs/s")
+    served_srv.close()
+    plain_srv.close()
     for sp, dg, f0, alt in pool:
         f0.free()
         alt.free()
@@ -639,7 +660,7 @@ def serve_bench(args) -> int:
         "config": {"workload": f"served: Poisson arrivals over the seeded job stream (pool of {len(pool)} geometry "
                                f"sets, 30% reuse with a new extent/field/function), 200 GB budget, "
                                f"{args.requests} requests per rate",
-                   "rates": rates},
+                   "rates": rates, "streams": args.streams, "max_batch": args.max_batch},
         "by_rate": table,
     }
     print(json.dumps(result), flush=True)

@@ -42,16 +42,23 @@ struct Buf {
   size_t cap = 0;
 };
 
-hipError_t ensure(Buf& b, size_t bytes) {
+// Grows a workspace buffer to at least `bytes`, in stream order on s (the
+// old buffer is freed and the new one allocated after the work already
+// queued on s, from the stream-ordered pool: no device-wide synchronisation,
+// so other contexts' streams keep running).
+hipError_t ensure(hipStream_t s, Buf& b, size_t bytes) {
   if (bytes <= b.cap && b.p) return hipSuccess;
+  const size_t b_old_cap = b.cap;
   if (b.p) {
-    hipError_t e = hipFree(b.p);
+    hipError_t e = hipFreeAsync(b.p, s);
     if (e != hipSuccess) return e;
     b.p = nullptr;
     b.cap = 0;
   }
-  size_t cap = std::max<size_t>(bytes, 256);
-  hipError_t e = hipMalloc(&b.p, cap);
+  // grow by at least half the old capacity: a workspace that keeps growing
+  // (batches of varying size) is reallocated a logarithmic number of times
+  size_t cap = std::max<size_t>(std::max<size_t>(bytes, 256), b_old_cap + b_old_cap / 2);
+  hipError_t e = hipMallocAsync(&b.p, cap, s);
   if (e == hipSuccess) b.cap = cap;
   return e;
 }
@@ -788,34 +795,34 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   const uint32_t n_tiles = (uint32_t)T;
 
   // workspace
-  HIP_TRY(ensure(c->jobtab, tab_bytes));
+  HIP_TRY(ensure(c->stream, c->jobtab, tab_bytes));
   const size_t misc_bytes = 64 + kBurnSlots * sizeof(unsigned long long);
-  HIP_TRY(ensure(c->misc, misc_bytes));
+  HIP_TRY(ensure(c->stream, c->misc, misc_bytes));
   // per group: the tile queue, then one word: the k_tile_heavy work counter
   const size_t queue_words = groups.size() * (kQueueWords + 1);
-  HIP_TRY(ensure(c->queue, queue_words * 4));
+  HIP_TRY(ensure(c->stream, c->queue, queue_words * 4));
   uint32_t* d_status = (uint32_t*)c->misc.p;
   unsigned long long* d_burns = (unsigned long long*)((char*)c->misc.p + 64);
-  if (any_poly) HIP_TRY(ensure(c->edges, (size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)));
-  if (any_lp) HIP_TRY(ensure(c->px, (size_t)std::max<int64_t>(V, 1) * 16));
-  if (any_lp) HIP_TRY(ensure(c->vflags, (size_t)std::max<int64_t>(V, 1)));
-  HIP_TRY(ensure(c->item_box, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox)));
-  HIP_TRY(ensure(c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
-  HIP_TRY(ensure(c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
+  if (any_poly) HIP_TRY(ensure(c->stream, c->edges, (size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)));
+  if (any_lp) HIP_TRY(ensure(c->stream, c->px, (size_t)std::max<int64_t>(V, 1) * 16));
+  if (any_lp) HIP_TRY(ensure(c->stream, c->vflags, (size_t)std::max<int64_t>(V, 1)));
+  HIP_TRY(ensure(c->stream, c->item_box, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox)));
+  HIP_TRY(ensure(c->stream, c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
+  HIP_TRY(ensure(c->stream, c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
   if (any_band) {
-    HIP_TRY(ensure(c->band_n, (size_t)(I + 1) * 4));
-    HIP_TRY(ensure(c->band_base, (size_t)(I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->band_n, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->stream, c->band_base, (size_t)(I + 1) * 8));
   }
   if (any_binned) {
-    HIP_TRY(ensure(c->counts, (size_t)(I + 1) * 4));
-    HIP_TRY(ensure(c->offsets, (size_t)(I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->counts, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->stream, c->offsets, (size_t)(I + 1) * 8));
   }
   const bool any_wide = any_poly && wide_mode != 1 && I > 0;
   if (any_wide) {
-    HIP_TRY(ensure(c->wide_units, (size_t)(I + 1) * 4));
-    HIP_TRY(ensure(c->wide_words, (size_t)(I + 1) * 4));
-    HIP_TRY(ensure(c->wide_ubase, (size_t)(I + 1) * 8));
-    HIP_TRY(ensure(c->wide_wbase, (size_t)(I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->wide_units, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->stream, c->wide_words, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->stream, c->wide_ubase, (size_t)(I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->wide_wbase, (size_t)(I + 1) * 8));
   }
   const JobDev* d_jobs = (const JobDev*)c->jobtab.p;
   const int64_t* d_vend = (const int64_t*)(d_jobs + n);
@@ -900,7 +907,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   std::memset(&ta, 0, sizeof(ta));
   uint64_t n_pairs = 0;
   if (any_binned) {
-    HIP_TRY(ensure(c->scan_tmp, scan_blocks((uint64_t)I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->scan_tmp, scan_blocks((uint64_t)I + 1) * 8));
     exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->counts.p, (uint64_t*)c->offsets.p,
                                        (uint64_t)I + 1, (uint64_t*)c->scan_tmp.p, s);
     if (!capture)
@@ -908,7 +915,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                              hipMemcpyDeviceToHost, s));
   }
   if (any_band) {
-    HIP_TRY(ensure(c->scan_tmp2, scan_blocks((uint64_t)I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->scan_tmp2, scan_blocks((uint64_t)I + 1) * 8));
     exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->band_n.p, (uint64_t*)c->band_base.p,
                                        (uint64_t)I + 1, (uint64_t*)c->scan_tmp2.p, s);
     if (!capture)
@@ -916,8 +923,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                              hipMemcpyDeviceToHost, s));
   }
   if (any_wide) {
-    HIP_TRY(ensure(c->scan_tmp5, scan_blocks((uint64_t)I + 1) * 8));
-    HIP_TRY(ensure(c->scan_tmp6, scan_blocks((uint64_t)I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->scan_tmp5, scan_blocks((uint64_t)I + 1) * 8));
+    HIP_TRY(ensure(c->stream, c->scan_tmp6, scan_blocks((uint64_t)I + 1) * 8));
     exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->wide_units.p, (uint64_t*)c->wide_ubase.p,
                                        (uint64_t)I + 1, (uint64_t*)c->scan_tmp5.p, s);
     exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->wide_words.p, (uint64_t*)c->wide_wbase.p,
@@ -944,8 +951,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     wide_words = std::min<uint64_t>(wide_words, std::max<uint64_t>(out_px / 4, (uint64_t)1 << 20));
     wide_words = std::min<uint64_t>(wide_words, (uint64_t)1 << 36);  // ItemDesc::wide holds words / 16
     if (wide_units > 0 && wide_words > 0) {
-      HIP_TRY(ensure(c->wide_item, wide_units * 4));
-      HIP_TRY(ensure(c->wide_masks, wide_words * 8));
+      HIP_TRY(ensure(c->stream, c->wide_item, wide_units * 4));
+      HIP_TRY(ensure(c->stream, c->wide_masks, wide_words * 8));
       WideAssignArgs wa;
       wa.desc = (ItemDesc*)c->desc.p;
       wa.n_items = I;
@@ -966,12 +973,12 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     const uint64_t nb = capture ? caps->band_entries : c->host_misc[2];
     c->last_band_entries = nb;
     if (nb > 0) {
-      HIP_TRY(ensure(c->band_cnt, (nb + 1) * 4));
-      HIP_TRY(ensure(c->band_off, (nb + 1) * 8));
-      HIP_TRY(ensure(c->scan_tmp3, scan_blocks(nb + 1) * 8));
+      HIP_TRY(ensure(c->stream, c->band_cnt, (nb + 1) * 4));
+      HIP_TRY(ensure(c->stream, c->band_off, (nb + 1) * 8));
+      HIP_TRY(ensure(c->stream, c->scan_tmp3, scan_blocks(nb + 1) * 8));
       zero_words(c->band_cnt.p, nb + 1, s);
       // list of the banded items (band_items), its length after them
-      HIP_TRY(ensure(c->band_items, ((size_t)I + 1) * 4));
+      HIP_TRY(ensure(c->stream, c->band_items, ((size_t)I + 1) * 4));
       uint32_t* n_list = (uint32_t*)c->band_items.p + I;
       zero_words(n_list, 1, s);
       hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I)), dim3(256), 0, s, (ItemDesc*)c->desc.p,
@@ -1002,7 +1009,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
         ne = c->host_misc[3];
       }
       c->last_band_edges = ne;
-      HIP_TRY(ensure(c->band_edges, std::max<uint64_t>(ne, 1) * sizeof(PolyEdge)));
+      HIP_TRY(ensure(c->stream, c->band_edges, std::max<uint64_t>(ne, 1) * sizeof(PolyEdge)));
       bd.band_edges = (PolyEdge*)c->band_edges.p;
       bd.edge_cap = ne;
       if (ne > 0) hipLaunchKernelGGL(k_band_scatter, dim3(bgrid), dim3(256), 0, s, bd);
@@ -1046,12 +1053,12 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     if (n_pairs >= (uint64_t)UINT32_MAX)
       return rz::fail(RZ_CAPACITY_ERROR, "more than 2^32 tile/item pairs in one call");
     const uint64_t cap = std::max<uint64_t>(n_pairs, 1);
-    HIP_TRY(ensure(c->keys_a, cap * 4));
-    HIP_TRY(ensure(c->vals_a, cap * 4));
-    HIP_TRY(ensure(c->keys_b, cap * 4));
-    HIP_TRY(ensure(c->vals_b, cap * 4));
-    HIP_TRY(ensure(c->tile_off, ((size_t)n_tiles + 1) * 4));
-    HIP_TRY(ensure(c->pair_desc, cap * sizeof(ItemDesc)));
+    HIP_TRY(ensure(c->stream, c->keys_a, cap * 4));
+    HIP_TRY(ensure(c->stream, c->vals_a, cap * 4));
+    HIP_TRY(ensure(c->stream, c->keys_b, cap * 4));
+    HIP_TRY(ensure(c->stream, c->vals_b, cap * 4));
+    HIP_TRY(ensure(c->stream, c->tile_off, ((size_t)n_tiles + 1) * 4));
+    HIP_TRY(ensure(c->stream, c->pair_desc, cap * sizeof(ItemDesc)));
     BinArgs ba;
     ba.jobs = d_jobs;
     ba.desc = (const ItemDesc*)c->desc.p;
@@ -1075,9 +1082,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     const uint32_t* sorted_vals = (const uint32_t*)c->vals_a.p;
     if (n_pairs > 0) {
       const uint64_t nbk = sort_blocks(n_pairs);
-      HIP_TRY(ensure(c->sort_hist, nbk * kRadix * 4));
-      HIP_TRY(ensure(c->sort_off, nbk * kRadix * 4));
-      HIP_TRY(ensure(c->sort_tmp, scan_blocks(nbk * kRadix) * 4));
+      HIP_TRY(ensure(c->stream, c->sort_hist, nbk * kRadix * 4));
+      HIP_TRY(ensure(c->stream, c->sort_off, nbk * kRadix * 4));
+      HIP_TRY(ensure(c->stream, c->sort_tmp, scan_blocks(nbk * kRadix) * 4));
       const bool in_b = radix_sort_pairs((uint32_t*)c->keys_a.p, (uint32_t*)c->vals_a.p,
                                          (uint32_t*)c->keys_b.p, (uint32_t*)c->vals_b.p, n_pairs, end_bit,
                                          (uint32_t*)c->sort_hist.p, (uint32_t*)c->sort_off.p,
@@ -1105,15 +1112,15 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     const Tuning bt = tuning_for(size_class(max_tile_w, n_tiles, n_pairs));
     const uint32_t heavy_min = bt.heavy_min ? bt.heavy_min : kHeavyItems;
     if (n_pairs >= heavy_min) {
-      HIP_TRY(ensure(c->heavy_cnt, ((size_t)n_tiles + 1) * 4));
-      HIP_TRY(ensure(c->heavy_base, ((size_t)n_tiles + 1) * 8));
-      HIP_TRY(ensure(c->scan_tmp4, scan_blocks((uint64_t)n_tiles + 1) * 8));
+      HIP_TRY(ensure(c->stream, c->heavy_cnt, ((size_t)n_tiles + 1) * 4));
+      HIP_TRY(ensure(c->stream, c->heavy_base, ((size_t)n_tiles + 1) * 8));
+      HIP_TRY(ensure(c->stream, c->scan_tmp4, scan_blocks((uint64_t)n_tiles + 1) * 8));
       // heavy flags, their ranks and the ordered list of heavy tiles
-      HIP_TRY(ensure(c->heavy_list, ((size_t)n_tiles + 1) * 4 * 3));
+      HIP_TRY(ensure(c->stream, c->heavy_list, ((size_t)n_tiles + 1) * 4 * 3));
       uint32_t* h_flag = (uint32_t*)c->heavy_list.p;
       uint32_t* h_rank = h_flag + (n_tiles + 1);
       uint32_t* h_list = h_rank + (n_tiles + 1);
-      HIP_TRY(ensure(c->scan_tmp7, scan_blocks((uint64_t)n_tiles + 1) * 4));
+      HIP_TRY(ensure(c->stream, c->scan_tmp7, scan_blocks((uint64_t)n_tiles + 1) * 4));
       hipLaunchKernelGGL(k_heavy_count, dim3(blocks_for((int64_t)n_tiles + 1)), dim3(256), 0, s,
                          (const uint32_t*)c->tile_off.p, n_tiles, heavy_min,
                          (uint32_t*)c->heavy_cnt.p, h_flag);
@@ -1125,9 +1132,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->heavy_cnt.p, (uint64_t*)c->heavy_base.p,
                                          (uint64_t)n_tiles + 1, (uint64_t*)c->scan_tmp4.p, s);
       const uint32_t slot_words = 16u * (uint32_t)(max_tile_w / 64);
-      HIP_TRY(ensure(c->heavy_pairs, (size_t)n_pairs * 4));
-      HIP_TRY(ensure(c->cover_list, (size_t)n_pairs * 4 + 4));
-      HIP_TRY(ensure(c->masks, (size_t)n_pairs * 4 * slot_words * 8));
+      HIP_TRY(ensure(c->stream, c->heavy_pairs, (size_t)n_pairs * 4));
+      HIP_TRY(ensure(c->stream, c->cover_list, (size_t)n_pairs * 4 + 4));
+      HIP_TRY(ensure(c->stream, c->masks, (size_t)n_pairs * 4 * slot_words * 8));
       uint32_t* cover_count = (uint32_t*)c->cover_list.p + n_pairs;
       zero_words(cover_count, 1, s);
       hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,

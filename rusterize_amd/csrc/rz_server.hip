@@ -3,16 +3,17 @@
 //
 // Included by rz_runtime.hip (it drives run_batch on the server's contexts).
 //
-// A worker thread owns n_streams contexts (one HIP stream each, default
-// priority).  Submitted requests wait in a FIFO queue.  Whenever a context is
-// idle, the worker admits queued requests in arrival order while their output
-// rasters and estimated workspace fit the memory budget (a request larger
-// than the whole budget is failed), up to max_batch of them, allocates their
-// outputs and runs them as one batch on that context.  Batches on different
-// contexts overlap on the device, so requests arriving while others run are
-// admitted as soon as a context and memory free up.  Completion is observed
-// through one event per batch.  Outputs stay on the device until the caller
-// releases them, which returns their memory to the budget.
+// n_streams worker threads each own one context (one HIP stream, default
+// priority).  Submitted requests wait in a FIFO queue.  An idle worker admits
+// queued requests in arrival order while their output rasters and estimated
+// workspace fit the memory budget (a request larger than the whole budget is
+// failed), up to max_batch of them, allocates their outputs, runs them as one
+// batch on its context and waits for it.  Batches of different workers
+// overlap on the device, and one batch's host round trips (the eager
+// pipeline reads a few totals back) do not hold up the others, so requests
+// arriving while others run are admitted as soon as a worker and memory free
+// up.  Outputs stay on the device until the caller releases them, which
+// returns their memory to the budget.
 #pragma once
 
 #include <chrono>
@@ -39,12 +40,6 @@ struct Request {
   Clock::time_point t_submit, t_start, t_done;
 };
 
-struct InFlight {
-  int slot = 0;
-  hipEvent_t done = nullptr;
-  std::vector<uint64_t> tickets;
-};
-
 // Device memory a request may need besides its output: edge records and
 // coordinates per vertex, descriptors and bins per item, and the wide
 // pre-pass masks (capped at 2 bytes per output pixel).
@@ -61,19 +56,27 @@ struct rz_server {
   uint64_t budget = 0;
   int max_batch = 64;
   std::vector<rz_context*> ctx;
-  std::vector<bool> busy;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<uint64_t> queue;
   std::unordered_map<uint64_t, Request> reqs;
-  std::vector<InFlight> inflight;
   std::unordered_multimap<size_t, void*> free_bufs;  // released outputs, by size
   uint64_t next_ticket = 1;
   uint64_t used = 0;       // outputs and workspace of admitted requests
   uint64_t cached = 0;     // bytes in free_bufs
-  uint64_t n_batches = 0, n_done = 0, peak_used = 0;
+  uint64_t n_batches = 0, n_done = 0, peak_used = 0, running = 0;
   bool stop = false;
-  std::thread worker;
+  std::vector<std::thread> workers;
+
+  // Output buffers come in size classes (four per power of two) so released
+  // outputs are reused by later requests of similar size: fresh allocations,
+  // and the hipFree of cached ones (which synchronises the device), stay rare.
+  static size_t size_class_bytes(size_t bytes) {
+    size_t c = 1u << 20;
+    while (c < bytes) c <<= 1;  // next power of two >= 1 MiB
+    const size_t q = c / 8;     // quarters of the octave [c / 2, c)
+    return c <= (1u << 20) ? c : ((bytes + q - 1) / q) * q;
+  }
 
   void* take_buffer(size_t bytes) {
     auto it = free_bufs.find(bytes);
@@ -95,45 +98,46 @@ struct rz_server {
     return p;
   }
 
-  // Completes the batches whose events have fired.
-  void reap() {
-    for (size_t k = 0; k < inflight.size();) {
-      InFlight& f = inflight[k];
-      if (hipEventQuery(f.done) != hipSuccess) {
-        ++k;
+  // Admits queued requests in arrival order while they fit (lock held).
+  std::vector<uint64_t> admit() {
+    std::vector<uint64_t> batch;
+    while (!queue.empty() && (int)batch.size() < max_batch) {
+      Request& r = reqs[queue.front()];
+      const uint64_t need = r.out_bytes + r.ws_bytes;
+      if (need > budget) {  // can never fit
+        r.state = 3;
+        r.status = RZ_CAPACITY_ERROR;
+        r.error = "request exceeds the server's memory budget";
+        r.t_done = Clock::now();
+        queue.pop_front();
+        cv_done.notify_all();
         continue;
       }
-      const auto now = Clock::now();
-      rz_status st = check_status_word(ctx[f.slot]);
-      for (uint64_t t : f.tickets) {
-        Request& r = reqs[t];
-        r.state = st == RZ_OK ? 2 : 3;
-        r.status = st;
-        if (st != RZ_OK) r.error = rz_last_error();
-        r.t_done = now;
-        used -= r.ws_bytes;  // the workspace estimate is released; the output stays
-        r.ws_bytes = 0;
-        ++n_done;
-      }
-      busy[f.slot] = false;
-      (void)hipEventDestroy(f.done);
-      inflight.erase(inflight.begin() + (long)k);
-      cv_done.notify_all();
+      if (used + need > budget) break;
+      void* out = take_buffer(r.out_bytes);
+      if (!out) break;  // device memory exhausted: wait for releases
+      r.job.out_dev = out;
+      used += need;
+      peak_used = std::max(peak_used, used);
+      r.state = 1;
+      r.t_start = Clock::now();
+      batch.push_back(r.ticket);
+      queue.pop_front();
     }
+    return batch;
   }
 
-  void loop() {
+  // One worker per context: admit a batch, run it, wait for it on this
+  // thread (the host round trips of one batch do not hold up the others),
+  // complete its requests.
+  void work(int slot) {
     (void)hipSetDevice(device);
+    rz_context* c = ctx[slot];
+    hipEvent_t ev = nullptr;
+    const bool have_ev = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
-      reap();
-      int slot = -1;
-      for (size_t k = 0; k < busy.size(); ++k)
-        if (!busy[k]) {
-          slot = (int)k;
-          break;
-        }
-      if (stop) {  // shutting down: fail what is still queued, drain what runs
+      if (stop) {  // shutting down: fail what is still queued (running batches finish)
         for (uint64_t t : queue) {
           Request& r = reqs[t];
           r.state = 3;
@@ -143,72 +147,47 @@ struct rz_server {
         }
         queue.clear();
         cv_done.notify_all();
-        if (inflight.empty()) break;
+        break;
       }
-      if (slot < 0 || queue.empty()) {
-        // nothing to launch: wait for a submission or poll the running batches
-        if (inflight.empty()) cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-        else cv_work.wait_for(lk, std::chrono::microseconds(50));
+      if (queue.empty()) {
+        cv_work.wait(lk, [&] { return stop || !queue.empty(); });
         continue;
       }
-      // admit queued requests in arrival order while they fit
-      std::vector<uint64_t> batch;
-      while (!queue.empty() && (int)batch.size() < max_batch) {
-        Request& r = reqs[queue.front()];
-        const uint64_t need = r.out_bytes + r.ws_bytes;
-        if (need > budget) {  // can never fit
-          r.state = 3;
-          r.status = RZ_CAPACITY_ERROR;
-          r.error = "request exceeds the server's memory budget";
-          r.t_done = Clock::now();
-          queue.pop_front();
-          cv_done.notify_all();
-          continue;
-        }
-        if (used + need > budget) break;
-        void* out = take_buffer(r.out_bytes);
-        if (!out) break;  // device memory exhausted: wait for releases
-        r.job.out_dev = out;
-        used += need;
-        peak_used = std::max(peak_used, used);
-        r.state = 1;
-        r.t_start = Clock::now();
-        batch.push_back(r.ticket);
-        queue.pop_front();
-      }
-      if (batch.empty()) {
-        cv_work.wait_for(lk, std::chrono::microseconds(50));
+      std::vector<uint64_t> batch = admit();
+      if (batch.empty()) {  // the budget is full: wait for releases
+        cv_work.wait_for(lk, std::chrono::microseconds(200));
         continue;
       }
       std::vector<rz_job> jobs;
       jobs.reserve(batch.size());
       for (uint64_t t : batch) jobs.push_back(reqs[t].job);
-      busy[slot] = true;
-      rz_context* c = ctx[slot];
+      ++running;
       lk.unlock();
       rz_status st = run_batch(c, jobs.data(), (int64_t)jobs.size());
-      hipEvent_t ev = nullptr;
-      if (st == RZ_OK && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
-          hipEventRecord(ev, c->stream) == hipSuccess) {
-        lk.lock();
-        inflight.push_back(InFlight{slot, ev, batch});
-        ++n_batches;
-      } else {
-        const std::string msg = st ? rz_last_error() : "event creation failed";
-        lk.lock();
-        for (uint64_t t : batch) {
-          Request& r = reqs[t];
-          r.state = 3;
-          r.status = st ? st : RZ_DEVICE_ERROR;
-          r.error = msg;
-          r.t_done = Clock::now();
-          used -= r.ws_bytes;
-          r.ws_bytes = 0;
-        }
-        busy[slot] = false;
-        cv_done.notify_all();
+      if (st == RZ_OK && !(have_ev && hipEventRecord(ev, c->stream) == hipSuccess &&
+                           hipEventSynchronize(ev) == hipSuccess))
+        st = rz::fail(RZ_DEVICE_ERROR, "waiting for a batch failed");
+      if (st == RZ_OK) st = check_status_word(c);
+      const std::string msg = st ? rz_last_error() : std::string();
+      const auto now = Clock::now();
+      lk.lock();
+      for (uint64_t t : batch) {
+        Request& r = reqs[t];
+        r.state = st == RZ_OK ? 2 : 3;
+        r.status = st;
+        r.error = msg;
+        r.t_done = now;
+        used -= r.ws_bytes;  // the workspace estimate is released; the output stays
+        r.ws_bytes = 0;
+        ++n_done;
       }
+      --running;
+      ++n_batches;
+      cv_done.notify_all();
+      cv_work.notify_all();  // workspace returned to the budget
     }
+    lk.unlock();
+    if (have_ev) (void)hipEventDestroy(ev);
   }
 };
 
@@ -230,9 +209,8 @@ extern "C" rz_status rz_server_create(int device, uint64_t memory_budget, int n_
       return st;
     }
     s->ctx.push_back(c);
-    s->busy.push_back(false);
   }
-  s->worker = std::thread([s] { s->loop(); });
+  for (int k = 0; k < n_streams; ++k) s->workers.emplace_back([s, k] { s->work(k); });
   *out = s;
   return RZ_OK;
 }
@@ -244,7 +222,8 @@ extern "C" rz_status rz_server_destroy(rz_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  if (s->worker.joinable()) s->worker.join();
+  for (std::thread& w : s->workers)
+    if (w.joinable()) w.join();
   (void)hipSetDevice(s->device);
   for (auto& kv : s->reqs)
     if (kv.second.job.out_dev) (void)hipFree(kv.second.job.out_dev);
@@ -262,7 +241,8 @@ extern "C" rz_status rz_server_submit(rz_server* s, const rz_job* job, uint64_t*
   r.job = *job;
   r.job.out_dev = nullptr;
   const uint32_t nl = job->layer_dev ? job->n_layers : 1u;
-  r.out_bytes = (size_t)nl * job->info.nrows * job->info.ncols * rz_dtype_size(job->dtype);
+  r.out_bytes = rz_server::size_class_bytes((size_t)nl * job->info.nrows * job->info.ncols *
+                                            rz_dtype_size(job->dtype));
   r.ws_bytes = workspace_estimate(*job);
   r.t_submit = Clock::now();
   {
@@ -314,7 +294,7 @@ extern "C" rz_status rz_server_get_stats(rz_server* s, rz_server_stats* out) {
   if (!s || !out) return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_server_get_stats");
   std::lock_guard<std::mutex> lk(s->mu);
   out->queued = s->queue.size();
-  out->in_flight_batches = s->inflight.size();
+  out->in_flight_batches = s->running;
   out->batches = s->n_batches;
   out->completed = s->n_done;
   out->memory_used = s->used;

@@ -148,6 +148,11 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
   constexpr int RSTEP = 64 / LPR;                   // rows per store instruction
   constexpr int RPL = WAVE_ROWS / RSTEP;            // rows per lane
   constexpr int NPIX = RPL * PPC;                   // pixels per lane
+  // 1- and 2-byte outputs of the polygon variants keep their pixels packed,
+  // four (two) per 32-bit word, and fold them lane-wise (fold_word)
+  constexpr bool PACKED = !LINES && sizeof(T) <= 2;
+  constexpr int NW = PACKED ? NPIX * (int)sizeof(T) / 4 : 1;  // words per lane
+  constexpr int PPW = 4 / (int)sizeof(T);                     // pixels per word (packed)
   constexpr int CNT_WORDS = LINES ? WAVE_ROWS * TW / 2 : 1;  // u16 counters, 2 per word
   static_assert(NPIX <= 64, "dedup bits assume <= 64 pixels per lane");
   __shared__ PolyEdge s_edge[4][64];
@@ -270,15 +275,23 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
     const bool hb_ok = a.heavy_base && !J.scan && pe - pb >= a.heavy_min;
     if (hb_ok) hbase = heavy_c[t];
 
-    T acc[NPIX];  // acc[k * PPC + j]: row srow + k*RSTEP, column ccol + j
+    T acc[PACKED ? 1 : NPIX];  // acc[k * PPC + j]: row srow + k*RSTEP, column ccol + j
+    uint32_t accw[NW];         // packed: pixel k * PPC + j in word (k * PPC + j) / PPW
+    const uint32_t bgw = PACKED ? lane_splat<T>(J.bg_bits) : 0u;
+    if constexpr (PACKED) {
 #pragma unroll
-    for (int j = 0; j < NPIX; ++j) acc[j] = bg;
+      for (int i = 0; i < NW; ++i) accw[i] = bgw;
+    } else {
+#pragma unroll
+      for (int j = 0; j < NPIX; ++j) acc[j] = bg;
+    }
 
     auto process_item = [&](const ItemDesc d, uint32_t pi) {
       // wave-uniform skip: the item box misses this strip
       if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) return;
       const int32_t kind = (int32_t)(d.kind_layer & 3u);
       const T v = from_bits<T>(d.value);
+      const uint64_t vbits = d.value;
       const int64_t vb = d.vbeg, ve = d.vbeg + d.nv;
       if (kind == 0) {
         uint64_t cov = 0;
@@ -311,9 +324,18 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
           const uint32_t bits = (uint32_t)(row_word >> (ccol & 63)) & ((1u << PPC) - 1u);
           if (__any(bits != 0u)) {  // wave-uniform skip of uncovered row groups
             burns += (unsigned long long)__popc(bits);
+            if constexpr (PACKED) {
+              const uint32_t vw = lane_splat<T>(vbits);
 #pragma unroll
-            for (int j = 0; j < PPC; ++j)
-              acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
+              for (int i = 0; i < PPC / PPW; ++i)
+                accw[k * (PPC / PPW) + i] = fold_word<T, FUN>(
+                    accw[k * (PPC / PPW) + i], vw, bgw,
+                    lane_mask<(int)sizeof(T)>((bits >> (i * PPW)) & ((1u << PPW) - 1u)));
+            } else {
+#pragma unroll
+              for (int j = 0; j < PPC; ++j)
+                acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
+            }
           }
         }
         mbuf ^= 1;  // the next polygon writes the other buffer
@@ -434,12 +456,21 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
         T* orow = (T*)J.out + ((uint64_t)layer * (uint64_t)nrows + (uint64_t)row) * (uint64_t)ncols;
         if (c + PPC <= ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
           u32x4 vec;
-          __builtin_memcpy(&vec, &acc[k * PPC], 16);
+          if constexpr (PACKED) vec = u32x4{accw[k * 4], accw[k * 4 + 1], accw[k * 4 + 2], accw[k * 4 + 3]};
+          else __builtin_memcpy(&vec, &acc[k * PPC], 16);
           __builtin_nontemporal_store(vec, (u32x4*)(orow + c));
         } else {
 #pragma unroll
           for (int j = 0; j < PPC; ++j)
-            if (c + j < ncols) orow[c + j] = acc[k * PPC + j];
+            if (c + j < ncols) {
+              if constexpr (PACKED) {
+                T x;
+                __builtin_memcpy(&x, reinterpret_cast<const char*>(accw) + (k * PPC + j) * sizeof(T), sizeof(T));
+                orow[c + j] = x;
+              } else {
+                orow[c + j] = acc[k * PPC + j];
+              }
+            }
         }
       }
     }
@@ -568,6 +599,9 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
   constexpr int RSTEP = 64 / LPR;
   constexpr int RPL = WAVE_ROWS / RSTEP;
   constexpr int NPIX = RPL * PPC;
+  constexpr bool PACKED = sizeof(T) <= 2;  // pixels packed in 32-bit words (fold_word)
+  constexpr int NW = PACKED ? NPIX * (int)sizeof(T) / 4 : 1;
+  constexpr int PPW = 4 / (int)sizeof(T);
   constexpr int SW = 16 * NWORD;          // words of one item's strip mask
   constexpr uint32_t kBlk = 256 / SW;     // items per block (16 or 4)
   __shared__ uint64_t s_mk[4][256];
@@ -608,9 +642,16 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
     const uint64_t* sbase = a.masks + (a.heavy_base[t] * 4u + (uint64_t)strip * cnt) * a.slot_words;
     const ItemDesc* dbase = a.pair_desc + pb;
 
-    T acc[NPIX];
+    T acc[PACKED ? 1 : NPIX];
+    uint32_t accw[NW];
+    const uint32_t bgw = PACKED ? lane_splat<T>(J.bg_bits) : 0u;
+    if constexpr (PACKED) {
 #pragma unroll
-    for (int j = 0; j < NPIX; ++j) acc[j] = bg;
+      for (int i = 0; i < NW; ++i) accw[i] = bgw;
+    } else {
+#pragma unroll
+      for (int j = 0; j < NPIX; ++j) acc[j] = bg;
+    }
 
     // block j0 into registers: masks (lane L: words 4L .. 4L + 3 of the
     // block) and descriptors (lane L: bytes 16 (L % 4) .. of descriptor L / 4)
@@ -642,6 +683,7 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
         const ItemDesc& d = ds[g];
         if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) continue;
         const T v = from_bits<T>(d.value);
+        const uint64_t vbits = d.value;
         const uint64_t* m = mk + g * SW;
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
@@ -649,9 +691,18 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
           const uint32_t bits = (uint32_t)(row_word >> (ccol & 63)) & ((1u << PPC) - 1u);
           if (__any(bits != 0u)) {
             burns += (unsigned long long)__popc(bits);
+            if constexpr (PACKED) {
+              const uint32_t vw = lane_splat<T>(vbits);
 #pragma unroll
-            for (int j = 0; j < PPC; ++j)
-              acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
+              for (int i = 0; i < PPC / PPW; ++i)
+                accw[k * (PPC / PPW) + i] = fold_word<T, FUN>(
+                    accw[k * (PPC / PPW) + i], vw, bgw,
+                    lane_mask<(int)sizeof(T)>((bits >> (i * PPW)) & ((1u << PPW) - 1u)));
+            } else {
+#pragma unroll
+              for (int j = 0; j < PPC; ++j)
+                acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
+            }
           }
         }
       }
@@ -666,12 +717,21 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
         T* orow = (T*)J.out + ((uint64_t)layer * (uint64_t)nrows + (uint64_t)row) * (uint64_t)ncols;
         if (c + PPC <= ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
           u32x4 vec;
-          __builtin_memcpy(&vec, &acc[k * PPC], 16);
+          if constexpr (PACKED) vec = u32x4{accw[k * 4], accw[k * 4 + 1], accw[k * 4 + 2], accw[k * 4 + 3]};
+          else __builtin_memcpy(&vec, &acc[k * PPC], 16);
           __builtin_nontemporal_store(vec, (u32x4*)(orow + c));
         } else {
 #pragma unroll
           for (int j = 0; j < PPC; ++j)
-            if (c + j < ncols) orow[c + j] = acc[k * PPC + j];
+            if (c + j < ncols) {
+              if constexpr (PACKED) {
+                T x;
+                __builtin_memcpy(&x, reinterpret_cast<const char*>(accw) + (k * PPC + j) * sizeof(T), sizeof(T));
+                orow[c + j] = x;
+              } else {
+                orow[c + j] = acc[k * PPC + j];
+              }
+            }
         }
       }
     }

@@ -135,7 +135,7 @@ __device__ inline int32_t wave_max16(int32_t v) {
 // choice for 8-byte outputs and the line / point variants.
 template <typename T, bool LINES>
 struct TileOccupancy {
-  static constexpr int value = LINES ? 1 : (sizeof(T) == 4 ? 7 : (sizeof(T) == 8 ? 5 : 6));
+  static constexpr int value = LINES ? 4 : (sizeof(T) == 4 ? 7 : (sizeof(T) == 8 ? 5 : 6));
 };
 
 template <typename T, int FUN, bool LINES>
@@ -343,10 +343,12 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
       }
       if (!LINES) return;  // the host only picks this variant without lines / points
       // ---- lines and points: multiplicities in wave-private u16 counters
+      bool hit = false;  // this lane counted a cell of the strip
       auto emit = [&](int64_t cx, int64_t cy) {
         if (cx < C0 || cx >= tile_c1 || cy < Rw || cy >= strip_r1) return;
         const uint32_t idx = (uint32_t)(cy - Rw) * TW + (uint32_t)(cx - C0);
         atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
+        hit = true;
       };
       const bool dedup = kind == 1 && dedup_job;
       uint64_t seen = 0u;  // PixelCache bits of this lane's pixels
@@ -380,6 +382,9 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
           // end cell of the last kept edge of an open line (burners.rs:86-89)
           if ((uint64_t)(s + 1) == d.last && !(f & VF_CLOSED)) emit(ix1, iy1);
         }
+        // nothing counted in this strip: the counters are still zero
+        if (!__any(hit)) continue;
+        hit = false;
         wave_sync();
         // apply the multiplicities of this lane's pixels, then clear them
 #pragma unroll

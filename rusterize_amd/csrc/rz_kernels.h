@@ -46,6 +46,13 @@ struct __align__(32) PolyEdge {
 
 enum : uint8_t { VF_HAS_NEXT = 1, VF_CLOSED = 2 };
 
+// Per-vertex record of a resident geometry set: bits 0-31 the local item,
+// 32-33 its kind, then whether the vertex has a next one in its ring, whether
+// the ring is closed (first coordinate == last), and whether the vertex
+// starts its ring.
+constexpr int VI_KIND_SHIFT = 32;
+constexpr uint64_t VI_HAS_NEXT = 1ull << 34, VI_CLOSED = 1ull << 35, VI_RING_START = 1ull << 36;
+
 // Per-item pixel bounding box: rows [r0, r1), cols [c0, c1).
 struct __align__(16) ItemBox {
   int32_t r0, r1, c0, c1;
@@ -58,7 +65,7 @@ struct __align__(16) ItemBox {
 // job's resident geometry set, indexed locally.
 struct JobDev {
   const double* xy;
-  const int32_t* vert_ring;
+  const uint64_t* vert_info;  // per local vertex: item, kind and ring flags (VI_*)
   const int64_t* ring_voff;
   const int32_t* ring_item;
   const int32_t* item_kind;

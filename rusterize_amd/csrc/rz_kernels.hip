@@ -70,10 +70,22 @@ __global__ void k_ring_item(const int64_t* item_ring_off, const int64_t* ring_vo
   if (i == n_items - 1) item_vbeg[n_items] = ring_voff[r1];
 }
 
-__global__ void k_vert_ring(const int64_t* ring_voff, int64_t n_rings, int32_t* vert_ring) {
+// The per-vertex record every rasterise call reads (built once per resident
+// geometry set, so k_prep needs one load per vertex instead of the chain
+// vertex -> ring -> item -> kind): see the VI_* layout.
+__global__ void k_vert_info(const int64_t* ring_voff, const int32_t* ring_item, const int32_t* item_kind,
+                            const double* xy, int64_t n_rings, uint64_t* vert_info) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rings) return;
-  for (int64_t v = ring_voff[r]; v < ring_voff[r + 1]; ++v) vert_ring[v] = (int32_t)r;
+  const int64_t rbeg = ring_voff[r], rend = ring_voff[r + 1];
+  if (rend <= rbeg) return;
+  const int32_t item = ring_item[r];
+  // geo-types LineString::is_closed: first coordinate == last coordinate
+  const bool closed = xy[2 * rbeg] == xy[2 * (rend - 1)] && xy[2 * rbeg + 1] == xy[2 * (rend - 1) + 1];
+  const uint64_t base = (uint64_t)(uint32_t)item | ((uint64_t)(item_kind[item] & 3) << VI_KIND_SHIFT) |
+                        (closed ? VI_CLOSED : 0ull);
+  for (int64_t v = rbeg; v < rend; ++v)
+    vert_info[v] = base | (v + 1 < rend ? VI_HAS_NEXT : 0ull) | (v == rbeg ? VI_RING_START : 0ull);
 }
 
 __global__ void k_init_items(ItemBox* box, unsigned long long* last, int64_t n_items) {
@@ -145,12 +157,11 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   if (active) {
     const RZ_CONST JobDev& J = as_const(a.jobs)[find_job_wave(a.job_vend, a.n_jobs, v)];
     const int64_t lv = v - J.v_base;
-    const int32_t ring = J.vert_ring[lv];
-    const int32_t litem = J.ring_item[ring];
-    item = J.i_base + litem;
-    const int32_t kind = J.item_kind[litem];
-    const int64_t rbeg = J.ring_voff[ring], rend = J.ring_voff[ring + 1];
-    const bool has_next = lv + 1 < rend;
+    const uint64_t vi = J.vert_info[lv];
+    item = J.i_base + (int64_t)(uint32_t)vi;
+    const int32_t kind = (int32_t)((vi >> VI_KIND_SHIFT) & 3u);
+    const bool has_next = (vi & VI_HAS_NEXT) != 0;
+    const bool closed = (vi & VI_CLOSED) != 0;
     const double rows = (double)J.nrows, cols = (double)J.ncols;
     // world -> pixel; a NaN coordinate stays that NaN (x86 semantics, x86_nan)
     auto px_x = [&](double X) { return isnan(X) ? X : (X - J.xmin) / J.xres; };
@@ -161,8 +172,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       PolyEdge e{0.0, 0.0, 0.0, 0, 0};
       // a ring left open by the caller (its last coordinate is not its
       // first) can leave rows with an odd crossing count
-      if (lv == rbeg && !(J.xy[2 * rbeg] == J.xy[2 * (rend - 1)] && J.xy[2 * rbeg + 1] == J.xy[2 * (rend - 1) + 1]))
-        odd = true;
+      if ((vi & VI_RING_START) && !closed) odd = true;
       if (has_next) {
         const double x1 = px_x(J.xy[2 * lv + 2]);
         const double y1 = px_y(J.xy[2 * lv + 3]);
@@ -223,9 +233,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     } else if (kind == 1) {  // line rings: extract_line
       a.px[v] = make_double2(x0, y0);
       uint8_t f = has_next ? VF_HAS_NEXT : 0;
-      // geo-types LineString::is_closed: first coordinate == last coordinate
-      if (J.xy[2 * rbeg] == J.xy[2 * (rend - 1)] && J.xy[2 * rbeg + 1] == J.xy[2 * (rend - 1) + 1])
-        f |= VF_CLOSED;
+      if (closed) f |= VF_CLOSED;  // geo-types LineString::is_closed
       a.vflags[v] = f;
       if (has_next) {
         const double x1 = px_x(J.xy[2 * lv + 2]);

@@ -151,7 +151,7 @@ struct rz_device_geometry {
   double* xy = nullptr;
   int32_t* item_geom = nullptr;
   int32_t* ring_item = nullptr;
-  int32_t* vert_ring = nullptr;
+  uint64_t* vert_info = nullptr;
   int64_t* item_vbeg = nullptr;
   bool has_poly = false, has_line = false, has_point = false;
   int64_t max_poly_nv = 0;  // most vertices of one polygon item
@@ -311,7 +311,7 @@ extern "C" rz_status rz_device_geometry_free(rz_device_geometry* dg) {
   (void)hipSetDevice(dg->ctx->device);
   (void)hipStreamSynchronize(dg->ctx->stream);
   void* ptrs[] = {dg->geom_item_off, dg->item_kind, dg->item_ring_off, dg->ring_voff, dg->xy,
-                  dg->item_geom,     dg->ring_item, dg->vert_ring,     dg->item_vbeg};
+                  dg->item_geom,     dg->ring_item, dg->vert_info,     dg->item_vbeg};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete dg;
@@ -361,7 +361,7 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
   if (e == hipSuccess) e = hipMalloc((void**)&dg->item_geom, std::max<size_t>(1, 4 * g->n_items));
   if (e == hipSuccess) e = hipMalloc((void**)&dg->ring_item, std::max<size_t>(1, 4 * g->n_rings));
   if (e == hipSuccess)
-    e = hipMalloc((void**)&dg->vert_ring, std::max<size_t>(1, 4 * g->n_vertices));
+    e = hipMalloc((void**)&dg->vert_info, std::max<size_t>(8, 8 * g->n_vertices));
   if (e == hipSuccess)
     e = hipMalloc((void**)&dg->item_vbeg, std::max<size_t>(8, 8 * (g->n_items + 1)));
   if (e != hipSuccess) {
@@ -373,7 +373,8 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
 
 // Derived maps of a resident geometry set whose offsets and coordinates are
 // on the device (or queued on the context's stream): item -> geometry,
-// ring -> item, item -> first vertex, vertex -> ring.  Frees dg on failure.
+// ring -> item, item -> first vertex, vertex -> item / kind / ring flags.
+// Frees dg on failure.
 static rz_status finish_geometry(rz_context* c, rz_device_geometry* dg, rz_device_geometry** out) {
   hipError_t e = hipSuccess;
   if (dg->G)
@@ -385,8 +386,8 @@ static rz_status finish_geometry(rz_context* c, rz_device_geometry* dg, rz_devic
   else
     (void)hipMemsetAsync(dg->item_vbeg, 0, 8, c->stream);
   if (dg->R)
-    hipLaunchKernelGGL(k_vert_ring, dim3(blocks_for(dg->R)), dim3(256), 0, c->stream,
-                       dg->ring_voff, dg->R, dg->vert_ring);
+    hipLaunchKernelGGL(k_vert_info, dim3(blocks_for(dg->R)), dim3(256), 0, c->stream, dg->ring_voff,
+                       dg->ring_item, dg->item_kind, dg->xy, dg->R, dg->vert_info);
   e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipGetLastError();
   if (e != hipSuccess) {
@@ -445,7 +446,7 @@ extern "C" rz_status rz_device_synth_job(rz_context* c, uint64_t seed, int64_t k
   if (e == hipSuccess) e = hipMalloc((void**)&dg->xy, std::max<size_t>(16, 16 * (size_t)dg->V));
   if (e == hipSuccess) e = hipMalloc((void**)&dg->item_geom, std::max<size_t>(4, 4 * (size_t)G));
   if (e == hipSuccess) e = hipMalloc((void**)&dg->ring_item, std::max<size_t>(4, 4 * (size_t)G));
-  if (e == hipSuccess) e = hipMalloc((void**)&dg->vert_ring, std::max<size_t>(4, 4 * (size_t)dg->V));
+  if (e == hipSuccess) e = hipMalloc((void**)&dg->vert_info, std::max<size_t>(8, 8 * (size_t)dg->V));
   if (e == hipSuccess) e = hipMalloc((void**)&dg->item_vbeg, 8 * ((size_t)G + 1));
   if (e != hipSuccess) {
     rz_device_geometry_free(dg);
@@ -729,7 +730,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     JobDev& J = tab[k];
     std::memset(&J, 0, sizeof(J));
     J.xy = dg->xy;
-    J.vert_ring = dg->vert_ring;
+    J.vert_info = dg->vert_info;
     J.ring_voff = dg->ring_voff;
     J.ring_item = dg->ring_item;
     J.item_kind = dg->item_kind;

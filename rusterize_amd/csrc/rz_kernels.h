@@ -315,8 +315,7 @@ struct CoverArgs {
   const uint32_t* pair_tile;    // [n_pairs] tile of every pair (sorted keys)
   const ItemDesc* pair_desc;    // [n_pairs]
   const uint64_t* heavy_base;   // [n_tiles + 1]
-  const uint32_t* heavy_pairs;  // [heavy pairs] pair index
-  const uint32_t* cover_list;   // [*cover_count] slots of heavy polygon pairs
+  const uint32_t* cover_list;   // [*cover_count] pair indices of heavy polygon pairs
   const uint32_t* cover_count;
   uint32_t n_tiles;
   const PolyEdge* edges;

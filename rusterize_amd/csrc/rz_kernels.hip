@@ -113,23 +113,30 @@ __global__ void k_fill_keys(uint32_t* keys, const uint64_t* n_pairs, uint64_t ca
 
 // Segmented (by item) min/max reduction across the wave, then one set of
 // atomics per item segment.  Items own contiguous vertex ranges, so lanes of
-// one item are contiguous.
-__device__ inline void reduce_box(int64_t item, ItemBox b, uint32_t rows, ItemBox* out,
-                                  unsigned long long* rows_out, bool active) {
+// one item are contiguous.  `stat` is item_last's contribution: polygon
+// lanes add the rows their edge crosses (saturating: a ranking statistic
+// only), line lanes take the maximum of (kept segment + 1).  An item has one
+// kind, so every segment combines with one of the two.
+__device__ inline void reduce_box(int64_t item, ItemBox b, uint64_t stat, bool line, ItemBox* out,
+                                  unsigned long long* last_out, bool active) {
   const int lane = threadIdx.x & 63;
   int64_t it = active ? item : -1;
+  // polygon row sums saturate below 2^31: 32-bit shuffles unless a line
+  // vertex (a 64-bit vertex index) is in the wave
+  const bool any_line = __ballot(line) != 0;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     int64_t oit = __shfl_up(it, off, 64);
     int32_t r0 = __shfl_up(b.r0, off, 64), r1 = __shfl_up(b.r1, off, 64);
     int32_t c0 = __shfl_up(b.c0, off, 64), c1 = __shfl_up(b.c1, off, 64);
-    uint32_t orows = (uint32_t)__shfl_up((int)rows, off, 64);
+    const uint64_t ostat = any_line ? (uint64_t)__shfl_up(stat, off, 64)
+                                    : (uint64_t)(uint32_t)__shfl_up((int)(uint32_t)stat, off, 64);
     if (lane >= off && oit == it) {
       b.r0 = min(b.r0, r0);
       b.r1 = max(b.r1, r1);
       b.c0 = min(b.c0, c0);
       b.c1 = max(b.c1, c1);
-      rows = min(rows + orows, 0x7FFFFFFFu);  // saturating: a ranking statistic only
+      stat = line ? max(stat, ostat) : min(stat + ostat, (uint64_t)0x7FFFFFFFu);
     }
   }
   int64_t nit = __shfl_down(it, 1, 64);
@@ -140,7 +147,10 @@ __device__ inline void reduce_box(int64_t item, ItemBox b, uint32_t rows, ItemBo
     atomicMin(&out[item].c0, b.c0);
     atomicMax(&out[item].c1, b.c1);
   }
-  if (active && tail && rows) atomicAdd(&rows_out[item], (unsigned long long)rows);
+  if (active && tail && stat) {
+    if (line) atomicMax(&last_out[item], (unsigned long long)stat);
+    else atomicAdd(&last_out[item], (unsigned long long)stat);
+  }
 }
 
 // One thread per vertex of the batch: world -> pixel (edges.rs:94-97),
@@ -152,12 +162,20 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   const bool active = v < a.n_vertices;
   int64_t item = -1;
   ItemBox box{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-  uint32_t crossed = 0;  // polygon edges: rows [ystart, yend) the edge crosses
-  bool odd = false;      // polygon edges: some row may end with an odd crossing count
-  if (active) {
-    const RZ_CONST JobDev& J = as_const(a.jobs)[find_job_wave(a.job_vend, a.n_jobs, v)];
+  uint64_t stat = 0;  // polygon edges: rows [ystart, yend) crossed; lines: kept segment + 1
+  bool line = false;
+  bool odd = false;   // polygon edges: some row may end with an odd crossing count
+  auto vertex = [&](const RZ_CONST JobDev& J) {
     const int64_t lv = v - J.v_base;
+    // world coordinates of this vertex and of the next one in the job, all
+    // loaded alongside the vertex record (none waits on it)
+    // (the next index is clamped to the job, so no load sits behind a branch;
+    // the last vertex of a job has no successor in its ring)
+    const double* xy = J.xy;
+    const int64_t ln = min(lv + 1, J.nv - 1);
     const uint64_t vi = J.vert_info[lv];
+    const double X0 = xy[2 * lv], Y0 = xy[2 * lv + 1];
+    const double X1 = xy[2 * ln], Y1 = xy[2 * ln + 1];
     item = J.i_base + (int64_t)(uint32_t)vi;
     const int32_t kind = (int32_t)((vi >> VI_KIND_SHIFT) & 3u);
     const bool has_next = (vi & VI_HAS_NEXT) != 0;
@@ -166,16 +184,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     // world -> pixel; a NaN coordinate stays that NaN (x86 semantics, x86_nan)
     auto px_x = [&](double X) { return isnan(X) ? X : (X - J.xmin) / J.xres; };
     auto px_y = [&](double Y) { return isnan(Y) ? Y : (J.ymax - Y) / J.yres; };
-    const double x0 = px_x(J.xy[2 * lv]);
-    const double y0 = px_y(J.xy[2 * lv + 1]);
+    const double x0 = px_x(X0);
+    const double y0 = px_y(Y0);
     if (kind == 0) {  // polygon rings: extract_ring + PolyEdge::new
       PolyEdge e{0.0, 0.0, 0.0, 0, 0};
       // a ring left open by the caller (its last coordinate is not its
       // first) can leave rows with an odd crossing count
       if ((vi & VI_RING_START) && !closed) odd = true;
       if (has_next) {
-        const double x1 = px_x(J.xy[2 * lv + 2]);
-        const double y1 = px_y(J.xy[2 * lv + 3]);
+        const double x1 = px_x(X1);
+        const double y1 = px_y(Y1);
         if (!(fabs(y0 - y1) >= 2.220446049250313e-16)) {
           // a skipped segment whose end points straddle a pixel-centre row
           // (or are not numbers) leaves rows with an odd crossing count
@@ -212,7 +230,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             if (e.ystart < e.yend) {
               box.r0 = e.ystart;
               box.r1 = e.yend;
-              crossed = (uint32_t)(e.yend - e.ystart);
+              stat = (uint64_t)(e.yend - e.ystart);
               // Crossings of this edge lie between its end points up to a few
               // ulps, so their span columns are within one pixel of these.
               const double ax = fabs(xt), bx = fabs(xb);
@@ -231,17 +249,18 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       }
       a.edges[v] = e;
     } else if (kind == 1) {  // line rings: extract_line
+      line = true;
       a.px[v] = make_double2(x0, y0);
       uint8_t f = has_next ? VF_HAS_NEXT : 0;
       if (closed) f |= VF_CLOSED;  // geo-types LineString::is_closed
       a.vflags[v] = f;
       if (has_next) {
-        const double x1 = px_x(J.xy[2 * lv + 2]);
-        const double y1 = px_y(J.xy[2 * lv + 3]);
+        const double x1 = px_x(X1);
+        const double y1 = px_y(Y1);
         const double mnx = fmin(x0, x1), mxx = fmax(x0, x1);
         const double mny = fmin(y0, y1), mxy = fmax(y0, y1);
         if (mnx < cols && mxx >= 0.0 && mny < rows && mxy >= 0.0) {  // edges.rs:130
-          atomicMax(&a.item_last[item], (unsigned long long)(v + 1));
+          stat = (uint64_t)(v + 1);
           // Bresenham visits cells between the floored end points only.
           const int64_t fx0 = sat_isize(floor(mnx)), fx1 = sat_isize(floor(mxx));
           const int64_t fy0 = sat_isize(floor(mny)), fy1 = sat_isize(floor(mxy));
@@ -259,8 +278,18 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         box = ItemBox{cy, cy + 1, cx, cx + 1};
       }
     }
+  };
+  // A wave inside one job (the common case) reads the job record with
+  // scalar loads, all at once; a wave across a job boundary per lane.
+  const int64_t v0 = __builtin_amdgcn_readfirstlane(v);
+  const int j0 = find_job(as_const(a.job_vend), a.n_jobs, v0);
+  const int64_t v_end = min(v0 + 64, a.n_vertices);  // the wave's vertices: [v0, v_end)
+  if (v_end <= as_const(a.job_vend)[j0]) {
+    if (active) vertex(as_const(a.jobs)[j0]);
+  } else if (active) {
+    vertex(as_const(a.jobs)[find_job(a.job_vend, a.n_jobs, v)]);
   }
-  reduce_box(item, box, crossed, a.item_box, a.item_last, active);
+  reduce_box(item, box, stat, line, a.item_box, a.item_last, active);
   if (odd) atomicOr(&a.item_last[item], kItemOddBit);
 }
 

@@ -79,7 +79,7 @@ struct rz_context {
   // workspace
   Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
-      heavy_cnt, heavy_base, heavy_pairs, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
+      heavy_cnt, heavy_base, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
       sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
       wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7;
   // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words
@@ -107,7 +107,7 @@ struct rz_context {
     return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
             &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue,
             &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
-            &heavy_pairs, &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
+            &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
             &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
             &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items, &heavy_list, &scan_tmp7};
   }
@@ -1133,15 +1133,13 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->heavy_cnt.p, (uint64_t*)c->heavy_base.p,
                                          (uint64_t)n_tiles + 1, (uint64_t*)c->scan_tmp4.p, s);
       const uint32_t slot_words = 16u * (uint32_t)(max_tile_w / 64);
-      HIP_TRY(ensure(c->stream, c->heavy_pairs, (size_t)n_pairs * 4));
       HIP_TRY(ensure(c->stream, c->cover_list, (size_t)n_pairs * 4 + 4));
       HIP_TRY(ensure(c->stream, c->masks, (size_t)n_pairs * 4 * slot_words * 8));
       uint32_t* cover_count = (uint32_t*)c->cover_list.p + n_pairs;
       zero_words(cover_count, 1, s);
       hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
                          sorted_keys, (const ItemDesc*)c->pair_desc.p, d_total, n_pairs,
-                         (const uint32_t*)c->tile_off.p, (const uint64_t*)c->heavy_base.p,
-                         (uint32_t*)c->heavy_pairs.p, (uint32_t*)c->cover_list.p, cover_count);
+                         (const uint64_t*)c->heavy_base.p, (uint32_t*)c->cover_list.p, cover_count);
       CoverArgs ca;
       ca.jobs = d_jobs;
       ca.job_tend = d_tend;
@@ -1149,7 +1147,6 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ca.pair_tile = sorted_keys;
       ca.pair_desc = (const ItemDesc*)c->pair_desc.p;
       ca.heavy_base = (const uint64_t*)c->heavy_base.p;
-      ca.heavy_pairs = (const uint32_t*)c->heavy_pairs.p;
       ca.cover_list = (const uint32_t*)c->cover_list.p;
       ca.cover_count = cover_count;
       ca.n_tiles = n_tiles;

@@ -517,23 +517,18 @@ __global__ void k_heavy_list(const uint32_t* heavy_flag, const uint32_t* heavy_r
   if (t < n_tiles && heavy_flag[t]) heavy_list[heavy_rank[t]] = t;
 }
 
-// Slot of every pair of a heavy tile, and the compact list of the slots that
-// hold polygon items other than wide ones (the only ones k_cover writes;
-// k_wide_cover writes the wide ones), appended with one atomic per wave.
+// The compact list of the pairs of heavy tiles that hold polygon items other
+// than wide ones (the only ones k_cover writes; k_wide_cover writes the wide
+// ones), appended with one atomic per wave.
 __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_desc,
-                              const uint64_t* n_pairs, uint64_t cap, const uint32_t* tile_off,
-                              const uint64_t* heavy_base, uint32_t* heavy_pairs, uint32_t* cover_list,
-                              uint32_t* cover_count) {
+                              const uint64_t* n_pairs, uint64_t cap, const uint64_t* heavy_base,
+                              uint32_t* cover_list, uint32_t* cover_count) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  uint64_t slot = 0;
   bool poly = false;
   if (i < min(*n_pairs, cap)) {
     const uint32_t t = pair_tile[i];
-    const uint64_t hb = heavy_base[t];
-    if (heavy_base[t + 1] > hb) {
-      slot = hb + (i - tile_off[t]);
-      heavy_pairs[slot] = (uint32_t)i;
+    if (heavy_base[t + 1] > heavy_base[t]) {
       const ItemDesc& d = pair_desc[i];
       poly = (d.kind_layer & 3u) == 0u && !(d.flags & kItemWide);  // k_wide_cover writes those
     }
@@ -544,7 +539,7 @@ __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_de
   const int leader = __builtin_ctzll(m);
   if (lane == leader) base = atomicAdd(cover_count, (uint32_t)__popcll(m));
   base = (uint32_t)__shfl((int)base, leader, 64);
-  if (poly) cover_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)slot;
+  if (poly) cover_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
 }
 
 template <int NWORD>
@@ -556,10 +551,14 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
   const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
   const RZ_CONST uint64_t* band_c = as_const(a.band_off);
   const uint64_t units = 4ull * *a.cover_count;
-  for (uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave; u < units; u += (uint64_t)gridDim.x * 4u) {
-    const uint64_t hp = a.cover_list[u >> 2];
+  const uint64_t ustride = (uint64_t)gridDim.x * 4u;
+  uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave;
+  // the next unit's pair is loaded one unit ahead
+  uint32_t i_next = u < units ? a.cover_list[u >> 2] : 0u;
+  for (; u < units; u += ustride) {
+    const uint32_t i = i_next;
+    if (u + ustride < units) i_next = a.cover_list[(u + ustride) >> 2];
     const int32_t strip = (int32_t)(u & 3u);
-    const uint32_t i = a.heavy_pairs[hp];
     const uint32_t t = a.pair_tile[i];
     const ItemDesc d = load_desc(as_const(a.pair_desc) + i);
     const RZ_CONST JobDev& J = jobs_c[d.job];
@@ -759,7 +758,7 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
 // (half, w) turns word w of the toggles of its row into coverage (prefix
 // parity across the words from one ballot, right limit from the rightmost
 // crossing) and stores it.
-__global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_wide_cover(WideCoverArgs a) {
   __shared__ PolyEdge s_edge[4][64];
   // rows padded by one word, so the same word of different rows falls in
   // different LDS banks
@@ -775,13 +774,21 @@ __global__ __launch_bounds__(256) void k_wide_cover(WideCoverArgs a) {
   uint64_t(*tog)[kWideWords + 1] = s_tog[wave];
   uint32_t* lp = s_lp[wave];
   int32_t* amax = s_amax[wave];
-  for (uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave; u < units; u += (uint64_t)gridDim.x * 4u) {
-    const uint32_t it = a.unit_item[u];
-    if ((int64_t)it >= a.n_items) continue;  // a unit of an item past the capacities
-    const uint64_t ub = a.unit_base[it];
-    if (u < ub || u >= a.unit_base[it + 1]) continue;
-    const ItemDesc d = load_desc(as_const(a.desc) + it);
-    if (!(d.flags & kItemWide)) continue;
+  if (a.n_items <= 0) return;
+  const uint64_t ustride = (uint64_t)gridDim.x * 4u;
+  uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave;
+  // the next unit's item is loaded one unit ahead, and a unit's range and
+  // descriptor are loaded together (clamped to a valid item) and checked
+  // afterwards, so a unit starts after one round trip instead of three
+  uint32_t it_next = u < units ? a.unit_item[u] : 0u;
+  for (; u < units; u += ustride) {
+    const uint32_t it = it_next;
+    if (u + ustride < units) it_next = a.unit_item[u + ustride];
+    const uint32_t itc = (int64_t)it < a.n_items ? it : (uint32_t)(a.n_items - 1);
+    const uint64_t ub = a.unit_base[itc], ue = a.unit_base[itc + 1];
+    const ItemDesc d = load_desc(as_const(a.desc) + itc);
+    // a unit of an item past the capacities, or a stale one
+    if ((int64_t)it >= a.n_items || u < ub || u >= ue || !(d.flags & kItemWide)) continue;
     const RZ_CONST JobDev& J = jobs_c[d.job];
     const int32_t nw = wide_row_words(d.c0, d.c1);
     const int32_t nwin = (nw + kWideWords - 1) / kWideWords;

@@ -438,8 +438,12 @@ __global__ __launch_bounds__(256) void k_band_count(BandArgs a) {
       const uint32_t L = min((uint32_t)kBandLds, ns - s0);
       for (uint32_t k = lane; k <= L; k += 64) cnt[k] = 0u;
       wave_sync();
+      // each lane's next edge is loaded while the current one is counted
+      int2 ny = make_int2(0, 0);
+      if (lane < d.nv) ny = *reinterpret_cast<const int2*>(&eb[lane].ystart);  // ystart, yend
       for (int32_t e = lane; e < d.nv; e += 64) {
-        const int2 yy = *reinterpret_cast<const int2*>(&eb[e].ystart);  // ystart, yend
+        const int2 yy = ny;
+        if (e + 64 < d.nv) ny = *reinterpret_cast<const int2*>(&eb[e + 64].ystart);
         if (!(yy.x < yy.y)) continue;
         const int32_t b0 = max(max(yy.x / kBandRows, sr0) - sr0, (int32_t)s0);
         const int32_t b1 = min(min((yy.y - 1) / kBandRows, sr1) - sr0, (int32_t)(s0 + L - 1));
@@ -483,10 +487,19 @@ __global__ __launch_bounds__(256) void k_band_scatter(BandArgs a) {
       const uint32_t L = min((uint32_t)kBandLds, ns - s0);
       for (uint32_t k = lane; k < L; k += 64) pos[k] = (uint32_t)(a.band_off[d.band - 1u + s0 + k] - base);
       wave_sync();
+      // the record as two 16-byte vectors (no struct copy through scratch);
+      // each lane's next edge is loaded while the current one is copied
+      uint4 nlo = make_uint4(0u, 0u, 0u, 0u), nhi = nlo;
+      if (lane < d.nv) {
+        nlo = reinterpret_cast<const uint4*>(eb + lane)[0];
+        nhi = reinterpret_cast<const uint4*>(eb + lane)[1];
+      }
       for (int32_t e = lane; e < d.nv; e += 64) {
-        // the record as two 16-byte vectors (no struct copy through scratch)
-        const uint4* src = reinterpret_cast<const uint4*>(eb + e);
-        const uint4 lo = src[0], hi = src[1];  // hi.z = ystart, hi.w = yend
+        const uint4 lo = nlo, hi = nhi;  // hi.z = ystart, hi.w = yend
+        if (e + 64 < d.nv) {
+          nlo = reinterpret_cast<const uint4*>(eb + e + 64)[0];
+          nhi = reinterpret_cast<const uint4*>(eb + e + 64)[1];
+        }
         const int32_t ys = (int32_t)hi.z, ye = (int32_t)hi.w;
         if (!(ys < ye)) continue;
         const int32_t b0 = max(max(ys / kBandRows, sr0) - sr0, (int32_t)s0);

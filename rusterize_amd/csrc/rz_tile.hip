@@ -775,6 +775,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   uint32_t* lp = s_lp[wave];
   int32_t* amax = s_amax[wave];
   if (a.n_items <= 0) return;
+  // the toggle rows start at zero, and each unit's finalize step clears the
+  // words it read (no crossing touches a word past the unit's window)
+  for (int idx = lane; idx < WAVE_ROWS * (kWideWords + 1); idx += 64) (&tog[0][0])[idx] = 0ull;
   const uint64_t ustride = (uint64_t)gridDim.x * 4u;
   uint64_t u = (uint64_t)blockIdx.x * 4u + (uint64_t)wave;
   // the next unit's item is loaded one unit ahead, and a unit's range and
@@ -806,7 +809,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       ebase = a.band_edges + b0;
       n_e = (uint32_t)(band_c[bi + 1u] - b0);
     }
-    for (int idx = lane; idx < WAVE_ROWS * (kWideWords + 1); idx += 64) (&tog[0][0])[idx] = 0ull;
     // the rightmost crossing of a row matters only with an odd count
     const bool odd_rows = (d.flags & kItemOddRows) != 0;
     if (lane < WAVE_ROWS) {
@@ -899,7 +901,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint64_t below = ((1ull << lane) - 1ull) & (half ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull);
     for (int r2 = 0; r2 < WAVE_ROWS; r2 += 2) {
       const int row = r2 + half;
-      const uint64_t t = w < ww ? tog[row][w] : 0ull;
+      uint64_t t = 0ull;
+      if (w < ww) {
+        t = tog[row][w];
+        tog[row][w] = 0ull;
+      }
       const uint64_t odd = __ballot(__popcll(t) & 1);
       const uint32_t carry = lp[row] ^ ((uint32_t)__popcll(odd & below) & 1u);
       uint64_t x = prefix_xor64(t) ^ (carry ? ~0ull : 0ull);
@@ -918,7 +924,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           for (int z = qq + 1; z < nwt; ++z) srow_p[z] = 0ull;
       }
     }
-    wave_sync();  // the state is read before the next unit clears it
+    wave_sync();  // the rows are read and cleared before the next unit's crossings
   }
 }
 

@@ -224,11 +224,12 @@ struct WideCoverArgs {
   uint64_t* wide_masks;
   int cov_mode;               // layout per chunk of edges: 0 the cheaper, 1 row-major, 2 edge-major
   // heavy tiles (null heavy_cnt: none): their items' masks also go to the
-  // heavy slots (see TileArgs), found through the item-ordered pair lists
+  // heavy slots (see TileArgs), at the item's place in the tile's pair list
   const uint32_t* heavy_cnt;  // [n_tiles] items of a heavy tile, 0 otherwise
   const uint64_t* heavy_base;
-  const uint32_t* tile_off;
-  const uint32_t* pair_item;  // [n_pairs] item of every pair, sorted by tile
+  const uint64_t* pair_off;   // [n_items + 1] an item's first pair in emission order
+  uint64_t pair_cap;          // pairs emitted at most (an item past it has none)
+  const uint32_t* wide_j;     // [pair_cap] by emission index: a wide item's place in a heavy tile
   uint64_t* masks;
   uint32_t slot_words;
 };

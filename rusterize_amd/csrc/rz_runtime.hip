@@ -81,7 +81,7 @@ struct rz_context {
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
       heavy_cnt, heavy_base, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
       sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
-      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7;
+      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7, wide_j;
   // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words
   uint64_t* host_misc = nullptr;
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
@@ -109,7 +109,8 @@ struct rz_context {
             &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
             &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
             &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
-            &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items, &heavy_list, &scan_tmp7};
+            &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items, &heavy_list, &scan_tmp7,
+            &wide_j};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -1021,8 +1022,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   // the wide pre-pass (after the bands it reads)
   // (with heavy tiles: their counts, slot bases, pair lists and slots, so the
   // wide items' masks also land in the heavy slots)
-  auto launch_wide = [&](const uint32_t* h_cnt, const uint64_t* h_base, const uint32_t* t_off,
-                         const uint32_t* p_item, uint64_t* masks, uint32_t slot_words) {
+  auto launch_wide = [&](const uint32_t* h_cnt, const uint64_t* h_base, const uint32_t* w_j,
+                         uint64_t* masks, uint32_t slot_words) {
     if (wide_units > 0) {
       WideCoverArgs wc;
       wc.jobs = d_jobs;
@@ -1039,8 +1040,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       wc.cov_mode = cov_mode;
       wc.heavy_cnt = h_cnt;
       wc.heavy_base = h_base;
-      wc.tile_off = t_off;
-      wc.pair_item = p_item;
+      wc.pair_off = (const uint64_t*)c->offsets.p;
+      wc.pair_cap = n_pairs;
+      wc.wide_j = w_j;
       wc.masks = masks;
       wc.slot_words = slot_words;
       const uint32_t wgrid = (uint32_t)std::min<uint64_t>((wide_units + 3) / 4, 4096);
@@ -1134,12 +1136,15 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                                          (uint64_t)n_tiles + 1, (uint64_t*)c->scan_tmp4.p, s);
       const uint32_t slot_words = 16u * (uint32_t)(max_tile_w / 64);
       HIP_TRY(ensure(c->stream, c->cover_list, (size_t)n_pairs * 4 + 4));
+      HIP_TRY(ensure(c->stream, c->wide_j, (size_t)n_pairs * 4));
       HIP_TRY(ensure(c->stream, c->masks, (size_t)n_pairs * 4 * slot_words * 8));
       uint32_t* cover_count = (uint32_t*)c->cover_list.p + n_pairs;
       zero_words(cover_count, 1, s);
-      hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
-                         sorted_keys, (const ItemDesc*)c->pair_desc.p, d_total, n_pairs,
-                         (const uint64_t*)c->heavy_base.p, (uint32_t*)c->cover_list.p, cover_count);
+      hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s, d_jobs,
+                         sorted_keys, sorted_vals, (const ItemDesc*)c->pair_desc.p, d_total, n_pairs,
+                         (const uint32_t*)c->tile_off.p, (const uint64_t*)c->heavy_base.p,
+                         (const uint64_t*)c->offsets.p, (uint32_t*)c->wide_j.p, (uint32_t*)c->cover_list.p,
+                         cover_count);
       CoverArgs ca;
       ca.jobs = d_jobs;
       ca.job_tend = d_tend;
@@ -1157,7 +1162,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ca.masks = (uint64_t*)c->masks.p;
       ca.slot_words = slot_words;
       launch_wide((const uint32_t*)c->heavy_cnt.p, (const uint64_t*)c->heavy_base.p,
-                  (const uint32_t*)c->tile_off.p, sorted_vals, (uint64_t*)c->masks.p, slot_words);
+                  (const uint32_t*)c->wide_j.p, (uint64_t*)c->masks.p, slot_words);
       const uint32_t cgrid = (uint32_t)std::min<uint64_t>(n_pairs, 2048);
       if (any_tw64) hipLaunchKernelGGL(k_cover<1>, dim3(cgrid), dim3(256), 0, s, ca);
       if (any_tw256) hipLaunchKernelGGL(k_cover<4>, dim3(cgrid), dim3(256), 0, s, ca);
@@ -1166,14 +1171,14 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       ta.slot_words = slot_words;
       ta.heavy_min = heavy_min;
     } else {
-      launch_wide(nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+      launch_wide(nullptr, nullptr, nullptr, nullptr, 0);
     }
   } else {
     if (c->profiling) {
       HIP_TRY(hipEventRecord(c->ev[2], s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
     }
-    launch_wide(nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+    launch_wide(nullptr, nullptr, nullptr, nullptr, 0);
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
 

@@ -519,10 +519,14 @@ __global__ void k_heavy_list(const uint32_t* heavy_flag, const uint32_t* heavy_r
 
 // The compact list of the pairs of heavy tiles that hold polygon items other
 // than wide ones (the only ones k_cover writes; k_wide_cover writes the wide
-// ones), appended with one atomic per wave.
-__global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_desc,
-                              const uint64_t* n_pairs, uint64_t cap, const uint64_t* heavy_base,
-                              uint32_t* cover_list, uint32_t* cover_count) {
+// ones), appended with one atomic per wave.  For a wide item's pair it also
+// records the item's place in the tile's pair list, at the pair's emission
+// index (k_bin_emit's order over the item's tile rectangle), where
+// k_wide_cover finds it with one load.
+__global__ void k_heavy_pairs(const JobDev* jobs, const uint32_t* pair_tile, const uint32_t* pair_item,
+                              const ItemDesc* pair_desc, const uint64_t* n_pairs, uint64_t cap,
+                              const uint32_t* tile_off, const uint64_t* heavy_base, const uint64_t* pair_off,
+                              uint32_t* wide_j, uint32_t* cover_list, uint32_t* cover_count) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool poly = false;
@@ -531,6 +535,14 @@ __global__ void k_heavy_pairs(const uint32_t* pair_tile, const ItemDesc* pair_de
     if (heavy_base[t + 1] > heavy_base[t]) {
       const ItemDesc& d = pair_desc[i];
       poly = (d.kind_layer & 3u) == 0u && !(d.flags & kItemWide);  // k_wide_cover writes those
+      int32_t tx0, tx1, ty0, ty1;
+      if ((d.flags & kItemWide) && tile_rect(d.r0, d.r1, d.c0, d.c1, jobs[d.job].tile_w, tx0, tx1, ty0, ty1)) {
+        const JobDev& J = jobs[d.job];
+        const uint32_t tl = (t - J.tile_base) % J.tiles_per_layer;
+        const int32_t ty = (int32_t)(tl / (uint32_t)J.ntx), tx = (int32_t)(tl % (uint32_t)J.ntx);
+        wide_j[pair_off[pair_item[i]] + (uint64_t)((ty - ty0) * (tx1 - tx0 + 1) + (tx - tx0))] =
+            (uint32_t)(i - tile_off[t]);
+      }
     }
   }
   const uint64_t m = __ballot(poly);
@@ -556,7 +568,7 @@ __global__ __launch_bounds__(256) void k_cover(CoverArgs a) {
   // the next unit's pair is loaded one unit ahead
   uint32_t i_next = u < units ? a.cover_list[u >> 2] : 0u;
   for (; u < units; u += ustride) {
-    const uint32_t i = i_next;
+    const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)i_next);  // wave-uniform
     if (u + ustride < units) i_next = a.cover_list[(u + ustride) >> 2];
     const int32_t strip = (int32_t)(u & 3u);
     const uint32_t t = a.pair_tile[i];
@@ -785,7 +797,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   // afterwards, so a unit starts after one round trip instead of three
   uint32_t it_next = u < units ? a.unit_item[u] : 0u;
   for (; u < units; u += ustride) {
-    const uint32_t it = it_next;
+    // (wave-uniform: the unit's header and descriptor go to scalar registers)
+    const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it_next);
     if (u + ustride < units) it_next = a.unit_item[u + ustride];
     const uint32_t itc = (int64_t)it < a.n_items ? it : (uint32_t)(a.n_items - 1);
     const uint64_t ub = a.unit_base[itc], ue = a.unit_base[itc + 1];
@@ -886,14 +899,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       const uint32_t t = J.tile_base + (d.kind_layer >> 2) * J.tiles_per_layer +
                          (uint32_t)(Rw / TILE_H) * (uint32_t)J.ntx + (uint32_t)tx;
       const uint32_t cnt = a.heavy_cnt[t];
-      if (cnt) {
-        uint32_t lo = a.tile_off[t], hi = lo + cnt;  // pair_item[lo .. hi) ascending
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (a.pair_item[mid] < it) lo = mid + 1;
-          else hi = mid;
-        }
-        const uint32_t j = lo - a.tile_off[t];
+      // the pair (it, t) in k_bin_emit's order over the item's tile
+      // rectangle: row Rw / TILE_H - ty0, column tx - tx0 (uniform part first)
+      int32_t tx0, tx1, ty0, ty1;
+      uint32_t j = ~0u;
+      if (cnt && a.pair_off[it + 1] <= a.pair_cap &&
+          tile_rect(d.r0, d.r1, d.c0, d.c1, J.tile_w, tx0, tx1, ty0, ty1)) {
+        const int64_t jrow = (int64_t)a.pair_off[it] + (int64_t)(Rw / TILE_H - ty0) * (tx1 - tx0 + 1) - tx0;
+        j = a.wide_j[jrow + tx];
+      }
+      if (j < cnt) {
         const uint32_t s = (uint32_t)((Rw % TILE_H) / WAVE_ROWS);
         slot = a.masks + (a.heavy_base[t] * 4u + (uint64_t)s * cnt + j) * a.slot_words + (uint32_t)(gw % nwt);
       }

@@ -429,9 +429,9 @@ __global__ __launch_bounds__(256) void k_band_count(BandArgs a) {
   uint32_t* cnt = s_cnt[wave];
   const uint32_t n_list = *a.n_list;
   for (uint32_t li = blockIdx.x * 4u + (uint32_t)wave; li < n_list; li += gridDim.x * 4u) {
-    const uint32_t i = a.items[li];
-    const uint32_t ns = a.band_n[i];
-    const ItemDesc d = a.desc[i];
+    const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.items[li]);  // wave-uniform
+    const uint32_t ns = as_const(a.band_n)[i];
+    const RZ_CONST ItemDesc& d = as_const(a.desc)[i];
     const int32_t sr0 = d.r0 / kBandRows, sr1 = (d.r1 - 1) / kBandRows;
     const PolyEdge* eb = a.edges + d.vbeg;
     for (uint32_t s0 = 0; s0 < ns; s0 += kBandLds) {
@@ -477,12 +477,12 @@ __global__ __launch_bounds__(256) void k_band_scatter(BandArgs a) {
   uint32_t* pos = s_pos[wave];
   const uint32_t n_list = *a.n_list;
   for (uint32_t li = blockIdx.x * 4u + (uint32_t)wave; li < n_list; li += gridDim.x * 4u) {
-    const uint32_t i = a.items[li];
-    const uint32_t ns = a.band_n[i];
-    const ItemDesc d = a.desc[i];
+    const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.items[li]);  // wave-uniform
+    const uint32_t ns = as_const(a.band_n)[i];
+    const RZ_CONST ItemDesc& d = as_const(a.desc)[i];
     const int32_t sr0 = d.r0 / kBandRows, sr1 = (d.r1 - 1) / kBandRows;
     const PolyEdge* eb = a.edges + d.vbeg;
-    const uint64_t base = a.band_off[d.band - 1u];
+    const uint64_t base = as_const(a.band_off)[d.band - 1u];
     for (uint32_t s0 = 0; s0 < ns; s0 += kBandLds) {
       const uint32_t L = min((uint32_t)kBandLds, ns - s0);
       for (uint32_t k = lane; k < L; k += 64) pos[k] = (uint32_t)(a.band_off[d.band - 1u + s0 + k] - base);

@@ -642,7 +642,7 @@ __global__ __launch_bounds__(256) void k_tile_heavy(TileArgs a) {
       u_end = min(u + grab, units);
       if (u >= units) break;
     }
-    const uint32_t t = a.heavy_list[h0 + (u >> 2)];
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.heavy_list[h0 + (u >> 2)]);
     const int32_t strip = (int32_t)(u & 3u);
     ++u;
     const RZ_CONST JobDev& J = jobs_c[find_job(tend_c, a.n_jobs, (int64_t)t)];

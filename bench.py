@@ -183,9 +183,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines", "batch", "serve"])
-    ap.add_argument("--rates", default="50,200,500,1000,2000,3000,4000",
+    ap.add_argument("--rates", default="50,200,500,1000,2000,3000,4000,5000",
                     help="serve workload: arrival rates (requests/s)")
-    ap.add_argument("--requests", type=int, default=200, help="serve workload: requests per rate")
+    ap.add_argument("--requests", type=int, default=1000,
+                    help="serve workload: requests per rate (long enough for a steady state at 5,000/s)")
     ap.add_argument("--streams", type=int, default=2, help="serve workload: server worker streams")
     ap.add_argument("--max-batch", type=int, default=64, help="serve workload: largest batch")
     ap.add_argument("--pool", type=int, default=48, help="serve workload: distinct geometry sets (stream jobs)")

@@ -45,21 +45,34 @@ def _run(cmd: list[str], verbose: bool) -> None:
         print(out, flush=True)
 
 
+# The tile kernel templates are instantiated once per output dtype, each in a
+# translation unit of its own (rz_tile_inst.hip), so they compile in parallel.
+TILE_DTYPES = [("u8", "uint8_t"), ("u16", "uint16_t"), ("u32", "uint32_t"), ("u64", "uint64_t"),
+               ("i8", "int8_t"), ("i16", "int16_t"), ("i32", "int32_t"), ("i64", "int64_t"),
+               ("f32", "float"), ("f64", "double")]
+
+
 def build_library(verbose: bool = False) -> Path:
     """Compiles and links librusterize_hip.so; returns its path."""
     BUILD.mkdir(exist_ok=True)
     host_obj = BUILD / "rz_host.o"
     dev_obj = BUILD / "rz_runtime.o"
+    tile_objs = [BUILD / f"rz_tile_{name}.o" for name, _ in TILE_DTYPES]
     jobs = [
-        [CXX, *HOST_FLAGS, "-c", str(CSRC / "rz_host.cpp"), "-o", str(host_obj)],
         [HIPCC, *HIP_FLAGS, "-c", str(CSRC / "rz_runtime.hip"), "-o", str(dev_obj)],
+        [CXX, *HOST_FLAGS, "-c", str(CSRC / "rz_host.cpp"), "-o", str(host_obj)],
+    ] + [
+        [HIPCC, *HIP_FLAGS, f"-DRZ_TILE_T={ctype}", f"-DRZ_TILE_NAME={name}", "-c",
+         str(CSRC / "rz_tile_inst.hip"), "-o", str(obj)]
+        for (name, ctype), obj in zip(TILE_DTYPES, tile_objs)
     ]
-    with ThreadPoolExecutor(max_workers=2) as ex:
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
         for f in [ex.submit(_run, j, verbose) for j in jobs]:
             f.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), str(host_obj),
-          str(dev_obj)], verbose)
+          str(dev_obj), *map(str, tile_objs)], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -16,8 +16,10 @@
 #include "rz_kernels.h"
 #include "rz_synth_job.h"
 
-// The kernels are compiled in this translation unit so every k_tile<T, FUN>
-// instantiation used by the dispatch below is emitted here.
+// The set-up, binning and pre-pass kernels are compiled in this translation
+// unit; the tile kernel instantiations come from rz_tile_inst.hip (one
+// translation unit per dtype, declared in rz_launch.h).
+#include "rz_launch.h"
 #include "rz_kernels.hip"
 #include "rz_sort.hip"
 #include "rz_tile.hip"
@@ -465,26 +467,6 @@ extern "C" rz_status rz_device_synth_job(rz_context* c, uint64_t seed, int64_t k
 
 namespace {
 
-// Persistent grid for one k_tile instantiation: enough workgroups to fill
-// every CU at the kernel's occupancy (no more than the units need), a
-// multiple of the 8 XCDs.
-template <typename T, int FUN, bool LINES>
-uint32_t tile_grid(uint64_t n_tiles) {
-  static uint32_t cached = 0;
-  if (!cached) {
-    int dev = 0, cus = 256, per_cu = 1;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile<T, FUN, LINES>, 256, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    cached = (uint32_t)(cus * per_cu);
-  }
-  const uint32_t g = (uint32_t)std::min<uint64_t>(cached, std::max<uint64_t>(n_tiles, 1));
-  return std::max<uint32_t>(NUM_XCD, (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
-}
-
 // ---- autotuned launch knobs
 //
 // Per size class (tile width, launch tiles, items per tile) the autotuner
@@ -531,101 +513,33 @@ Tuning tuning_for(uint32_t cls) {
   return it == ts.table.end() ? Tuning{} : it->second;
 }
 
-// Tiles per queue grab: the tuned value, else 4 once every wave has many
-// tiles (measured best of 1, 2, 4 on the 40,000^2 workload), else 1.
-// RUSTERIZE_CHUNK overrides both.
-inline uint32_t queue_chunk(uint64_t n_tiles, uint32_t grid, uint32_t tuned) {
-  if (const char* e = std::getenv("RUSTERIZE_CHUNK")) {
-    const int v = std::atoi(e);
-    if (v >= 1 && v <= 4096) return (uint32_t)v;
-  }
-  if (tuned) return tuned;
-  return n_tiles >= (uint64_t)grid * 16 ? 4u : 1u;
-}
-
-template <typename T, int FUN>
-void launch_tile(TileArgs a, bool lines, hipStream_t s) {
-  const uint64_t tiles = (uint64_t)(a.t_hi - a.t_lo);
-  if (lines) {
-    const uint32_t g = tile_grid<T, FUN, true>(tiles);
-    a.chunk = queue_chunk(tiles, g, a.chunk);
-    hipLaunchKernelGGL((k_tile<T, FUN, true>), dim3(g), dim3(256), 0, s, a);
-  } else {
-    const uint32_t g = tile_grid<T, FUN, false>(tiles);
-    a.chunk = queue_chunk(tiles, g, a.chunk);
-    hipLaunchKernelGGL((k_tile<T, FUN, false>), dim3(g), dim3(256), 0, s, a);
-  }
-}
-
-template <typename T>
-void launch_tile_fun(int fun, const TileArgs& a, bool lines, hipStream_t s) {
-  switch (fun) {
-    case 0: launch_tile<T, 0>(a, lines, s); break;
-    case 1: launch_tile<T, 1>(a, lines, s); break;
-    case 2: launch_tile<T, 2>(a, lines, s); break;
-    case 3: launch_tile<T, 3>(a, lines, s); break;
-    case 4: launch_tile<T, 4>(a, lines, s); break;
-    case 5: launch_tile<T, 5>(a, lines, s); break;
-    default: launch_tile<T, 6>(a, lines, s); break;
-  }
-}
-
-// k_tile_heavy: a persistent grid (the work counter hands out units)
-template <typename T, int FUN>
-void launch_heavy(const TileArgs& a, hipStream_t s) {
-  static uint32_t cached = 0;
-  if (!cached) {
-    int dev = 0, cus = 256, per_cu = 1;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_heavy<T, FUN>, 256, 0) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    cached = (uint32_t)(cus * per_cu);
-  }
-  hipLaunchKernelGGL((k_tile_heavy<T, FUN>), dim3(cached), dim3(256), 0, s, a);
-}
-
-template <typename T>
-void launch_heavy_fun(int fun, const TileArgs& a, hipStream_t s) {
-  switch (fun) {
-    case 0: launch_heavy<T, 0>(a, s); break;
-    case 1: launch_heavy<T, 1>(a, s); break;
-    case 2: launch_heavy<T, 2>(a, s); break;
-    case 3: launch_heavy<T, 3>(a, s); break;
-    case 4: launch_heavy<T, 4>(a, s); break;
-    case 5: launch_heavy<T, 5>(a, s); break;
-    default: launch_heavy<T, 6>(a, s); break;
-  }
-}
-
 void launch_heavy_dispatch(rz_dtype dt, int fun, const TileArgs& a, hipStream_t s) {
   switch (dt) {
-    case RZ_U8: launch_heavy_fun<uint8_t>(fun, a, s); break;
-    case RZ_U16: launch_heavy_fun<uint16_t>(fun, a, s); break;
-    case RZ_U32: launch_heavy_fun<uint32_t>(fun, a, s); break;
-    case RZ_U64: launch_heavy_fun<uint64_t>(fun, a, s); break;
-    case RZ_I8: launch_heavy_fun<int8_t>(fun, a, s); break;
-    case RZ_I16: launch_heavy_fun<int16_t>(fun, a, s); break;
-    case RZ_I32: launch_heavy_fun<int32_t>(fun, a, s); break;
-    case RZ_I64: launch_heavy_fun<int64_t>(fun, a, s); break;
-    case RZ_F32: launch_heavy_fun<float>(fun, a, s); break;
-    default: launch_heavy_fun<double>(fun, a, s); break;
+    case RZ_U8: launch_heavy_u8(fun, a, s); break;
+    case RZ_U16: launch_heavy_u16(fun, a, s); break;
+    case RZ_U32: launch_heavy_u32(fun, a, s); break;
+    case RZ_U64: launch_heavy_u64(fun, a, s); break;
+    case RZ_I8: launch_heavy_i8(fun, a, s); break;
+    case RZ_I16: launch_heavy_i16(fun, a, s); break;
+    case RZ_I32: launch_heavy_i32(fun, a, s); break;
+    case RZ_I64: launch_heavy_i64(fun, a, s); break;
+    case RZ_F32: launch_heavy_f32(fun, a, s); break;
+    default: launch_heavy_f64(fun, a, s); break;
   }
 }
 
 void launch_tile_dispatch(rz_dtype dt, int fun, const TileArgs& a, bool lines, hipStream_t s) {
   switch (dt) {
-    case RZ_U8: launch_tile_fun<uint8_t>(fun, a, lines, s); break;
-    case RZ_U16: launch_tile_fun<uint16_t>(fun, a, lines, s); break;
-    case RZ_U32: launch_tile_fun<uint32_t>(fun, a, lines, s); break;
-    case RZ_U64: launch_tile_fun<uint64_t>(fun, a, lines, s); break;
-    case RZ_I8: launch_tile_fun<int8_t>(fun, a, lines, s); break;
-    case RZ_I16: launch_tile_fun<int16_t>(fun, a, lines, s); break;
-    case RZ_I32: launch_tile_fun<int32_t>(fun, a, lines, s); break;
-    case RZ_I64: launch_tile_fun<int64_t>(fun, a, lines, s); break;
-    case RZ_F32: launch_tile_fun<float>(fun, a, lines, s); break;
-    default: launch_tile_fun<double>(fun, a, lines, s); break;
+    case RZ_U8: launch_tile_u8(fun, a, lines, s); break;
+    case RZ_U16: launch_tile_u16(fun, a, lines, s); break;
+    case RZ_U32: launch_tile_u32(fun, a, lines, s); break;
+    case RZ_U64: launch_tile_u64(fun, a, lines, s); break;
+    case RZ_I8: launch_tile_i8(fun, a, lines, s); break;
+    case RZ_I16: launch_tile_i16(fun, a, lines, s); break;
+    case RZ_I32: launch_tile_i32(fun, a, lines, s); break;
+    case RZ_I64: launch_tile_i64(fun, a, lines, s); break;
+    case RZ_F32: launch_tile_f32(fun, a, lines, s); break;
+    default: launch_tile_f64(fun, a, lines, s); break;
   }
 }
 

@@ -20,6 +20,12 @@ Prints one JSON line.  Timing: W untimed warm-up steps, then exactly K steps
 bracketed by device synchronisation; per-stage times (the tile kernel's
 among them) come from HIP events recorded on the context's own stream over
 the timed region.
+
+Every single-raster run also times the CPU oracle on a bounded sample of the
+same workload (`cpu_baseline`) and compares the GPU's full-size outputs of
+the sampled calls with the oracle's bit for bit (`parity`); the run exits
+non-zero on any differing pixel.  The default run adds the star-polygon and
+line workloads as `legs`, each with its own roofline, baseline and parity.
 """
 from __future__ import annotations
 
@@ -132,28 +138,58 @@ def make_workload(name: str, n_poly: int) -> Workload:
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(wl: Workload):
-    """The CPU restatement (oracle/rz_oracle.c) timed on a bounded sample of
-    the same workload, one thread per output band like the reference's rayon
-    loop over bands."""
+def cpu_sample(wl: Workload):
+    """The bounded CPU sample of a workload: (grid, calls, reps, description).
+    The oracle runs these calls on that grid; the same rows of the GPU's
+    full-size outputs must equal them bit for bit."""
     from oracle import oracle
 
     if wl.name == "polygons":
-        # the top 5,000-row band: the same computation for those rows
+        # the top 5,000-row band of the 40,000-row grid: every polygon edge and
+        # span of those rows is computed exactly as on the full grid (rows are
+        # independent in the scanline engine)
         info = oracle.make_info(wl.W, 5000, 0.0, float(wl.W), float(wl.H - 5000), float(wl.H), 1.0, 1.0)
-        dt, fun, sample = "int32", "last", "rows 0-5000 of 40000 (8 bands x 5000 x 40000 int32), fun=last, all polygons"
-    else:
-        info = oracle.make_info(wl.W, wl.H, 0.0, float(wl.W), 0.0, float(wl.H), 1.0, 1.0)
-        dt, fun = wl.calls[0]
-        sample = f"the whole call ({dt}, fun={fun})"
+        calls = [("int32", f) for f in ("count", "any", "first", "last", "min", "max")] + \
+                [("uint8", "count"), ("float32", "max")]
+        return info, calls, 1, ("rows 0-5000 of 40000 (8 bands x 5000 x 40000), int32 x 6 funs, uint8 count, "
+                                "float32 max, all polygons")
+    info = oracle.make_info(wl.W, wl.H, 0.0, float(wl.W), 0.0, float(wl.H), 1.0, 1.0)
+    if wl.name == "stars":
+        return info, list(wl.calls), 100, "the whole call, 100 repetitions"
+    return info, list(wl.calls), 1, "the whole calls (" + ", ".join(f"{d} {f}" for d, f in wl.calls) + ")"
+
+
+def cpu_baseline(wl: Workload):
+    """The CPU restatement (oracle/rz_oracle.c) timed on a bounded sample of
+    the same workload (cpu_sample), one thread per output band like the
+    reference's rayon loop over bands.  Returns the baseline record and the
+    oracle's rasters of the sample's calls (the parity references)."""
+    from oracle import oracle
+
+    info, calls, reps, sample = cpu_sample(wl)
     threads = min(wl.bands, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    _, burns = oracle.rasterize(wl.geom, info, dt, fun, field=wl.fields[dt], layer_of_geom=wl.layers,
-                                n_layers=wl.bands, background=wl.bg, threads=threads, return_burns=True)
-    secs = time.perf_counter() - t0
-    return {"value": burns / secs, "unit": "pixel-burns/s", "cores": threads, "kind": "port",
-            "sample": f"{sample}; {burns} burns in {secs:.2f} s incl. background fill; "
-                      f"CPU {cpu_model()}, nproc={os.cpu_count()}"}
+    refs, burns, secs = {}, 0, 0.0
+    for dt, fun in calls:
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out, b = oracle.rasterize(wl.geom, info, dt, fun, field=wl.fields[dt], layer_of_geom=wl.layers,
+                                      n_layers=wl.bands, background=wl.bg, threads=threads, return_burns=True)
+            secs += time.perf_counter() - t0
+            burns += b
+        refs[(dt, fun)] = out
+    rec = {"value": burns / secs, "unit": "pixel-burns/s", "cores": threads, "kind": "port",
+           "sample": f"{sample}; {burns} burns in {secs:.2f} s incl. background fill; "
+                     f"CPU {cpu_model()}, nproc={os.cpu_count()}"}
+    return rec, refs, int(info.nrows)
+
+
+def same_bits(gpu: np.ndarray, ref: np.ndarray) -> int:
+    """Pixels that differ (floats compared by bits, any NaN equal to any NaN)."""
+    if gpu.dtype.kind == "f":
+        gn, rn = np.isnan(gpu), np.isnan(ref)
+        u = f"u{gpu.itemsize}"
+        return int(((gn != rn) | (~gn & (gpu.view(u) != ref.view(u)))).sum())
+    return int((gpu != ref).sum())
 
 
 # --------------------------------------------------------------------- runs
@@ -177,53 +213,13 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines", "batch", "serve"])
-    ap.add_argument("--rates", default="50,200,500,1000,2000,3000,4000,5000",
-                    help="serve workload: arrival rates (requests/s)")
-    ap.add_argument("--requests", type=int, default=1000,
-                    help="serve workload: requests per rate (long enough for a steady state at 5,000/s)")
-    ap.add_argument("--streams", type=int, default=2, help="serve workload: server worker streams")
-    ap.add_argument("--max-batch", type=int, default=64, help="serve workload: largest batch")
-    ap.add_argument("--pool", type=int, default=48, help="serve workload: distinct geometry sets (stream jobs)")
-    ap.add_argument("--jobs", type=int, default=256, help="batch workload: jobs of the seeded job stream")
-    ap.add_argument("--autotune", type=int, default=0,
-                    help="batch workload: autotune the launch knobs on this many jobs first")
-    ap.add_argument("--no-graph", action="store_true", help="batch workload: skip the captured-graph leg")
-    ap.add_argument("--budget-gb", type=float, default=150.0,
-                    help="batch workload: device memory the jobs' inputs, outputs and workspace may use")
-    ap.add_argument("--polygons", type=int, default=1_000_000, help="polygon count of the default workload")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", default=None,
-                    help="DTYPE:FUN -- one warm-up and three timed calls of that combination only "
-                         "(for counter collection)")
-    ap.add_argument("--copy-bytes", type=int, default=0,
-                    help="run only a device-to-device copy of this many bytes (the plain copy "
-                         "kernel that calibrates HBM traffic counters)")
-    args = ap.parse_args()
-    if args.copy_bytes:
-        return copy_calibration(args.copy_bytes)
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the rasteriser shards by output rows with no exchange "
-              "(see DESIGN.md); multi-GPU runs are not part of this build", file=sys.stderr)
-        return 2
-    if args.workload == "batch":
-        return batch_bench(args)
-    if args.workload == "serve":
-        return serve_bench(args)
-    steps = args.steps if args.steps is not None else {"polygons": 3, "stars": 200, "lines": 5}[args.workload]
-
+def measure(wl: Workload, steps: int, warmup: int, with_cpu: bool, only: str | None = None) -> dict:
+    """Times `steps` steps of a single-raster workload with inputs resident in
+    HBM, breaks the time down by stage and dtype, counts the exact burns, and
+    (with_cpu) times the oracle on its bounded sample and compares the GPU's
+    outputs of the sample's calls with it bit for bit."""
     L = _lib.lib()
-    t0 = time.perf_counter()
-    wl = make_workload(args.workload, args.polygons)
     g = wl.geom
-    log(f"inputs: {g.n_geoms} geometries, {g.n_items} items, {g.n_rings} rings, {g.n_vertices} vertices "
-        f"({time.perf_counter() - t0:.1f} s)")
-
     ctx = c_void_p()
     check(L.rz_context_create(0, byref(ctx)))
     dg = c_void_p()
@@ -261,8 +257,14 @@ def main() -> int:
         n = L.rz_context_stage_times(ctx, times, 8)
         return {L.rz_stage_name(i).decode(): times[i] / div for i in range(n)}
 
-    if args.only:
-        dt, fun = args.only.split(":")
+    def release():
+        for p in list(d_field.values()) + ([d_lay] if d_lay is not None else []) + [d_out]:
+            L.rz_device_free(ctx, p)
+        L.rz_device_geometry_free(dg)
+        L.rz_context_destroy(ctx)
+
+    if only:
+        dt, fun = only.split(":")
         call(dt, fun)  # warm-up
         check(L.rz_context_synchronize(ctx))
         check(L.rz_context_set_profiling(ctx, 1))
@@ -270,14 +272,14 @@ def main() -> int:
         for _ in range(3):
             call(dt, fun)
         check(L.rz_context_synchronize(ctx))
-        print(json.dumps({"only": args.only, "stages_ms": stage_times(3),
-                          "b_alg_tile": wl.b_alg_tile(np.dtype(dt).itemsize)}), flush=True)
-        return 0
+        res = {"only": only, "stages_ms": stage_times(3), "b_alg_tile": wl.b_alg_tile(np.dtype(dt).itemsize)}
+        release()
+        return res
 
     # HIP events around every stage of every call, recorded on the context's
     # stream and resolved after the timed region (no added synchronisation)
     check(L.rz_context_set_profiling(ctx, 1))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     check(L.rz_context_synchronize(ctx))
     L.rz_context_reset_stage_times(ctx)
@@ -285,7 +287,7 @@ def main() -> int:
     for k in range(steps):
         step()
         if steps <= 10 or (k + 1) % max(1, steps // 5) == 0:
-            log(f"step {k + 1}/{steps} queued")
+            log(f"{wl.name}: step {k + 1}/{steps} queued")
     check(L.rz_context_synchronize(ctx))
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / steps
@@ -325,25 +327,39 @@ def main() -> int:
     call_ms = sum(stages.values())
     achieved = tile_bytes / (tile_ms * 1e-3)
 
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(wl)
-
-    for p in list(d_field.values()) + ([d_lay] if d_lay is not None else []) + [d_out]:
-        L.rz_device_free(ctx, p)
-    L.rz_device_geometry_free(dg)
-    L.rz_context_destroy(ctx)
+    cpu, parity = None, None
+    if with_cpu:
+        log(f"{wl.name}: cpu baseline ...")
+        cpu, refs, rows = cpu_baseline(wl)
+        # the GPU's full-size outputs of the sample's calls, rows [0, rows) of
+        # every band, against the oracle bit for bit
+        bad = {}
+        for (dt, fun), ref in refs.items():
+            call(dt, fun)
+            check(L.rz_context_synchronize(ctx))
+            esz = np.dtype(dt).itemsize
+            got = np.empty((wl.bands, rows, wl.W), dtype=dt)
+            for b_ in range(wl.bands):
+                src = c_void_p(d_out.value + b_ * wl.H * wl.W * esz)
+                check(L.rz_memcpy_d2h(ctx, got[b_].ctypes.data, src, rows * wl.W * esz))
+            n_bad = same_bits(got, ref)
+            if n_bad:
+                bad[f"{dt}:{fun}"] = n_bad
+        parity = {"status": "ok" if not bad else "FAILED", "differing_pixels": bad,
+                  "checked": [f"{dt}:{fun}" for dt, fun in refs],
+                  "rows": [0, rows], "bands": wl.bands}
+        log(f"{wl.name}: parity {parity['status']} {bad}")
+    release()
 
     dtypes = sorted({dt for dt, _ in wl.calls})
-    result = {
+    return {
         "metric": METRIC,
         "value": burns_per_step / (ms_per_step * 1e-3),
         "unit": "pixel-burns/s",
         "n_gpus": 1,
         "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "warmup": warmup,
+        "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -371,9 +387,79 @@ def main() -> int:
             "call_frac": round(call_bytes / (call_ms * 1e-3) / HBM_PEAK, 4),
         },
         "cpu_baseline": cpu,
+        "parity": parity,
     }
+
+
+# steps of the legs that ride along the default (large single raster) line
+LEG_STEPS = {"stars": 400, "lines": 5}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="polygons", choices=["polygons", "stars", "lines", "batch", "serve"])
+    ap.add_argument("--rates", default="50,200,500,1000,2000,3000,4000,5000",
+                    help="serve workload: arrival rates (requests/s)")
+    ap.add_argument("--requests", type=int, default=1000,
+                    help="serve workload: requests per rate (long enough for a steady state at 5,000/s)")
+    ap.add_argument("--streams", type=int, default=2, help="serve workload: server worker streams")
+    ap.add_argument("--max-batch", type=int, default=64, help="serve workload: largest batch")
+    ap.add_argument("--pool", type=int, default=48, help="serve workload: distinct geometry sets (stream jobs)")
+    ap.add_argument("--jobs", type=int, default=256, help="batch workload: jobs of the seeded job stream")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="batch workload: autotune the launch knobs on this many jobs first")
+    ap.add_argument("--no-graph", action="store_true", help="batch workload: skip the captured-graph leg")
+    ap.add_argument("--budget-gb", type=float, default=150.0,
+                    help="batch workload: device memory the jobs' inputs, outputs and workspace may use")
+    ap.add_argument("--polygons", type=int, default=1_000_000, help="polygon count of the default workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="default workload: skip the star-polygon and line legs")
+    ap.add_argument("--only", default=None,
+                    help="DTYPE:FUN -- one warm-up and three timed calls of that combination only "
+                         "(for counter collection)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run only a device-to-device copy of this many bytes (the plain copy "
+                         "kernel that calibrates HBM traffic counters)")
+    args = ap.parse_args()
+    if args.copy_bytes:
+        return copy_calibration(args.copy_bytes)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the rasteriser shards by output rows with no exchange "
+              "(see DESIGN.md); multi-GPU runs are not part of this build", file=sys.stderr)
+        return 2
+    if args.workload == "batch":
+        return batch_bench(args)
+    if args.workload == "serve":
+        return serve_bench(args)
+    steps = args.steps if args.steps is not None else {"polygons": 3, "stars": 200, "lines": 5}[args.workload]
+
+    t0 = time.perf_counter()
+    wl = make_workload(args.workload, args.polygons)
+    g = wl.geom
+    log(f"inputs: {g.n_geoms} geometries, {g.n_items} items, {g.n_rings} rings, {g.n_vertices} vertices "
+        f"({time.perf_counter() - t0:.1f} s)")
+    result = measure(wl, steps, args.warmup, not args.no_cpu, args.only)
+    if args.only:
+        print(json.dumps(result), flush=True)
+        return 0
+    ok = result["parity"] is None or result["parity"]["status"] == "ok"
+    if args.workload == "polygons" and not args.no_legs:
+        # the other single-raster configurations, each with its own roofline,
+        # CPU baseline and parity check
+        legs = {}
+        for name in ("stars", "lines"):
+            leg_wl = make_workload(name, 0)
+            r = measure(leg_wl, LEG_STEPS[name], max(1, args.warmup), not args.no_cpu)
+            ok &= r["parity"] is None or r["parity"]["status"] == "ok"
+            legs[name] = {k: r[k] for k in ("value", "unit", "ms_per_step", "steps", "dtype", "config", "roofline",
+                                            "cpu_baseline", "parity")}
+        result["legs"] = legs
     print(json.dumps(result), flush=True)
-    return 0
+    return 0 if ok else 1
 
 
 # ------------------------------------------------------------ batched jobs

@@ -205,13 +205,14 @@ def _rusterize(geometry, raw_raster_info: dict, pypixel_fn: str, pydf=None, pyfi
             raise RuntimeError("Geometry and by lengths must match")
     if field is not None and len(field) != geoms.n_geoms:
         raise RuntimeError("Geometry and field lengths must match")
-
-    out = _lib.rasterize(geoms, info, pydtype, pypixel_fn, field, scalar, layer, n_layers, background,
-                         all_touched=bool(pytouched))
     if pyencoding == "xarray":
-        import xarray  # noqa: F401  (the reference builds a DataArray here)
+        # the reference builds a DataArray from the raster; xarray / rioxarray
+        # are not available here, so this fails before any device work
+        import xarray  # noqa: F401
         raise NotImplementedError("encoding='xarray' output is not implemented")
-    return out
+
+    return _lib.rasterize(geoms, info, pydtype, pypixel_fn, field, scalar, layer, n_layers, background,
+                          all_touched=bool(pytouched))
 
 
 def rusterize(

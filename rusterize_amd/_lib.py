@@ -357,15 +357,6 @@ def rasterize(geom: Geometry, info: RzRasterInfo, dtype: str, fun: str, field: n
     return out
 
 
-def device_available() -> bool:
-    """True when a HIP device is visible (does not initialise the runtime)."""
-    try:
-        import torch  # noqa: F401  (device count without initialising HIP)
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return os.path.exists("/dev/kfd")
-
-
 def job_spec(seed: int, k: int) -> dict:
     """Size of job k of the seeded job stream: side, n_poly, fun, n_vertices."""
     side, n_poly, fun, nv = c_int32(), c_int32(), c_int32(), c_int64()

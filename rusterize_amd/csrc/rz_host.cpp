@@ -36,6 +36,9 @@ rz_status check_geometry(const rz_geometry* g) {
     return fail(RZ_VALUE_ERROR, "geometry arrays missing");
   if (g->geom_item_offsets[0] != 0 || g->geom_item_offsets[g->n_geoms] != g->n_items)
     return fail(RZ_VALUE_ERROR, "geom_item_offsets do not span the items");
+  // every ring belongs to an item (the per-vertex maps index items by ring)
+  if (g->n_items == 0 && (g->n_rings != 0 || g->n_vertices != 0))
+    return fail(RZ_VALUE_ERROR, "rings or vertices without items");
   for (int64_t i = 0; i < g->n_geoms; ++i)
     if (g->geom_item_offsets[i + 1] < g->geom_item_offsets[i])
       return fail(RZ_VALUE_ERROR, "geom_item_offsets not monotone");
@@ -252,11 +255,16 @@ static void close_ring(std::vector<double>& pts) {
 
 namespace {
 
+// Collections and multi geometries nest at most this deep in the parsers
+// (deeper input is rejected instead of exhausting the stack).
+constexpr int kMaxNesting = 64;
+
 struct WktParser {
   const char* s;
   const char* e;
   rz_geometry_buffers* out;
   std::string err;
+  int depth = 0;
 
   void ws() {
     while (s < e && (*s == ' ' || *s == '\t' || *s == '\n' || *s == '\r')) ++s;
@@ -371,6 +379,16 @@ struct WktParser {
     return expect(')');
   }
   bool geometry() {
+    if (depth >= kMaxNesting) {
+      err = "geometry nested deeper than 64 levels";
+      return false;
+    }
+    ++depth;
+    const bool okg = geometry_body();
+    --depth;
+    return okg;
+  }
+  bool geometry_body() {
     std::string kw = word();
     auto starts = [&](const char* b) { return kw.rfind(b, 0) == 0; };
     if (starts("GEOMETRYCOLLECTION")) {
@@ -482,6 +500,7 @@ struct WkbParser {
   const uint8_t* e;
   rz_geometry_buffers* out;
   std::string err;
+  int depth = 0;
 
   bool u8(uint8_t* v) {
     if (p + 1 > e) return fail_trunc();
@@ -508,7 +527,14 @@ struct WkbParser {
     std::memcpy(v, &x, 8);
     return true;
   }
+  // Fails unless n records of at least `bytes` bytes each can follow.
+  bool fits(uint32_t n, size_t bytes) {
+    if ((uint64_t)n * bytes > (uint64_t)(e - p)) return fail_trunc();
+    return true;
+  }
   bool coords(bool le, int nd, uint32_t n, std::vector<double>& pts) {
+    if (!fits(n, 8u * (size_t)nd)) return false;
+    pts.reserve(pts.size() + 2 * (size_t)n);
     for (uint32_t i = 0; i < n; ++i) {
       double v[4];
       for (int k = 0; k < nd; ++k)
@@ -521,6 +547,16 @@ struct WkbParser {
   // Parses one WKB geometry. `top` marks the outermost level. `skip` is set
   // when the geometry is an empty point (geo cannot represent it).
   bool geometry(bool* skip) {
+    if (depth >= kMaxNesting) {
+      err = "geometry nested deeper than 64 levels";
+      return false;
+    }
+    ++depth;
+    const bool okg = geometry_body(skip);
+    --depth;
+    return okg;
+  }
+  bool geometry_body(bool* skip) {
     uint8_t bo;
     if (!u8(&bo)) return false;
     if (bo > 1) {
@@ -566,7 +602,7 @@ struct WkbParser {
         return true;
       }
       case 3: {  // Polygon
-        if (!u32(le, &n)) return false;
+        if (!u32(le, &n) || !fits(n, 4)) return false;  // each ring holds a 4-byte count
         std::vector<std::vector<double>> rings(n);
         for (uint32_t r = 0; r < n; ++r) {
           uint32_t m;
@@ -579,7 +615,7 @@ struct WkbParser {
         return true;
       }
       case 4: {  // MultiPoint
-        if (!u32(le, &n)) return false;
+        if (!u32(le, &n) || !fits(n, 21)) return false;  // a point member is 21 bytes at least
         std::vector<double> pts;
         for (uint32_t i = 0; i < n; ++i) {
           uint8_t b2;
@@ -610,9 +646,9 @@ struct WkbParser {
       }
       case 5:    // MultiLineString
       case 6: {  // MultiPolygon
-        if (!u32(le, &n)) return false;
+        if (!u32(le, &n) || !fits(n, 9)) return false;  // a member header is 9 bytes at least
         rz_geometry_buffers tmp;
-        WkbParser sub{p, e, &tmp, {}};
+        WkbParser sub{p, e, &tmp, {}, depth};
         for (uint32_t i = 0; i < n; ++i)
           if (!sub.geometry(nullptr)) {
             err = sub.err;
@@ -633,7 +669,7 @@ struct WkbParser {
         return true;
       }
       case 7: {  // GeometryCollection
-        if (!u32(le, &n)) return false;
+        if (!u32(le, &n) || !fits(n, 9)) return false;
         for (uint32_t i = 0; i < n; ++i) {
           bool s2 = false;
           if (!geometry(&s2)) return false;
@@ -649,11 +685,33 @@ struct WkbParser {
 
 }  // namespace
 
-extern "C" rz_status rz_geometry_from_wkt(const char* const* texts, const int64_t* lengths,
-                                          int64_t n, rz_geometry_buffers** out) {
+// The parser bodies allocate with std::vector; an allocation failure (or any
+// other exception) is mapped to a status here, so nothing propagates across
+// the C boundary.
+template <typename F>
+static rz_status guarded(F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return rz::fail(RZ_CAPACITY_ERROR, "Cannot parse geometry: out of host memory");
+  } catch (const std::exception& ex) {
+    return rz::fail(RZ_PARSE_ERROR, std::string("Cannot parse geometry: ") + ex.what());
+  }
+}
+
+static rz_status from_wkt(const char* const* texts, const int64_t* lengths, int64_t n,
+                          rz_geometry_buffers** out) {
+  if (n < 0 || (n > 0 && (!texts || !lengths)) || !out)
+    return rz::fail(RZ_VALUE_ERROR, "bad argument to rz_geometry_from_wkt");
   auto* b = new (std::nothrow) rz_geometry_buffers();
   if (!b) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
+  *out = b;  // the caller frees it if a parser throws
   for (int64_t i = 0; i < n; ++i) {
+    if (lengths[i] < 0 || (lengths[i] > 0 && !texts[i])) {
+      delete b;
+      *out = nullptr;
+      return rz::fail(RZ_VALUE_ERROR, "bad WKT length");
+    }
     WktParser ps{texts[i], texts[i] + lengths[i], b, {}};
     bool okp = ps.geometry();
     ps.ws();
@@ -663,6 +721,7 @@ extern "C" rz_status rz_geometry_from_wkt(const char* const* texts, const int64_
     }
     if (!okp) {
       delete b;
+      *out = nullptr;
       return rz::fail(RZ_PARSE_ERROR, "Cannot parse geometry. Check that the WKT is valid. (" +
                                            ps.err + " in geometry " + std::to_string(i) + ")");
     }
@@ -672,17 +731,38 @@ extern "C" rz_status rz_geometry_from_wkt(const char* const* texts, const int64_
   return RZ_OK;
 }
 
-extern "C" rz_status rz_geometry_from_wkb(const uint8_t* const* blobs, const int64_t* lengths,
+extern "C" rz_status rz_geometry_from_wkt(const char* const* texts, const int64_t* lengths,
                                           int64_t n, rz_geometry_buffers** out) {
+  rz_geometry_buffers* b = nullptr;
+  rz_status st = guarded([&] { return from_wkt(texts, lengths, n, &b); });
+  if (st != RZ_OK) {
+    delete b;  // null unless a parser threw
+    return st;
+  }
+  *out = b;
+  return RZ_OK;
+}
+
+static rz_status from_wkb(const uint8_t* const* blobs, const int64_t* lengths, int64_t n,
+                          rz_geometry_buffers** out) {
+  if (n < 0 || (n > 0 && (!blobs || !lengths)) || !out)
+    return rz::fail(RZ_VALUE_ERROR, "bad argument to rz_geometry_from_wkb");
   auto* b = new (std::nothrow) rz_geometry_buffers();
   if (!b) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
+  *out = b;  // the caller frees it if a parser throws
   for (int64_t i = 0; i < n; ++i) {
     size_t items_before = b->item_kind.size();
     size_t rings_before = b->ring_vertex_offsets.size();
+    if (lengths[i] < 0 || (lengths[i] > 0 && !blobs[i])) {
+      delete b;
+      *out = nullptr;
+      return rz::fail(RZ_VALUE_ERROR, "bad WKB length");
+    }
     WkbParser ps{blobs[i], blobs[i] + lengths[i], b, {}};
     bool skip = false;
     if (!ps.geometry(&skip)) {
       delete b;
+      *out = nullptr;
       return rz::fail(RZ_PARSE_ERROR,
                       "Cannot parse geometry. Check that the WKB bytes are valid. (" + ps.err +
                           " in geometry " + std::to_string(i) + ")");
@@ -696,6 +776,18 @@ extern "C" rz_status rz_geometry_from_wkb(const uint8_t* const* blobs, const int
       continue;
     }
     b->end_geometry();
+  }
+  *out = b;
+  return RZ_OK;
+}
+
+extern "C" rz_status rz_geometry_from_wkb(const uint8_t* const* blobs, const int64_t* lengths,
+                                          int64_t n, rz_geometry_buffers** out) {
+  rz_geometry_buffers* b = nullptr;
+  rz_status st = guarded([&] { return from_wkb(blobs, lengths, n, &b); });
+  if (st != RZ_OK) {
+    delete b;  // null unless a parser threw
+    return st;
   }
   *out = b;
   return RZ_OK;

@@ -82,6 +82,7 @@ struct JobDev {
   const void* field;         // [n_geoms] of the batch dtype, or null
   uint64_t scalar_bits, bg_bits;
   const uint32_t* layer_of_geom;  // may be null
+  uint32_t n_layers;         // bands of the output; a layer index >= n_layers is an error
   void* out;                 // [layers][nrows][ncols]
 };
 
@@ -197,6 +198,7 @@ struct DescArgs {
   int wide_mode;            // wide pre-pass: 0 chosen per item, 1 never, 2 every polygon item
   uint32_t* wide_units;     // [n_items] k_wide_cover units of a wide item (0: not wide), or null
   uint32_t* wide_words;     // [n_items] row-mask words of a wide item
+  uint32_t* status;         // bit 4 set when a geometry's layer index is >= the job's n_layers
 };
 
 // Wide items within the capacities get their flag, mask base and unit list.

@@ -267,8 +267,16 @@ __global__ void k_item_desc(DescArgs a) {
   const int j = find_job_wave(a.job_iend, a.n_jobs, i);
   const RZ_CONST JobDev& J = as_const(a.jobs)[j];
   const int64_t li = i - J.i_base;
-  const ItemBox b = a.item_box[i];
+  ItemBox b = a.item_box[i];
   const int32_t g = J.item_geom[li];
+  uint32_t layer = J.layer_of_geom ? J.layer_of_geom[g] : 0u;
+  if (layer >= J.n_layers) {
+    // an out-of-range band index burns nothing (its tiles would be another
+    // band's or another job's) and fails the call
+    atomicOr(a.status, 16u);
+    layer = 0u;
+    b = ItemBox{0, 0, 0, 0};
+  }
   ItemDesc d;
   d.r0 = b.r0;
   d.r1 = b.r1;
@@ -276,7 +284,6 @@ __global__ void k_item_desc(DescArgs a) {
   d.c1 = b.c1;
   d.vbeg = J.v_base + J.item_vbeg[li];
   d.nv = (int32_t)(J.item_vbeg[li + 1] - J.item_vbeg[li]);
-  const uint32_t layer = J.layer_of_geom ? J.layer_of_geom[g] : 0u;
   d.kind_layer = (uint32_t)J.item_kind[li] | (layer << 2);
   uint64_t v = J.scalar_bits;
   if (J.field) {

@@ -215,13 +215,20 @@ extern "C" size_t rz_context_workspace_bytes(const rz_context* c) {
   return s;
 }
 
+// Reads the error flags the context's kernels raised (the caller has
+// synchronised the stream) and clears them, so each failure is reported once.
 static rz_status check_status_word(rz_context* c) {
   uint32_t* st = (uint32_t*)((char*)c->misc.p);
-  HIP_TRY(hipMemcpy(&c->host_misc[1], st, sizeof(uint32_t), hipMemcpyDeviceToHost));
-  uint32_t flags = (uint32_t)c->host_misc[1];
+  uint32_t flags = 0;
+  HIP_TRY(hipMemcpy(&flags, st, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (flags) {
+    const uint32_t zero = 0;
+    HIP_TRY(hipMemcpy(st, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
   if (flags & 1u)
     return rz::fail(RZ_CAPACITY_ERROR,
                     "line coordinates beyond +-2^60 pixels are outside the supported range");
+  if (flags & 16u) return rz::fail(RZ_VALUE_ERROR, "layer index out of range");
   if (flags & (4u | 8u))
     return rz::fail(RZ_CAPACITY_ERROR,
                     "the batch outgrew the capacities it was captured with; capture it again");
@@ -685,6 +692,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     J.scalar_bits = j.scalar_bits;
     J.bg_bits = j.background_bits;
     J.layer_of_geom = j.layer_dev;
+    J.n_layers = nl;
     J.out = j.out_dev;
     const bool lines = has_lines(j);
     any_tw64 |= TW == 64;
@@ -811,6 +819,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     da.wide_mode = wide_mode;
     da.wide_units = any_wide ? (uint32_t*)c->wide_units.p : nullptr;
     da.wide_words = any_wide ? (uint32_t*)c->wide_words.p : nullptr;
+    da.status = d_status;
     hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));

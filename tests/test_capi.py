@@ -165,6 +165,57 @@ def test_parse_errors():
         Geometry.from_wkb([b"\x01\x01\x00"])
 
 
+def test_wkb_counts_beyond_the_blob_are_rejected():
+    # a ring / member / vertex count larger than the remaining bytes is a
+    # parse error before anything is allocated for it
+    for blob in (b"\x01" + struct.pack("<II", 3, 0xFFFFFFFF),                    # polygon rings
+                 b"\x01" + struct.pack("<II", 2, 0xFFFFFFFF),                    # line vertices
+                 b"\x01" + struct.pack("<II", 4, 0x7FFFFFFF),                    # multipoint members
+                 b"\x01" + struct.pack("<II", 6, 0x10000000),                    # multipolygon members
+                 b"\x01" + struct.pack("<II", 7, 0xFFFFFFFF),                    # collection members
+                 b"\x01" + struct.pack("<III", 3, 1, 0x40000000)):               # ring vertices
+        with pytest.raises(ValueError, match="Cannot parse geometry"):
+            Geometry.from_wkb([blob])
+    # truncated after a valid header
+    with pytest.raises(ValueError, match="truncated"):
+        Geometry.from_wkb([wkb_point(1, 2)[:-3]])
+
+
+def test_nesting_depth_is_capped():
+    deep = "GEOMETRYCOLLECTION (" * 100 + "POINT (1 1)" + ")" * 100
+    with pytest.raises(ValueError, match="nested deeper"):
+        Geometry.from_wkt([deep])
+    ok = "GEOMETRYCOLLECTION (" * 10 + "POINT (1 1)" + ")" * 10
+    assert Geometry.from_wkt([ok]).n_items == 1
+    blob = wkb_point(1, 1)
+    for _ in range(100):
+        blob = b"\x01" + struct.pack("<II", 7, 1) + blob
+    with pytest.raises(ValueError, match="nested deeper"):
+        Geometry.from_wkb([blob])
+
+
+def test_rings_without_items_are_rejected():
+    # n_items == 0 with rings / vertices: the per-vertex maps would index an
+    # item that does not exist, so both the host copy and the device upload
+    # refuse the view before any launch
+    g = _lib.RzGeometry()
+    gio = np.zeros(2, np.int64)
+    rvo = np.array([0, 3], np.int64)
+    xy = np.zeros(6, np.float64)
+    g.n_geoms, g.n_items, g.n_rings, g.n_vertices = 1, 0, 1, 3
+    g.geom_item_offsets = gio.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    g.item_kind = None
+    g.item_ring_offsets = None
+    g.ring_vertex_offsets = rvo.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    g.xy = xy.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    g.ring_flags = None
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.rz_geometry_from_arrays(ctypes.byref(g), ctypes.byref(h)) == _lib.RZ_VALUE_ERROR
+    assert L.rz_device_geometry_upload(None, ctypes.byref(g), ctypes.byref(h)) == _lib.RZ_VALUE_ERROR
+    assert "without items" in _lib.last_error()
+
+
 def test_synth_is_deterministic():
     a = Geometry.synth(2, 5, 200, 100.0, 100.0, 2.0, 50.0, 0.1, 0.2)
     b = Geometry.synth(2, 5, 200, 100.0, 100.0, 2.0, 50.0, 0.1, 0.2)
@@ -181,8 +232,18 @@ def test_synth_is_deterministic():
     np.testing.assert_array_equal(f, f32.astype(np.uint8))  # numpy wrap semantics
 
 
+def device_available() -> bool:
+    """True when a HIP device is visible (counting devices does not
+    initialise the HIP runtime)."""
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return Path("/dev/kfd").exists()
+
+
 def test_device_entry_points_fail_loudly_without_gpu():
-    if _lib.device_available():
+    if device_available():
         pytest.skip("a GPU is present; the GPU tests cover the device path")
     g = Geometry.from_wkt([sq(0, 0, 4, 4)])
     info = _lib.raster_info(g, shape=(4, 4))

@@ -266,3 +266,79 @@ def test_autotuner_fills_table_and_keeps_results(ctx):
     finally:
         autotune_clear()
         b.free()
+
+
+def test_large_stream_jobs_batched_and_replayed(ctx):
+    # jobs of the seeded 4,096-job stream at full size: 7,652 px with rings of
+    # up to 4,047 vertices (job 41), 31,047 polygons on 6,952 px (job 761),
+    # 30,570 polygons on 681 px (job 267) and 8,009 px (job 334); batched,
+    # then replayed from a captured graph, every output against the oracle
+    # on the host generator's geometry
+    from rusterize_amd.device import BatchGraph
+    seed, picks = 4, (41, 267, 334, 761)
+    geoms, jobs, refs, bufs = [], [], [], []
+    g = None
+    try:
+        for k in picks:
+            sp = _lib.job_spec(seed, k)
+            dg = DeviceGeometry.synth_job(ctx, seed, k)
+            geoms.append(dg)
+            field = _lib.job_field(seed, k)
+            d_field = DeviceBuffer.from_array(ctx, field)
+            out = DeviceBuffer(ctx, sp["side"] ** 2 * 8)
+            bufs += [d_field, out]
+            info = grid(sp["side"], sp["side"])
+            jobs.append(Job(dg, info, "float64", sp["fun"], out, field=d_field, background=float("nan")))
+            print(f"stream job {k}: {sp}", flush=True)
+            refs.append(oracle.rasterize(Geometry.synth_job(seed, k), oracle.copy_info(info), "float64", sp["fun"],
+                                         field=field, background=float("nan")))
+        batch_rasterize(ctx, jobs)
+        ctx.synchronize()
+        for k, j, ref in zip(picks, jobs, refs):
+            assert_same(j.result(), ref, f"stream job {k} batched")
+        g = BatchGraph(jobs, headroom=1.0)
+        for rep in range(2):
+            for j in jobs:
+                j.out.upload(np.full(j.out_bytes, 0x3C, np.uint8))
+            g.launch()
+            g.synchronize()
+            for k, j, ref in zip(picks, jobs, refs):
+                assert_same(j.result(), ref, f"stream job {k} graph replay {rep}")
+    finally:
+        if g is not None:
+            g.close()
+        for b in bufs:
+            b.free()
+        for d in geoms:
+            d.free()
+
+
+def test_out_of_range_layer_fails_the_batch_and_spares_other_jobs(ctx):
+    # a band index >= n_layers burns nothing and fails the call; the other
+    # jobs of the batch still equal the oracle
+    b = Batch(ctx, SPECS, seed=6)
+    try:
+        k_bad = 2  # 4 bands
+        g = b.host_geoms[SPECS[k_bad][0]]
+        layers = np.zeros(g.n_geoms, np.uint32)
+        layers[5] = 4
+        b.jobs[k_bad].layers.upload(layers)
+        batch_rasterize(ctx, b.jobs)
+        with pytest.raises(ValueError, match="layer index out of range"):
+            ctx.synchronize()
+        for k, j in enumerate(b.jobs):
+            if k != k_bad:
+                assert_same(j.result(), b.refs[k], f"job {k} next to the failing one")
+        # the failing geometry alone is dropped: the rest of that job burns as
+        # the oracle does without it
+        keep = np.ones(g.n_geoms, bool)
+        keep[5] = False
+        field = field_for(SPECS[k_bad][5], g.n_geoms, seed=100 + k_bad)
+        field[::13] = 0
+        ref = oracle.rasterize(g.subset(np.flatnonzero(keep)), oracle.copy_info(b.jobs[k_bad].info), SPECS[k_bad][5],
+                               SPECS[k_bad][6], field=field[keep], layer_of_geom=layers[keep], n_layers=4,
+                               background=SPECS[k_bad][8])
+        assert_same(b.jobs[k_bad].result(), ref, "failing job without its bad geometry")
+        ctx.synchronize()  # the error was reported once
+    finally:
+        b.free()

@@ -119,3 +119,72 @@ def test_admission_waits_for_memory_and_rejects_oversized(ctx):
         srv.close()
         field.free()
         dg.free()
+
+
+def test_served_stream_requests_match_oracle(ctx):
+    # the served workload's request stream at test size: Poisson arrivals
+    # (500 requests/s) drawn from the seeded batched-job stream (jobs of at most
+    # 2,048 px and 150,000 vertices, generated on the device), 30 % of them
+    # reusing a cached geometry set with a new window, field and function;
+    # every response against the oracle
+    seed = 4
+    picks = []
+    for k in range(600):
+        sp = _lib.job_spec(seed, k)
+        if sp["side"] <= 2048 and sp["n_vertices"] <= 150000:
+            picks.append((k, sp))
+        if len(picks) == 45:
+            break
+    rng = np.random.default_rng(11)
+    cache, reqs, bufs, devs = [], [], [], []
+    srv = Server(0, budget_bytes=8 * 10**9, n_streams=2, max_batch=16)
+    try:
+        t, nxt = 0.0, 0
+        for _ in range(64):
+            t += rng.exponential(1.0 / 500.0)
+            if cache and (rng.random() < 0.3 or nxt == len(picks)):
+                k, sp, dg, host = cache[int(rng.integers(0, len(cache)))]
+                w = max(16, int(sp["side"] * rng.uniform(0.3, 1.0)))
+                x0, y0 = float(rng.integers(-64, sp["side"] - w + 64)), float(rng.integers(-64, sp["side"] - w + 64))
+                field = rng.random(host.n_geoms)
+                info, fun = grid(w, w, x0, y0), _lib.FUNS[int(rng.integers(0, 7))]
+            else:
+                k, sp = picks[nxt]
+                nxt += 1
+                dg = DeviceGeometry.synth_job(ctx, seed, k)
+                devs.append(dg)
+                host = Geometry.synth_job(seed, k)
+                cache.append((k, sp, dg, host))
+                field = _lib.job_field(seed, k)
+                info, fun = grid(sp["side"], sp["side"]), sp["fun"]
+            d_field = DeviceBuffer.from_array(ctx, field)
+            bufs.append(d_field)
+            reqs.append((t, Job(dg, info, "float64", fun, None, field=d_field, background=float("nan")), host, field, k))
+        ctx.synchronize()
+        tickets = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def producer():
+            for i, (ta, job, _, _, _) in enumerate(reqs):
+                d = t0 + ta - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(job)
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        for i, (ta, job, host, field, k) in enumerate(reqs):
+            got = srv.result(ctx, tickets[i])
+            ref = oracle.rasterize(host, oracle.copy_info(job.info), "float64", job.fun, field=field,
+                                   background=float("nan"))
+            assert_same(got, ref, f"served request {i} (stream job {k}, {job.fun}, {job.shape})")
+            srv.release(tickets[i])
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["batches"] < len(reqs)
+    finally:
+        srv.close()
+        for b in bufs:
+            b.free()
+        for d in devs:
+            d.free()

@@ -33,8 +33,17 @@
  *   pixel cache      golden standard_output_sum_custom_shape (xres != yres)
  *   pixel functions  golden rasters (sum), R known answers, documented
  *                    triplets of python/docs/python.md:106-136
- *   burn_line_all_touched: parity unpinned (the reference tests it only
- *                    against GDAL, which is absent; not on the device path).
+ *   all-touched      (tests/test_all_touched_oracle.py; the reference tests
+ *                    it only against GDAL, which is absent) the R known
+ *                    answer, closed forms of the axis-aligned branches and
+ *                    of slope-1 diagonals, and exact rational geometry on
+ *                    generic segments and triangles: a segment burns its
+ *                    supercover, a polygon every cell whose interior it
+ *                    meets; count with dedup is 1 on the touched set; a
+ *                    planted walker slip fails the supercover check.
+ *                    Cases within the walker's tolerances (segments through
+ *                    grid corners, near-axis segments) are pinned only by
+ *                    the closed forms and the R answer.
  */
 #include <math.h>
 #include <pthread.h>

@@ -246,6 +246,25 @@ rz_status rz_rasterize_device(rz_context* ctx, const rz_device_geometry* dg,
                               const void* background_host, int all_touched,
                               rz_sum_mode sum_mode, void* out_dev);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as rz_rasterize_device, with no
+ * allocation of any kind: the workspace is a fixed 64 MiB array in the
+ * library's device code (one per device, held until the process ends),
+ * whose address rz_context_create resolves once per device.  Every item box
+ * is tested by every tile strip (no binning), so it suits small and medium
+ * geometry sets; a set whose workspace exceeds 64 MiB (about 49 B per vertex
+ * plus 88 B per item) fails with RZ_CAPACITY_ERROR, and all_touched with
+ * RZ_VALUE_ERROR.  The call is synchronous: it returns once out_dev is
+ * written.  Calls on one device share the array and are serialised by the
+ * library; ctx must come from rz_context_create. */
+rz_status rz_rasterize_device_noalloc(rz_context* ctx, const rz_device_geometry* dg,
+                                      const rz_raster_info* info, rz_dtype dtype, rz_fun fun,
+                                      const void* field_dev, const void* scalar_host,
+                                      const uint32_t* layer_dev, uint32_t n_layers,
+                                      const void* background_host, int all_touched,
+                                      rz_sum_mode sum_mode, void* out_dev);
+
 /* ------------------------------------------------------------- batching -- */
 
 /* No reference counterpart (the reference rasterises one call at a time):

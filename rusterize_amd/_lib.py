@@ -102,6 +102,8 @@ _SIGNATURES = [
     ("rz_device_synth_job", c_int, [c_void_p, c_uint64, c_int64, POINTER(c_void_p)]),
     ("rz_rasterize_device", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
+    ("rz_rasterize_device_noalloc", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p,
+                                            c_void_p, c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_batch_rasterize", c_int, [c_void_p, POINTER(RzJob), c_int64]),
     ("rz_batch_graph_create", c_int, [c_int, POINTER(RzJob), c_int64, c_double, POINTER(c_void_p)]),
     ("rz_batch_graph_launch", c_int, [c_void_p]),

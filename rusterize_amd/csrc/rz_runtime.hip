@@ -160,6 +160,34 @@ struct rz_device_geometry {
   int64_t max_poly_nv = 0;  // most vertices of one polygon item
 };
 
+// ------------------------------------------------ allocation-free workspace --
+// Device memory of rz_rasterize_device_noalloc: a fixed array in the code
+// object, so that entry point never allocates.  Every device the library
+// runs on holds its own copy until the process ends.  In host code the name
+// `scratch` is only a stand-in; its device address comes from
+// hipGetSymbolAddress, taken once per device when the device's first
+// context is created (rz_context_create) and kept in g_scratch.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;              // guards g_scratch
+static void* g_scratch[kMaxDevices] = {};    // device address of `scratch` per device
+static std::mutex g_noalloc_mu[kMaxDevices]; // one allocation-free call per device at a time
+
+// Resolves `scratch` on `device` (the current device); once per device.
+static rz_status scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return RZ_OK;  // that device has no allocation-free path
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return RZ_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return rz::fail(RZ_DEVICE_ERROR, std::string("HIP error '") + hipGetErrorString(e) +
+                                         "' resolving the allocation-free workspace");
+  g_scratch[device] = p;
+  return RZ_OK;
+}
+
 // ------------------------------------------------------------- contexts --
 
 static rz_status set_device(int device) {
@@ -184,6 +212,11 @@ extern "C" rz_status rz_context_create(int device, rz_context** out) {
   auto* c = new (std::nothrow) rz_context();
   if (!c) return rz::fail(RZ_CAPACITY_ERROR, "out of host memory");
   c->device = device;
+  st = scratch_setup(device);
+  if (st) {
+    delete c;
+    return st;
+  }
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipHostMalloc((void**)&c->host_misc, 6 * sizeof(uint64_t), hipHostMallocDefault));
   HIP_TRY(hipEventCreateWithFlags(&c->tab_ev, hipEventDisableTiming));
@@ -584,6 +617,53 @@ struct BatchCaps {
   uint64_t pairs, band_entries, band_edges, wide_units, wide_words;
 };
 
+// The device record of job j, whose vertex / item / tile ranges start at
+// v_base / i_base / t_base of its batch (J.scan is left 0); returns the
+// job's tile count (tiles per band x bands).
+static uint64_t job_record(const rz_job& j, JobDev& J, int64_t v_base, int64_t i_base, uint64_t t_base) {
+  const rz_device_geometry* dg = j.geometry;
+  const int esz = (int)rz_dtype_size(j.dtype);
+  const int TW = tile_w(esz);
+  const uint32_t nl = j.layer_dev ? j.n_layers : 1u;
+  const int32_t nrows = (int32_t)j.info.nrows, ncols = (int32_t)j.info.ncols;
+  const int32_t ntx = (ncols + TW - 1) / TW, nty = (nrows + TILE_H - 1) / TILE_H;
+  const uint64_t tpl = (uint64_t)ntx * (uint64_t)nty, jt = tpl * nl;
+  std::memset(&J, 0, sizeof(J));
+  J.xy = dg->xy;
+  J.vert_info = dg->vert_info;
+  J.ring_voff = dg->ring_voff;
+  J.ring_item = dg->ring_item;
+  J.item_kind = dg->item_kind;
+  J.item_geom = dg->item_geom;
+  J.item_vbeg = dg->item_vbeg;
+  J.v_base = v_base;
+  J.i_base = i_base;
+  J.nv = dg->V;
+  J.ni = dg->I;
+  J.xmin = j.info.xmin;
+  J.ymax = j.info.ymax;
+  J.xres = j.info.xres;
+  J.yres = j.info.yres;
+  J.nrows = nrows;
+  J.ncols = ncols;
+  J.ntx = ntx;
+  J.nty = nty;
+  J.esz = esz;
+  J.tile_w = TW;
+  J.tiles_per_layer = (uint32_t)std::min<uint64_t>(tpl, UINT32_MAX);
+  J.tile_base = (uint32_t)std::min<uint64_t>(t_base, UINT32_MAX);
+  J.n_tiles = (uint32_t)std::min<uint64_t>(jt, UINT32_MAX);
+  J.fun = (int32_t)j.fun;
+  J.dedup_lines = j.info.xres != j.info.yres;
+  J.field = j.field_dev;
+  J.scalar_bits = j.scalar_bits;
+  J.bg_bits = j.background_bits;
+  J.layer_of_geom = j.layer_dev;
+  J.n_layers = nl;
+  J.out = j.out_dev;
+  return jt;
+}
+
 static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                            const BatchCaps* caps = nullptr) {
   const bool capture = caps != nullptr;
@@ -643,40 +723,11 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   for (size_t k = 0; k < n; ++k) {
     const rz_job& j = jobs_in[order[k]];
     const rz_device_geometry* dg = j.geometry;
-    const int esz = (int)rz_dtype_size(j.dtype);
-    const int TW = tile_w(esz);
-    const uint32_t nl = j.layer_dev ? j.n_layers : 1u;
-    const int32_t nrows = (int32_t)j.info.nrows, ncols = (int32_t)j.info.ncols;
-    const int32_t ntx = (ncols + TW - 1) / TW, nty = (nrows + TILE_H - 1) / TILE_H;
-    const uint64_t tpl = (uint64_t)ntx * (uint64_t)nty, jt = tpl * nl;
     JobDev& J = tab[k];
-    std::memset(&J, 0, sizeof(J));
-    J.xy = dg->xy;
-    J.vert_info = dg->vert_info;
-    J.ring_voff = dg->ring_voff;
-    J.ring_item = dg->ring_item;
-    J.item_kind = dg->item_kind;
-    J.item_geom = dg->item_geom;
-    J.item_vbeg = dg->item_vbeg;
-    J.v_base = V;
-    J.i_base = I;
-    J.nv = dg->V;
-    J.ni = dg->I;
-    J.xmin = j.info.xmin;
-    J.ymax = j.info.ymax;
-    J.xres = j.info.xres;
-    J.yres = j.info.yres;
-    J.nrows = nrows;
-    J.ncols = ncols;
-    J.ntx = ntx;
-    J.nty = nty;
-    J.esz = esz;
-    J.tile_w = TW;
-    J.tiles_per_layer = (uint32_t)std::min<uint64_t>(tpl, UINT32_MAX);
-    J.tile_base = (uint32_t)std::min<uint64_t>(T, UINT32_MAX);
-    J.n_tiles = (uint32_t)std::min<uint64_t>(jt, UINT32_MAX);
-    J.fun = (int32_t)j.fun;
-    J.dedup_lines = j.info.xres != j.info.yres;
+    const uint64_t jt = job_record(j, J, V, I, T);
+    const int TW = J.tile_w;
+    const uint32_t nl = J.n_layers;
+    const int32_t nrows = J.nrows, ncols = J.ncols;
     const uint64_t ni = (uint64_t)dg->I;
     // scan: every strip tests every item box; otherwise the items are binned
     // into per-tile lists.  Binning a lone call costs a host round trip and
@@ -688,12 +739,6 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     else if (binning == -1) scan = true;
     J.scan = scan ? 1 : 0;
     any_binned |= !scan && ni > 0;
-    J.field = j.field_dev;
-    J.scalar_bits = j.scalar_bits;
-    J.bg_bits = j.background_bits;
-    J.layer_of_geom = j.layer_dev;
-    J.n_layers = nl;
-    J.out = j.out_dev;
     const bool lines = has_lines(j);
     any_tw64 |= TW == 64;
     any_tw256 |= TW == 256;
@@ -1331,6 +1376,149 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
   std::memcpy(&j.background_bits, bg_host, esz);
   j.out_dev = out_dev;
   return run_batch(c, &j, 1);
+}
+
+// ------------------------------------------- allocation-free entry point --
+
+// Writes the job record and its prefix ends (a batch of one) into the
+// workspace; the record travels as a kernel argument, so no host copy and no
+// staging buffer are involved.
+__global__ void k_put_job(JobDev J, int64_t nv, int64_t ni, int64_t nt, JobDev* tab) {
+  if (threadIdx.x == 0) {
+    tab[0] = J;
+    int64_t* ends = (int64_t*)(tab + 1);
+    ends[0] = nv;
+    ends[1] = ni;
+    ends[2] = nt;
+  }
+}
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_geometry* dg,
+                                                 const rz_raster_info* info, rz_dtype dtype, rz_fun fun,
+                                                 const void* field_dev, const void* scalar_host,
+                                                 const uint32_t* layer_dev, uint32_t n_layers,
+                                                 const void* bg_host, int all_touched,
+                                                 rz_sum_mode sum_mode, void* out_dev) {
+  if (!c || !dg || !info || !bg_host || !out_dev)
+    return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_rasterize_device_noalloc");
+  if ((unsigned)dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
+  if ((unsigned)fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
+  if (!field_dev && !scalar_host) return rz::fail(RZ_VALUE_ERROR, "no field and no scalar value");
+  if (all_touched)
+    return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not available on the allocation-free path");
+  (void)sum_mode;  // the ordered fold is exact, so it also serves the unordered mode
+  if (info->nrows == 0 || info->ncols == 0) return RZ_OK;  // empty raster: nothing to write
+  if (c->device < 0 || c->device >= kMaxDevices || !g_scratch[c->device])
+    return rz::fail(RZ_DEVICE_ERROR, "no allocation-free workspace on this device");
+  const size_t esz = rz_dtype_size(dtype);
+  rz_job j;
+  std::memset(&j, 0, sizeof(j));
+  j.geometry = dg;
+  j.info = *info;
+  j.dtype = dtype;
+  j.fun = fun;
+  j.field_dev = field_dev;
+  if (!field_dev) std::memcpy(&j.scalar_bits, scalar_host, esz);
+  j.layer_dev = layer_dev;
+  j.n_layers = layer_dev ? n_layers : 1u;
+  std::memcpy(&j.background_bits, bg_host, esz);
+  j.out_dev = out_dev;
+  JobDev J;
+  const uint64_t jt = job_record(j, J, 0, 0, 0);
+  J.scan = 1;  // no bins: every strip tests the item boxes itself
+  if (jt >= ((uint64_t)1 << 29))
+    return rz::fail(RZ_CAPACITY_ERROR, "more than 2^29 output tiles in one call or batch");
+  const int64_t V = dg->V, I = dg->I;
+  const bool lines = dg->has_line || dg->has_point;
+  // workspace layout in `scratch`: job record + ends, status / burn words,
+  // tile queue, item boxes, item stats, descriptors, edges, line coordinates
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const size_t o_tab = take(sizeof(JobDev) + 3 * sizeof(int64_t));
+  const size_t o_misc = take(64 + kBurnSlots * sizeof(unsigned long long));
+  const size_t o_queue = take((kQueueWords + 1) * 4);
+  const size_t o_box = take((size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox));
+  const size_t o_last = take((size_t)std::max<int64_t>(I, 1) * 8);
+  const size_t o_desc = take((size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc));
+  const size_t o_edges = dg->has_poly ? take((size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)) : 0;
+  const size_t o_px = lines ? take((size_t)std::max<int64_t>(V, 1) * 16) : 0;
+  const size_t o_vflags = lines ? take((size_t)std::max<int64_t>(V, 1)) : 0;
+  if (off > kScratchBytes)
+    return rz::fail(RZ_CAPACITY_ERROR,
+                    "the geometry needs " + std::to_string(off) +
+                        " bytes of workspace; the allocation-free entry point holds at most " +
+                        std::to_string(kScratchBytes) + " (use rz_rasterize_device)");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu[c->device]);
+  HIP_TRY(hipSetDevice(c->device));
+  char* base = (char*)g_scratch[c->device];
+  hipStream_t s = c->stream;
+  JobDev* d_tab = (JobDev*)(base + o_tab);
+  const int64_t* d_ends = (const int64_t*)(d_tab + 1);
+  uint32_t* d_status = (uint32_t*)(base + o_misc);
+  uint32_t* d_queue = (uint32_t*)(base + o_queue);
+  ItemBox* d_box = (ItemBox*)(base + o_box);
+  unsigned long long* d_last = (unsigned long long*)(base + o_last);
+  ItemDesc* d_desc = (ItemDesc*)(base + o_desc);
+  hipLaunchKernelGGL(k_put_job, dim3(1), dim3(64), 0, s, J, V, I, (int64_t)jt, d_tab);
+  zero_words(d_status, 16, s);  // the status word (the previous call's contents stay otherwise)
+  zero_words(d_queue, kQueueWords + 1, s);
+  if (I > 0) hipLaunchKernelGGL(k_init_items, dim3(blocks_for(I)), dim3(256), 0, s, d_box, d_last, I);
+  if (V > 0) {
+    PrepArgs pa;
+    pa.jobs = d_tab;
+    pa.job_vend = d_ends;
+    pa.n_jobs = 1;
+    pa.n_vertices = V;
+    pa.edges = dg->has_poly ? (PolyEdge*)(base + o_edges) : nullptr;
+    pa.px = lines ? (double2*)(base + o_px) : nullptr;
+    pa.vflags = lines ? (uint8_t*)(base + o_vflags) : nullptr;
+    pa.item_box = d_box;
+    pa.item_last = d_last;
+    hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
+  }
+  if (I > 0) {
+    DescArgs da;
+    std::memset(&da, 0, sizeof(da));
+    da.jobs = d_tab;
+    da.job_iend = d_ends + 1;
+    da.n_jobs = 1;
+    da.n_items = I;
+    da.item_box = d_box;
+    da.item_last = d_last;
+    da.desc = d_desc;
+    da.wide_mode = 1;
+    da.status = d_status;
+    hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
+  }
+  TileArgs ta;
+  std::memset(&ta, 0, sizeof(ta));
+  ta.jobs = d_tab;
+  ta.job_tend = d_ends + 2;
+  ta.n_jobs = 1;
+  ta.desc = d_desc;
+  ta.edges = dg->has_poly ? (const PolyEdge*)(base + o_edges) : nullptr;
+  ta.px = lines ? (const double2*)(base + o_px) : nullptr;
+  ta.vflags = lines ? (const uint8_t*)(base + o_vflags) : nullptr;
+  ta.status = d_status;
+  ta.t_lo = 0;
+  ta.t_hi = (uint32_t)jt;
+  ta.queue = d_queue;
+  ta.heavy_ctr = d_queue + kQueueWords;
+  launch_tile_dispatch(dtype, (int)fun, ta, lines, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  uint32_t flags = 0;
+  HIP_TRY(hipMemcpy(&flags, d_status, sizeof(flags), hipMemcpyDeviceToHost));
+  if (flags & 1u)
+    return rz::fail(RZ_CAPACITY_ERROR, "line coordinates beyond +-2^60 pixels are outside the supported range");
+  if (flags & 16u) return rz::fail(RZ_VALUE_ERROR, "layer index out of range");
+  return RZ_OK;
 }
 
 // --------------------------------------------------------------- serving --

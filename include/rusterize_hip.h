@@ -265,6 +265,31 @@ rz_status rz_rasterize_device_noalloc(rz_context* ctx, const rz_device_geometry*
                                       const void* background_host, int all_touched,
                                       rz_sum_mode sum_mode, void* out_dev);
 
+/* ---------------------------------------------------------- sparse output -- */
+
+/* encoding="sparse": replaces SparseArrayWriter (rust/src/encoding/writers.rs:
+ * 85-142) and SparseArray (rust/src/encoding/arrays.rs:61-209).  Every write
+ * of the call becomes a (row, col, value) triplet, in the reference's order:
+ * bands in order; in a band its geometries in order; a polygon's rows top to
+ * bottom and each row's columns left to right; a line's kept segments in
+ * order and each segment's cells in Bresenham order, the end cell last;
+ * points in order.  value is the geometry's field value (or the scalar).
+ * The triplets stay on the device until rz_sparse_free. */
+typedef struct rz_sparse rz_sparse;
+rz_status rz_rasterize_sparse(const rz_geometry* geoms, const rz_raster_info* info, rz_dtype dtype,
+                              const void* field, const void* scalar, const uint32_t* layer_of_geom,
+                              uint32_t n_layers, int all_touched, rz_sparse** out);
+/* Triplets of each band (n_layers values; SparseArray offsets). */
+rz_status rz_sparse_counts(const rz_sparse* sparse, uint64_t* per_band);
+/* Every triplet, band after band, into host arrays of the total count
+ * (rows and cols as u64, values of the call's dtype); any may be NULL. */
+rz_status rz_sparse_copy(const rz_sparse* sparse, uint64_t* rows, uint64_t* cols, void* values);
+/* SparseArray::build_array (arrays.rs:102-141): each band's triplets folded
+ * in order with pixel function fun, from the background, into out
+ * (host, [n_layers][nrows][ncols] of the call's dtype). */
+rz_status rz_sparse_to_dense(const rz_sparse* sparse, rz_fun fun, const void* background, void* out);
+rz_status rz_sparse_free(rz_sparse* sparse);
+
 /* ------------------------------------------------------------- batching -- */
 
 /* No reference counterpart (the reference rasterises one call at a time):

@@ -184,8 +184,6 @@ def _rusterize(geometry, raw_raster_info: dict, pypixel_fn: str, pydf=None, pyfi
         raise ValueError("Unknown pixel function")
     if pydtype not in _lib.DTYPES or pyencoding not in ("xarray", "numpy", "sparse"):
         raise PanicException("not implemented: Invalid dtype or encoding provided.")
-    if pyencoding == "sparse":
-        raise NotImplementedError("encoding='sparse' is not available on the device path")
 
     background = _background(pybackground, pydtype)
     field, scalar = None, None
@@ -211,6 +209,11 @@ def _rusterize(geometry, raw_raster_info: dict, pypixel_fn: str, pydf=None, pyfi
         import xarray  # noqa: F401
         raise NotImplementedError("encoding='xarray' output is not implemented")
 
+    if pyencoding == "sparse":
+        from .sparse import rasterize_sparse
+
+        return rasterize_sparse(geoms, info, pydtype, pypixel_fn, field, scalar, layer, n_layers, background,
+                                names, all_touched=bool(pytouched))
     return _lib.rasterize(geoms, info, pydtype, pypixel_fn, field, scalar, layer, n_layers, background,
                           all_touched=bool(pytouched))
 

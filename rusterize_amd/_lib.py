@@ -104,6 +104,12 @@ _SIGNATURES = [
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_rasterize_device_noalloc", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p,
                                             c_void_p, c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
+    ("rz_rasterize_sparse", c_int, [POINTER(RzGeometry), POINTER(RzRasterInfo), c_int, c_void_p, c_void_p,
+                                    POINTER(c_uint32), c_uint32, c_int, POINTER(c_void_p)]),
+    ("rz_sparse_counts", c_int, [c_void_p, POINTER(c_uint64)]),
+    ("rz_sparse_copy", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("rz_sparse_to_dense", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    ("rz_sparse_free", c_int, [c_void_p]),
     ("rz_batch_rasterize", c_int, [c_void_p, POINTER(RzJob), c_int64]),
     ("rz_batch_graph_create", c_int, [c_int, POINTER(RzJob), c_int64, c_double, POINTER(c_void_p)]),
     ("rz_batch_graph_launch", c_int, [c_void_p]),

@@ -242,6 +242,18 @@ __device__ inline ItemDesc load_desc(const RZ_CONST ItemDesc* p) {
   return d;
 }
 
+// The box (r0, r1, c0, c1) and kind | band word of a descriptor: its first
+// 32 bytes (two 16-byte loads), all a box test needs.
+struct BoxLayer {
+  int32_t r0, r1, c0, c1;
+  uint32_t kind_layer;
+};
+__device__ inline BoxLayer load_box(const RZ_CONST ItemDesc* p) {
+  const RZ_CONST u32x4* q = (const RZ_CONST u32x4*)p;
+  const u32x4 a = q[0], b = q[1];
+  return BoxLayer{(int32_t)a[0], (int32_t)a[1], (int32_t)a[2], (int32_t)a[3], b[3]};
+}
+
 // First index of XCD x's contiguous share of n.
 __device__ inline uint32_t xcd_unit_lo(uint32_t x, uint32_t n) {
   const uint32_t per = n / NUM_XCD, rem = n % NUM_XCD;

@@ -44,7 +44,23 @@ struct __align__(32) PolyEdge {
   int32_t ystart, yend;
 };
 
-enum : uint8_t { VF_HAS_NEXT = 1, VF_CLOSED = 2 };
+// Segment record of one line vertex (extract_line, rust/src/geo/edges.rs:
+// 112-134), 16 bytes, built by k_prep: the floored pixel coordinates of the
+// segment's end points (the cells Bresenham starts and stops at,
+// burners.rs:54-57), each biased by kLsBias so it fits 31 bits:
+//   x = ix0 + kLsBias, y = iy0 + kLsBias, z = ix1 + kLsBias,
+//   w = (iy1 + kLsBias) | kLsFlag when flagged.
+// In the per-vertex array the flag says the segment's ring is closed
+// (geo-types LineString::is_closed); in an edge band it says the segment's
+// end cell is written (the last kept edge of an open line, burners.rs:86-89).
+// x == kLsNone: the vertex starts no kept segment (last vertex of its ring, or
+// dropped by extract_line's bounding-box test); x == kLsFar: a kept segment
+// with a floored coordinate beyond +-2^30 (recomputed from the world
+// coordinates, in 64 or 128 bits).  A point vertex (extract_point,
+// edges.rs:79-88) holds its cell, x = column, y = row, or x == kLsNone
+// outside the raster.
+constexpr uint32_t kLsNone = 0xFFFFFFFFu, kLsFar = 0xFFFFFFFEu, kLsFlag = 0x80000000u;
+constexpr int64_t kLsBias = (int64_t)1 << 30;
 
 // Per-vertex record of a resident geometry set: bits 0-31 the local item,
 // 32-33 its kind, then whether the vertex has a next one in its ring, whether
@@ -125,8 +141,7 @@ struct PrepArgs {
   int n_jobs;
   int64_t n_vertices;       // batch total
   PolyEdge* edges;          // [n_vertices]
-  double2* px;              // [n_vertices] pixel coordinates (lines / points)
-  uint8_t* vflags;          // [n_vertices]
+  uint4* lseg;              // [n_vertices] segment / cell records of line and point vertices
   ItemBox* item_box;        // [n_items]
   unsigned long long* item_last;  // [n_items] lines: last kept segment + 1 (global; 0: none);
                                   // polygons: the sum over edges of the rows each crosses
@@ -141,9 +156,9 @@ struct __align__(64) ItemDesc {
   int32_t nv;               // vertex count
   uint32_t kind_layer;      // item kind | output band << 2
   uint64_t value;           // field value bits (dtype of the call)
-  uint64_t last;            // lines: last kept segment + 1 (0: none)
+  uint64_t last;            // lines: last kept segment + 1 (0: none; kItemFarBit masked off)
   uint32_t job;
-  uint32_t band;            // polygons with banded edges: first band entry + 1 (0: not banded)
+  uint32_t band;            // banded polygon or line item: first band entry + 1 (0: not banded)
   uint32_t flags;           // kItem* bits
   uint32_t wide;            // kItemWide: first word of the item's row masks / 16
 };
@@ -163,6 +178,8 @@ constexpr uint32_t kItemWide = 1u;
 constexpr uint32_t kItemOddRows = 2u;
 // k_prep marks such items in bit 63 of their item_last word.
 constexpr unsigned long long kItemOddBit = 1ull << 63;
+// ... and line items with a kLsFar segment in bit 62 (they are never banded).
+constexpr unsigned long long kItemFarBit = 1ull << 62;
 
 // Row masks of a kItemWide item: strip k (rows from (r0 / 16 + k) * 16) holds
 // 16 rows of nw = (c1 - 1) / 64 - c0 / 64 + 1 words each (word w covers
@@ -182,8 +199,39 @@ __host__ __device__ inline int32_t wide_strip_count(int32_t r0, int32_t r1) { re
 // edges of the strip starting at row Rw in
 // band_edges[band_off[e] .. band_off[e + 1]) (any order: the crossing
 // toggles and the maximum commute).
+// Line items with more than kLineBandMinSegs segments over two or more
+// strips (and no kLsFar segment) are banded the same way: a segment goes to
+// every strip the rows of its cells [min(iy0, iy1), max(iy0, iy1)] meet, as
+// a 16-byte segment record in band_segs[band_off[e] ..) whose flag marks the
+// end cell (any order: a line's writes all carry one value, so only their
+// number per pixel matters).
 constexpr int kBandMinEdges = 128;
+constexpr int kLineBandMinSegs = 32;
 constexpr int kBandRows = WAVE_ROWS;
+
+struct InitArgs {
+  ItemBox* box;             // [n_items]
+  unsigned long long* last; // [n_items]
+  int64_t n_items;
+  static constexpr int kRanges = 6;
+  uint32_t* zero[kRanges];  // word ranges cleared (n_zero[r] words from zero[r])
+  uint64_t n_zero[kRanges];
+  // Adds a range to clear; false when every slot is taken.
+  __host__ bool clear(void* p, uint64_t n) {
+    for (int r = 0; r < kRanges; ++r)
+      if (n_zero[r] == 0) {
+        zero[r] = (uint32_t*)p;
+        n_zero[r] = n;
+        return true;
+      }
+    return n == 0;
+  }
+  __host__ uint64_t words() const {
+    uint64_t w = 0;
+    for (int r = 0; r < kRanges; ++r) w += n_zero[r];
+    return w;
+  }
+};
 
 struct DescArgs {
   const JobDev* jobs;
@@ -246,9 +294,11 @@ struct BandArgs {
   const uint32_t* items;    // [*n_list] the banded items with assigned entries
   const uint32_t* n_list;
   const PolyEdge* edges;    // [n_vertices]
+  const uint4* lseg;        // [n_vertices] line segment records
   uint32_t* band_cnt;       // [n_bands] edges per band entry (count pass)
   const uint64_t* band_off; // [n_bands + 1] exclusive scan of band_cnt (scatter pass)
-  PolyEdge* band_edges;     // [edge_cap] (scatter pass)
+  PolyEdge* band_edges;     // [edge_cap] polygon entries (scatter pass), or null
+  uint4* band_segs;         // [edge_cap] line entries (scatter pass), or null
   uint64_t edge_cap;
   uint32_t* status;         // bit 3 set when the banded edges exceed edge_cap
 };
@@ -279,8 +329,9 @@ struct TileArgs {
   // binned jobs: pairs sorted by tile, tile_off[t]..tile_off[t+1]
   const uint32_t* tile_off;  // [n_tiles + 1], or null when no job is binned
   const ItemDesc* pair_desc;  // descriptors in pair order
-  const uint64_t* band_off;   // edge bands of banded polygon items
+  const uint64_t* band_off;   // edge bands of banded polygon and line items
   const PolyEdge* band_edges;
+  const uint4* band_segs;
   // heavy tiles: heavy_base[t] .. heavy_base[t + 1] index the tile's items in
   // the pre-pass masks (null: no heavy tiles); slot_words u64 per strip mask,
   // item j of tile t, strip s at slot heavy_base[t] * 4 + s * (items of t) + j
@@ -299,8 +350,7 @@ struct TileArgs {
   // scan jobs: every item of the job is tested against the tile
   const ItemDesc* desc;
   const PolyEdge* edges;
-  const double2* px;
-  const uint8_t* vflags;
+  const uint4* lseg;        // line / point vertex records (kLs*)
   unsigned long long* burns;  // [kBurnSlots] counters, or null
   uint32_t* status;         // error flags
 };
@@ -403,6 +453,23 @@ __host__ __device__ inline uint64_t row_coverage_word(const uint64_t* tog, uint3
   return word & (lim <= 0 ? 0ull : (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull)));
 }
 
+// floor(a / b) for 0 <= a, 0 < b.  When both fit 52 bits the f64 quotient
+// is used: a and b convert exactly and the correctly rounded quotient is
+// within 1/2 of a / b (which is below 2^52), so its truncation is the floor
+// or one off, and one integer check settles it.  The 64-bit integer
+// division it replaces is a long instruction sequence on the GPU.
+template <typename I>
+__host__ __device__ inline I div_floor_pos(I a, I b) {
+  if (a < ((I)1 << 52) && b < ((I)1 << 52)) {
+    I q = (I)(int64_t)((double)(int64_t)a / (double)(int64_t)b);
+    const I r = a - q * b;
+    if (r < 0) --q;
+    else if (r >= b) ++q;
+    return q;
+  }
+  return a / b;
+}
+
 // Cells of Standard::burn_line for one segment between integer cells
 // (rust/src/rasterization/burners.rs:53-84), excluding the end cell, that
 // fall in the window [X0,X1) x [Y0,Y1), in loop order.  Closed form of the
@@ -450,12 +517,12 @@ __host__ __device__ inline void walk_window(int64_t ix0, int64_t iy0, int64_t ix
   } else {
     if (jlo > 0) {  // j(k) >= jlo  <=>  k >= ceil((2n*jlo - n) / 2m)
       const I num = two_n * (I)jlo - n;
-      const I kmin = (num + two_m - 1) / two_m;
+      const I kmin = div_floor_pos<I>(num + two_m - 1, two_m);
       if (kmin > klo) klo = kmin;
     }
     if ((I)jhi < m) {  // j(k) <= jhi  <=>  k <= ceil((2n*(jhi+1) - n) / 2m) - 1
       const I num = two_n * (I)(jhi + 1) - n;
-      const I kmax = (num + two_m - 1) / two_m - 1;
+      const I kmax = div_floor_pos<I>(num + two_m - 1, two_m) - 1;
       if (kmax < khi) khi = kmax;
     }
   }
@@ -463,7 +530,7 @@ __host__ __device__ inline void walk_window(int64_t ix0, int64_t iy0, int64_t ix
   // j and its remainder at klo, then stepped: each iteration adds 2m to the
   // numerator and at most one 2n (m <= n).
   I numer = two_m * klo + n;
-  I j = numer / two_n;
+  I j = div_floor_pos<I>(numer, two_n);
   I rem = numer - j * two_n;
   for (I k = klo; k <= khi; ++k) {
     const int64_t u = u0 + su * (int64_t)k, w = w0 + sw * (int64_t)j;
@@ -473,6 +540,71 @@ __host__ __device__ inline void walk_window(int64_t ix0, int64_t iy0, int64_t ix
     if (rem >= two_n) {
       rem -= two_n;
       ++j;
+    }
+  }
+}
+
+// walk_window for end cells within +-2^30 and a window inside the int32
+// range (every tile strip): the same cells in the same order, the k range
+// set up in 64 bits and the loop stepped in 32 (its positions stay inside
+// the window; the remainder test rem + 2m >= 2n is rem >= 2n - 2m, so the
+// remainder, below 2n < 2^32, never overflows).
+template <typename F>
+__host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t ix1, int32_t iy1,
+                                              int32_t X0, int32_t X1, int32_t Y0, int32_t Y1, F&& emit) {
+  const int32_t dxs = ix1 - ix0, dys = iy1 - iy0;
+  const int64_t dx = dxs < 0 ? -(int64_t)dxs : dxs, D = dys < 0 ? -(int64_t)dys : dys;
+  const int32_t sx = ix0 < ix1 ? 1 : -1, sy = iy0 < iy1 ? 1 : -1;
+  const bool xmaj = dx >= D;
+  const int64_t n = xmaj ? dx : D;  // iterations (major steps)
+  const int64_t m = xmaj ? D : dx;  // minor steps in total
+  if (n == 0) return;
+  const int32_t u0 = xmaj ? ix0 : iy0, w0 = xmaj ? iy0 : ix0;
+  const int32_t su = xmaj ? sx : sy, sw = xmaj ? sy : sx;
+  const int64_t U0 = xmaj ? X0 : Y0, U1 = xmaj ? X1 : Y1;
+  const int64_t W0 = xmaj ? Y0 : X0, W1 = xmaj ? Y1 : X1;
+  int64_t klo, khi;
+  {
+    int64_t lo, hi;  // k in [lo, hi] from the major-axis window
+    if (su > 0) { lo = U0 - u0; hi = U1 - 1 - u0; }
+    else { lo = u0 - (U1 - 1); hi = u0 - U0; }
+    if (hi < 0 || lo >= n || lo > hi) return;
+    klo = lo < 0 ? 0 : lo;
+    khi = hi >= n ? n - 1 : hi;
+  }
+  int64_t jlo, jhi;  // minor index window
+  if (sw > 0) { jlo = W0 - w0; jhi = W1 - 1 - w0; }
+  else { jlo = w0 - (W1 - 1); jhi = w0 - W0; }
+  if (jhi < 0 || jlo > m || jlo > jhi) return;
+  const int64_t two_n = 2 * n, two_m = 2 * m;
+  if (m == 0) {
+    if (jlo > 0) return;
+  } else {
+    if (jlo > 0) {
+      const int64_t kmin = div_floor_pos<int64_t>(two_n * jlo - n + two_m - 1, two_m);
+      if (kmin > klo) klo = kmin;
+    }
+    if (jhi < m) {
+      const int64_t kmax = div_floor_pos<int64_t>(two_n * (jhi + 1) - n + two_m - 1, two_m) - 1;
+      if (kmax < khi) khi = kmax;
+    }
+  }
+  if (klo > khi) return;
+  const int64_t numer = two_m * klo + n;
+  const int64_t j = div_floor_pos<int64_t>(numer, two_n);
+  uint32_t rem = (uint32_t)(numer - j * two_n);
+  const uint32_t tm = (uint32_t)two_m, gap = (uint32_t)(two_n - two_m);
+  int32_t u = u0 + su * (int32_t)klo, w = w0 + sw * (int32_t)j;
+  const int32_t cnt = (int32_t)(khi - klo + 1);
+  for (int32_t i = 0; i < cnt; ++i) {
+    if (xmaj) emit(u, w);
+    else emit(w, u);
+    u += su;
+    if (rem >= gap) {
+      rem -= gap;
+      w += sw;
+    } else {
+      rem += tm;
     }
   }
 }

@@ -41,11 +41,24 @@ __global__ void k_vert_info(const int64_t* ring_voff, const int32_t* ring_item, 
     vert_info[v] = base | (v + 1 < rend ? VI_HAS_NEXT : 0ull) | (v == rbeg ? VI_RING_START : 0ull);
 }
 
-__global__ void k_init_items(ItemBox* box, unsigned long long* last, int64_t n_items) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_items) return;
-  box[i] = ItemBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-  last[i] = 0ull;
+// Start of a call: every item's box and statistic reset, and the words the
+// call counts into (status and burn counters, tile queues, the zero tails of
+// the per-item scans) cleared, in one launch.
+__global__ void k_init(InitArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n_items) {
+    a.box[i] = ItemBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    a.last[i] = 0ull;
+  }
+  uint64_t t = (uint64_t)i;
+#pragma unroll
+  for (int r = 0; r < InitArgs::kRanges; ++r) {
+    if (t < a.n_zero[r]) {
+      a.zero[r][t] = 0u;
+      break;
+    }
+    t -= a.n_zero[r];
+  }
 }
 
 // Clears n words (a kernel rather than a memset node, so captured batches
@@ -118,6 +131,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   uint64_t stat = 0;  // polygon edges: rows [ystart, yend) crossed; lines: kept segment + 1
   bool line = false;
   bool odd = false;   // polygon edges: some row may end with an odd crossing count
+  bool far = false;   // line segments: a floored coordinate beyond +-2^30
   auto vertex = [&](const RZ_CONST JobDev& J) {
     const int64_t lv = v - J.v_base;
     // world coordinates of this vertex and of the next one in the job, all
@@ -203,10 +217,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       a.edges[v] = e;
     } else if (kind == 1) {  // line rings: extract_line
       line = true;
-      a.px[v] = make_double2(x0, y0);
-      uint8_t f = has_next ? VF_HAS_NEXT : 0;
-      if (closed) f |= VF_CLOSED;  // geo-types LineString::is_closed
-      a.vflags[v] = f;
+      uint4 rec = make_uint4(kLsNone, 0u, 0u, 0u);
       if (has_next) {
         const double x1 = px_x(X1);
         const double y1 = px_y(Y1);
@@ -221,15 +232,27 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
           box.c1 = (int32_t)max((int64_t)0, min(fx1 + 1, (int64_t)J.ncols));
           box.r0 = (int32_t)max((int64_t)0, min(fy0, (int64_t)J.nrows));
           box.r1 = (int32_t)max((int64_t)0, min(fy1 + 1, (int64_t)J.nrows));
+          // the segment's end cells (burners.rs:54-57), biased into 31 bits
+          const int64_t ix0 = sat_isize(floor(x0)), iy0 = sat_isize(floor(y0));
+          const int64_t ix1 = sat_isize(floor(x1)), iy1 = sat_isize(floor(y1));
+          if (fx0 > -kLsBias && fx1 < kLsBias && fy0 > -kLsBias && fy1 < kLsBias) {
+            rec = make_uint4((uint32_t)(ix0 + kLsBias), (uint32_t)(iy0 + kLsBias), (uint32_t)(ix1 + kLsBias),
+                             (uint32_t)(iy1 + kLsBias) | (closed ? kLsFlag : 0u));
+          } else {
+            rec = make_uint4(kLsFar, 0u, 0u, closed ? kLsFlag : 0u);
+            far = true;
+          }
         }
       }
+      a.lseg[v] = rec;
     } else {  // points: extract_point
-      a.px[v] = make_double2(x0, y0);
-      a.vflags[v] = 0;
+      uint4 rec = make_uint4(kLsNone, 0u, 0u, 0u);
       if (x0 >= 0.0 && x0 < cols && y0 >= 0.0 && y0 < rows) {  // edges.rs:85
         const int32_t cx = (int32_t)x0, cy = (int32_t)y0;
         box = ItemBox{cy, cy + 1, cx, cx + 1};
+        rec = make_uint4((uint32_t)cx, (uint32_t)cy, 0u, 0u);
       }
+      a.lseg[v] = rec;
     }
   };
   // A wave inside one job (the common case) reads the job record with
@@ -244,6 +267,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   }
   reduce_box(item, box, stat, line, a.item_box, a.item_last, active);
   if (odd) atomicOr(&a.item_last[item], kItemOddBit);
+  if (far) atomicOr(&a.item_last[item], kItemFarBit);
 }
 
 // ------------------------------------------------------- item descriptors --
@@ -293,14 +317,16 @@ __global__ void k_item_desc(DescArgs a) {
   }
   d.value = v;
   const int32_t kind = J.item_kind[li];
-  const bool banded = a.band_n && kind == 0 && d.nv > kBandMinEdges && d.r0 < d.r1 &&
-                      (d.r1 - 1) / kBandRows > d.r0 / kBandRows;
-  d.last = kind == 1 ? a.item_last[i] : 0ull;
+  const uint64_t S_raw = a.item_last[i];
+  const bool tall = d.r0 < d.r1 && (d.r1 - 1) / kBandRows > d.r0 / kBandRows;
+  const bool banded = a.band_n && tall &&
+                      ((kind == 0 && d.nv > kBandMinEdges) ||
+                       (kind == 1 && d.nv > kLineBandMinSegs && !(S_raw & kItemFarBit)));
+  d.last = kind == 1 ? (S_raw & ~kItemFarBit) : 0ull;
   d.job = (uint32_t)j;
   d.band = 0;
   d.flags = 0;
   d.wide = 0;
-  const uint64_t S_raw = a.item_last[i];
   if (kind == 0 && (S_raw & kItemOddBit)) d.flags |= kItemOddRows;
   a.desc[i] = d;
   if (a.wide_units) {
@@ -398,12 +424,22 @@ __global__ __launch_bounds__(256) void k_band_count(BandArgs a) {
       const uint32_t L = min((uint32_t)kBandLds, ns - s0);
       for (uint32_t k = lane; k <= L; k += 64) cnt[k] = 0u;
       wave_sync();
-      // each lane's next edge is loaded while the current one is counted
+      // each lane's next edge is loaded while the current one is counted;
+      // its rows [ystart, yend), or for a line segment those of its cells
+      const bool line = (d.kind_layer & 3u) == 1u;  // wave-uniform
+      const uint4* sb = a.lseg + d.vbeg;
+      auto rows_of = [&](int32_t e) -> int2 {
+        if (!line) return *reinterpret_cast<const int2*>(&eb[e].ystart);
+        const uint4 r = sb[e];
+        if (r.x >= kLsFar) return make_int2(0, 0);
+        const int32_t y0 = (int32_t)((int64_t)r.y - kLsBias), y1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
+        return make_int2(max(min(y0, y1), 0), max(y0, y1) + 1);
+      };
       int2 ny = make_int2(0, 0);
-      if (lane < d.nv) ny = *reinterpret_cast<const int2*>(&eb[lane].ystart);  // ystart, yend
+      if (lane < d.nv) ny = rows_of(lane);
       for (int32_t e = lane; e < d.nv; e += 64) {
         const int2 yy = ny;
-        if (e + 64 < d.nv) ny = *reinterpret_cast<const int2*>(&eb[e + 64].ystart);
+        if (e + 64 < d.nv) ny = rows_of(e + 64);
         if (!(yy.x < yy.y)) continue;
         const int32_t b0 = max(max(yy.x / kBandRows, sr0) - sr0, (int32_t)s0);
         const int32_t b1 = min(min((yy.y - 1) / kBandRows, sr1) - sr0, (int32_t)(s0 + L - 1));
@@ -447,6 +483,32 @@ __global__ __launch_bounds__(256) void k_band_scatter(BandArgs a) {
       const uint32_t L = min((uint32_t)kBandLds, ns - s0);
       for (uint32_t k = lane; k < L; k += 64) pos[k] = (uint32_t)(a.band_off[d.band - 1u + s0 + k] - base);
       wave_sync();
+      if ((d.kind_layer & 3u) == 1u) {  // a line item (wave-uniform): segment records
+        const uint4* sb = a.lseg + d.vbeg;
+        uint4 nr = make_uint4(kLsNone, 0u, 0u, 0u);
+        if (lane < d.nv) nr = sb[lane];
+        for (int32_t e = lane; e < d.nv; e += 64) {
+          uint4 r = nr;
+          if (e + 64 < d.nv) nr = sb[e + 64];
+          if (r.x >= kLsFar) continue;
+          const int32_t y0 = (int32_t)((int64_t)r.y - kLsBias);
+          const int32_t y1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
+          const int32_t ys = max(min(y0, y1), 0), ye = max(y0, y1) + 1;
+          // the band record's flag: this is the last kept edge of an open
+          // line, whose end cell is written (burners.rs:86-89)
+          const bool end = (uint64_t)(d.vbeg + e + 1) == d.last && !(r.w & kLsFlag);
+          r.w = (r.w & ~kLsFlag) | (end ? kLsFlag : 0u);
+          const int32_t b0 = max(max(ys / kBandRows, sr0) - sr0, (int32_t)s0);
+          const int32_t b1 = min(min((ye - 1) / kBandRows, sr1) - sr0, (int32_t)(s0 + L - 1));
+          for (int32_t b = b0; b <= b1; ++b) {
+            const uint64_t slot = base + atomicAdd(&pos[b - s0], 1u);
+            if (slot < a.edge_cap) a.band_segs[slot] = r;
+            else atomicOr(a.status, 8u);
+          }
+        }
+        wave_sync();
+        continue;
+      }
       // the record as two 16-byte vectors (no struct copy through scratch);
       // each lane's next edge is loaded while the current one is copied
       uint4 nlo = make_uint4(0u, 0u, 0u, 0u), nhi = nlo;

@@ -79,11 +79,11 @@ struct rz_context {
   int device = 0;
   hipStream_t stream = nullptr;
   // workspace
-  Buf edges, px, vflags, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
+  Buf edges, lseg, item_box, item_last, desc, pair_desc, counts, offsets, keys_a, vals_a, keys_b,
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
       heavy_cnt, heavy_base, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
       sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
-      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7, wide_j;
+      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7, wide_j, band_segs;
   // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words
   uint64_t* host_misc = nullptr;
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
@@ -106,13 +106,13 @@ struct rz_context {
   uint64_t last_wide_units = 0, last_wide_words = 0;
   bool burns_pending = false;
   std::vector<Buf*> all() {
-    return {&edges,  &px,     &vflags, &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
+    return {&edges,  &lseg,   &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
             &vals_a, &keys_b, &vals_b, &tile_off, &temp,      &misc, &pair_desc, &jobtab,  &queue,
             &band_n, &band_base, &band_cnt, &band_off, &band_edges, &heavy_cnt, &heavy_base,
             &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
             &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
             &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items, &heavy_list, &scan_tmp7,
-            &wide_j};
+            &wide_j, &band_segs};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -158,6 +158,7 @@ struct rz_device_geometry {
   int64_t* item_vbeg = nullptr;
   bool has_poly = false, has_line = false, has_point = false;
   int64_t max_poly_nv = 0;  // most vertices of one polygon item
+  int64_t max_line_nv = 0;  // most vertices of one line item
 };
 
 // ------------------------------------------------ allocation-free workspace --
@@ -383,10 +384,10 @@ extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry*
   for (int64_t i = 0; i < g->n_items; ++i) {
     int k = g->item_kind[i];
     if (g->item_ring_offsets[i + 1] == g->item_ring_offsets[i]) continue;
-    if (k == RZ_ITEM_POLYGONS)
-      dg->max_poly_nv = std::max<int64_t>(dg->max_poly_nv,
-                                          g->ring_vertex_offsets[g->item_ring_offsets[i + 1]] -
-                                              g->ring_vertex_offsets[g->item_ring_offsets[i]]);
+    const int64_t nv = g->ring_vertex_offsets[g->item_ring_offsets[i + 1]] -
+                       g->ring_vertex_offsets[g->item_ring_offsets[i]];
+    if (k == RZ_ITEM_POLYGONS) dg->max_poly_nv = std::max<int64_t>(dg->max_poly_nv, nv);
+    if (k == RZ_ITEM_LINES) dg->max_line_nv = std::max<int64_t>(dg->max_line_nv, nv);
     dg->has_poly |= k == RZ_ITEM_POLYGONS;
     dg->has_line |= k == RZ_ITEM_LINES;
     dg->has_point |= k == RZ_ITEM_POINTS;
@@ -664,13 +665,16 @@ static uint64_t job_record(const rz_job& j, JobDev& J, int64_t v_base, int64_t i
   return jt;
 }
 
+// masks_only: the set-up and the wide pre-pass only, for every polygon item
+// and with no cap on the masks (the sparse encoding reads them); no bins, no
+// tile launches.
 static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
-                           const BatchCaps* caps = nullptr) {
+                           const BatchCaps* caps = nullptr, bool masks_only = false) {
   const bool capture = caps != nullptr;
   if (n_in <= 0) return RZ_OK;
   if (n_in >= (int64_t)1 << 24) return rz::fail(RZ_CAPACITY_ERROR, "more than 2^24 jobs in one batch");
   for (int64_t k = 0; k < n_in; ++k) {
-    rz_status st = check_job_common(jobs_in[k], true);
+    rz_status st = check_job_common(jobs_in[k], !masks_only);
     if (st) return st;
   }
   HIP_TRY(hipSetDevice(c->device));
@@ -696,6 +700,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     if (std::strcmp(wm, "never") == 0) wide_mode = 1;
     else if (std::strcmp(wm, "always") == 0) wide_mode = 2;
   }
+  if (masks_only) wide_mode = 2;
   auto has_lines = [](const rz_job& j) { return j.geometry->has_line || j.geometry->has_point; };
   std::vector<int64_t> order((size_t)n_in);
   for (int64_t k = 0; k < n_in; ++k) order[(size_t)k] = k;
@@ -718,6 +723,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   int64_t V = 0, I = 0;
   uint64_t T = 0, out_px = 0;
   bool any_binned = false, any_poly = false, any_lp = false, any_band = false;
+  bool any_poly_band = false, any_line_band = false;
   bool any_tw64 = false, any_tw256 = false;
   int max_tile_w = 64;
   for (size_t k = 0; k < n; ++k) {
@@ -735,7 +741,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     // item-tile tests; in a batch that cost is shared by every job.
     const uint64_t scan_limit = n_in == 1 ? ((uint64_t)1 << 22) : ((uint64_t)1 << 16);
     bool scan = ni <= 64 || (ni <= 2048 && ni * std::max<uint64_t>(jt, 1) <= scan_limit);
-    if (binning == 1) scan = ni == 0;
+    if (masks_only) scan = true;
+    else if (binning == 1) scan = ni == 0;
     else if (binning == -1) scan = true;
     J.scan = scan ? 1 : 0;
     any_binned |= !scan && ni > 0;
@@ -744,7 +751,9 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     any_tw256 |= TW == 256;
     max_tile_w = std::max(max_tile_w, TW);
     any_poly |= dg->has_poly;
-    any_band |= dg->has_poly && dg->max_poly_nv > kBandMinEdges;
+    any_poly_band |= dg->has_poly && dg->max_poly_nv > kBandMinEdges;
+    any_line_band |= dg->has_line && dg->max_line_nv > kLineBandMinSegs;
+    any_band = any_poly_band || any_line_band;
     any_lp |= lines;
     if (groups.empty() || groups.back().dtype != j.dtype || groups.back().fun != (int)j.fun ||
         groups.back().lines != lines)
@@ -773,8 +782,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   uint32_t* d_status = (uint32_t*)c->misc.p;
   unsigned long long* d_burns = (unsigned long long*)((char*)c->misc.p + 64);
   if (any_poly) HIP_TRY(ensure(c->stream, c->edges, (size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)));
-  if (any_lp) HIP_TRY(ensure(c->stream, c->px, (size_t)std::max<int64_t>(V, 1) * 16));
-  if (any_lp) HIP_TRY(ensure(c->stream, c->vflags, (size_t)std::max<int64_t>(V, 1)));
+  if (any_lp) HIP_TRY(ensure(c->stream, c->lseg, (size_t)std::max<int64_t>(V, 1) * sizeof(uint4)));
   HIP_TRY(ensure(c->stream, c->item_box, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemBox)));
   HIP_TRY(ensure(c->stream, c->item_last, (size_t)std::max<int64_t>(I, 1) * 8));
   HIP_TRY(ensure(c->stream, c->desc, (size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc)));
@@ -787,6 +795,12 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     HIP_TRY(ensure(c->stream, c->offsets, (size_t)(I + 1) * 8));
   }
   const bool any_wide = any_poly && wide_mode != 1 && I > 0;
+  // A small eager call whose only pre-pass is the wide one sizes the wide
+  // buffers for the most they can hold (2 bytes per output pixel, every
+  // unit at least 16 words) instead of reading the totals back: no host
+  // round trip.
+  const bool wide_fixed =
+      !capture && !masks_only && any_wide && !any_binned && !any_band && out_px <= ((uint64_t)1 << 26);
   if (any_wide) {
     HIP_TRY(ensure(c->stream, c->wide_units, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->stream, c->wide_words, (size_t)(I + 1) * 4));
@@ -825,12 +839,25 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     c->tab_dev = c->jobtab.p;
     c->tab_dev_bytes = tab_bytes;
   }
-  zero_words(c->misc.p, (c->profiling > 1 ? misc_bytes : 64) / 4, s);
-
-  // 1. edges / pixel coordinates / item boxes, then the item descriptors
-  if (I > 0)
-    hipLaunchKernelGGL(k_init_items, dim3(blocks_for(I)), dim3(256), 0, s, (ItemBox*)c->item_box.p,
-                       (unsigned long long*)c->item_last.p, I);
+  // 1. item boxes reset and the call's counters cleared (one launch), edges /
+  // pixel coordinates / item boxes, then the item descriptors
+  {
+    InitArgs ia;
+    std::memset(&ia, 0, sizeof(ia));
+    ia.box = (ItemBox*)c->item_box.p;
+    ia.last = (unsigned long long*)c->item_last.p;
+    ia.n_items = I;
+    ia.clear(c->misc.p, (c->profiling > 1 ? misc_bytes : 64) / 4);
+    ia.clear(c->queue.p, queue_words);
+    if (any_binned) ia.clear((uint32_t*)c->counts.p + I, 1);
+    if (any_band) ia.clear((uint32_t*)c->band_n.p + I, 1);
+    if (any_wide) {
+      ia.clear((uint32_t*)c->wide_units.p + I, 1);
+      ia.clear((uint32_t*)c->wide_words.p + I, 1);
+    }
+    const uint64_t nz = std::max<uint64_t>((uint64_t)I, ia.words());
+    hipLaunchKernelGGL(k_init, dim3(blocks_for((int64_t)nz)), dim3(256), 0, s, ia);
+  }
   if (V > 0) {
     PrepArgs pa;
     pa.jobs = d_jobs;
@@ -838,17 +865,10 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     pa.n_jobs = (int)n;
     pa.n_vertices = V;
     pa.edges = (PolyEdge*)c->edges.p;
-    pa.px = (double2*)c->px.p;
-    pa.vflags = (uint8_t*)c->vflags.p;
+    pa.lseg = (uint4*)c->lseg.p;
     pa.item_box = (ItemBox*)c->item_box.p;
     pa.item_last = (unsigned long long*)c->item_last.p;
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
-  }
-  if (any_binned) zero_words((uint32_t*)c->counts.p + I, 1, s);
-  if (any_band) zero_words((uint32_t*)c->band_n.p + I, 1, s);
-  if (any_wide) {
-    zero_words((uint32_t*)c->wide_units.p + I, 1, s);
-    zero_words((uint32_t*)c->wide_words.p + I, 1, s);
   }
   if (I > 0) {
     DescArgs da;
@@ -899,14 +919,14 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                                        (uint64_t)I + 1, (uint64_t*)c->scan_tmp5.p, s);
     exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->wide_words.p, (uint64_t*)c->wide_wbase.p,
                                        (uint64_t)I + 1, (uint64_t*)c->scan_tmp6.p, s);
-    if (!capture) {
+    if (!capture && !wide_fixed) {
       HIP_TRY(hipMemcpyAsync(&c->host_misc[4], (const uint64_t*)c->wide_ubase.p + I, 8,
                              hipMemcpyDeviceToHost, s));
       HIP_TRY(hipMemcpyAsync(&c->host_misc[5], (const uint64_t*)c->wide_wbase.p + I, 8,
                              hipMemcpyDeviceToHost, s));
     }
   }
-  if (!capture && (any_binned || any_band || any_wide)) HIP_TRY(hipStreamSynchronize(s));
+  if (!capture && (any_binned || any_band || (any_wide && !wide_fixed))) HIP_TRY(hipStreamSynchronize(s));
   // wide pre-pass: unit list, flags and mask bases (before the descriptors
   // are copied per pair).  The masks take at most 2 bytes per output pixel
   // (16 overlapping wide items per pixel at one bit); items past that stay
@@ -914,11 +934,15 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   uint64_t wide_units = 0, wide_words = 0;
   c->last_wide_units = c->last_wide_words = 0;
   if (any_wide) {
-    wide_units = capture ? caps->wide_units : c->host_misc[4];
-    wide_words = capture ? caps->wide_words : c->host_misc[5];
+    // (wide_fixed: the most the masks may hold, from the output size; the
+    // actual totals stay on the device, where k_wide_assign and k_wide_cover
+    // read them)
+    const uint64_t words_cap = std::max<uint64_t>(out_px / 4, (uint64_t)1 << 20);
+    wide_units = capture ? caps->wide_units : (wide_fixed ? words_cap / 16 : c->host_misc[4]);
+    wide_words = capture ? caps->wide_words : (wide_fixed ? words_cap : c->host_misc[5]);
     c->last_wide_units = wide_units;
     c->last_wide_words = wide_words;
-    wide_words = std::min<uint64_t>(wide_words, std::max<uint64_t>(out_px / 4, (uint64_t)1 << 20));
+    if (!masks_only) wide_words = std::min<uint64_t>(wide_words, words_cap);
     wide_words = std::min<uint64_t>(wide_words, (uint64_t)1 << 36);  // ItemDesc::wide holds words / 16
     if (wide_units > 0 && wide_words > 0) {
       HIP_TRY(ensure(c->stream, c->wide_item, wide_units * 4));
@@ -960,9 +984,11 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       bd.items = (const uint32_t*)c->band_items.p;
       bd.n_list = n_list;
       bd.edges = (const PolyEdge*)c->edges.p;
+      bd.lseg = (const uint4*)c->lseg.p;
       bd.band_cnt = (uint32_t*)c->band_cnt.p;
       bd.band_off = (const uint64_t*)c->band_off.p;
       bd.band_edges = nullptr;
+      bd.band_segs = nullptr;
       bd.edge_cap = 0;
       bd.status = d_status;
       const unsigned bgrid = (unsigned)std::min<uint64_t>((nb + 3) / 4, 16384);
@@ -979,12 +1005,21 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
         ne = c->host_misc[3];
       }
       c->last_band_edges = ne;
-      HIP_TRY(ensure(c->stream, c->band_edges, std::max<uint64_t>(ne, 1) * sizeof(PolyEdge)));
-      bd.band_edges = (PolyEdge*)c->band_edges.p;
+      // polygon and line entries share one index space; each kind's
+      // records go to an array of its own record size
+      if (any_poly_band) {
+        HIP_TRY(ensure(c->stream, c->band_edges, std::max<uint64_t>(ne, 1) * sizeof(PolyEdge)));
+        bd.band_edges = (PolyEdge*)c->band_edges.p;
+      }
+      if (any_line_band) {
+        HIP_TRY(ensure(c->stream, c->band_segs, std::max<uint64_t>(ne, 1) * sizeof(uint4)));
+        bd.band_segs = (uint4*)c->band_segs.p;
+      }
       bd.edge_cap = ne;
       if (ne > 0) hipLaunchKernelGGL(k_band_scatter, dim3(bgrid), dim3(256), 0, s, bd);
       ta.band_off = (const uint64_t*)c->band_off.p;
-      ta.band_edges = (const PolyEdge*)c->band_edges.p;
+      ta.band_edges = bd.band_edges;
+      ta.band_segs = bd.band_segs;
     }
   }
   // the wide pre-pass (after the bands it reads)
@@ -1013,7 +1048,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       wc.wide_j = w_j;
       wc.masks = masks;
       wc.slot_words = slot_words;
-      const uint32_t wgrid = (uint32_t)std::min<uint64_t>((wide_units + 3) / 4, 4096);
+      const uint32_t wgrid = (uint32_t)std::min<uint64_t>((wide_units + 3) / 4, wide_fixed ? 512 : 4096);
       hipLaunchKernelGGL(k_wide_cover, dim3(wgrid), dim3(256), 0, s, wc);
       ta.wide_masks = (const uint64_t*)c->wide_masks.p;
     }
@@ -1148,17 +1183,16 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     }
     launch_wide(nullptr, nullptr, nullptr, nullptr, 0);
   }
+  if (masks_only) return RZ_OK;
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[4], s));
 
-  // 3. tiles: one persistent work-queue launch per group
-  zero_words(c->queue.p, queue_words, s);
+  // 3. tiles: one persistent work-queue launch per group (queues cleared by k_init)
   ta.jobs = d_jobs;
   ta.job_tend = d_tend;
   ta.n_jobs = (int)n;
   ta.desc = (const ItemDesc*)c->desc.p;
   ta.edges = (const PolyEdge*)c->edges.p;
-  ta.px = (const double2*)c->px.p;
-  ta.vflags = (const uint8_t*)c->vflags.p;
+  ta.lseg = (const uint4*)c->lseg.p;
   ta.burns = c->profiling > 1 ? d_burns : nullptr;
   ta.status = d_status;
   for (size_t g = 0; g < groups.size(); ++g) {
@@ -1447,8 +1481,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
   const size_t o_last = take((size_t)std::max<int64_t>(I, 1) * 8);
   const size_t o_desc = take((size_t)std::max<int64_t>(I, 1) * sizeof(ItemDesc));
   const size_t o_edges = dg->has_poly ? take((size_t)std::max<int64_t>(V, 1) * sizeof(PolyEdge)) : 0;
-  const size_t o_px = lines ? take((size_t)std::max<int64_t>(V, 1) * 16) : 0;
-  const size_t o_vflags = lines ? take((size_t)std::max<int64_t>(V, 1)) : 0;
+  const size_t o_lseg = lines ? take((size_t)std::max<int64_t>(V, 1) * sizeof(uint4)) : 0;
   if (off > kScratchBytes)
     return rz::fail(RZ_CAPACITY_ERROR,
                     "the geometry needs " + std::to_string(off) +
@@ -1466,9 +1499,19 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
   unsigned long long* d_last = (unsigned long long*)(base + o_last);
   ItemDesc* d_desc = (ItemDesc*)(base + o_desc);
   hipLaunchKernelGGL(k_put_job, dim3(1), dim3(64), 0, s, J, V, I, (int64_t)jt, d_tab);
-  zero_words(d_status, 16, s);  // the status word (the previous call's contents stay otherwise)
-  zero_words(d_queue, kQueueWords + 1, s);
-  if (I > 0) hipLaunchKernelGGL(k_init_items, dim3(blocks_for(I)), dim3(256), 0, s, d_box, d_last, I);
+  {
+    // item boxes, the status word and the tile queue (the previous call's
+    // contents stay in the array otherwise)
+    InitArgs ia;
+    std::memset(&ia, 0, sizeof(ia));
+    ia.box = d_box;
+    ia.last = d_last;
+    ia.n_items = I;
+    ia.clear(d_status, 16);
+    ia.clear(d_queue, kQueueWords + 1);
+    const uint64_t nz = std::max<uint64_t>((uint64_t)I, ia.words());
+    hipLaunchKernelGGL(k_init, dim3(blocks_for((int64_t)nz)), dim3(256), 0, s, ia);
+  }
   if (V > 0) {
     PrepArgs pa;
     pa.jobs = d_tab;
@@ -1476,8 +1519,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
     pa.n_jobs = 1;
     pa.n_vertices = V;
     pa.edges = dg->has_poly ? (PolyEdge*)(base + o_edges) : nullptr;
-    pa.px = lines ? (double2*)(base + o_px) : nullptr;
-    pa.vflags = lines ? (uint8_t*)(base + o_vflags) : nullptr;
+    pa.lseg = lines ? (uint4*)(base + o_lseg) : nullptr;
     pa.item_box = d_box;
     pa.item_last = d_last;
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
@@ -1503,8 +1545,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
   ta.n_jobs = 1;
   ta.desc = d_desc;
   ta.edges = dg->has_poly ? (const PolyEdge*)(base + o_edges) : nullptr;
-  ta.px = lines ? (const double2*)(base + o_px) : nullptr;
-  ta.vflags = lines ? (const uint8_t*)(base + o_vflags) : nullptr;
+  ta.lseg = lines ? (const uint4*)(base + o_lseg) : nullptr;
   ta.status = d_status;
   ta.t_lo = 0;
   ta.t_hi = (uint32_t)jt;
@@ -1590,6 +1631,10 @@ extern "C" rz_status rz_rasterize(const rz_geometry* geoms, const rz_raster_info
   return st;
 }
 
+// ------------------------------------------------------- sparse encoding --
+
+#include "rz_sparse.hip"
+
 // ------------------------------------------------------------ test hooks --
 
 // Host evaluation of the Bresenham closed form used by the tile kernel
@@ -1605,9 +1650,13 @@ extern "C" int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, in
     }
     ++n;
   };
-  const int64_t lim = (int64_t)1 << 30;
+  const int64_t lim = (int64_t)1 << 30, wl = (int64_t)1 << 31;
   if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
-      iy1 > -lim && iy1 < lim)
+      iy1 > -lim && iy1 < lim && X0 > -wl && X1 < wl && Y0 > -wl && Y1 < wl && X0 <= X1 && Y0 <= Y1)
+    walk_window32((int32_t)ix0, (int32_t)iy0, (int32_t)ix1, (int32_t)iy1, (int32_t)X0, (int32_t)X1,
+                  (int32_t)Y0, (int32_t)Y1, emit);
+  else if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
+           iy1 > -lim && iy1 < lim)
     walk_window<int64_t>(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, emit);
   else
     walk_window<__int128>(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, emit);

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const In* in, uint6
     v += vals[k];
   }
   Out total;
-  Out run = block_exclusive_scan<Out>(v, s_tmp, &total) + bsum[blockIdx.x];
+  Out run = block_exclusive_scan<Out>(v, s_tmp, &total) + (bsum ? bsum[blockIdx.x] : (Out)0);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < n) out[base + k] = run;
@@ -92,11 +92,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const In* in, uint6
 inline uint64_t scan_blocks(uint64_t n) { return std::max<uint64_t>(1, (n + kScanBlock - 1) / kScanBlock); }
 
 // Exclusive prefix sums out[i] = in[0] + ... + in[i-1] over n elements.
-// tmp: scan_blocks(n) elements of Out.
+// tmp: scan_blocks(n) elements of Out.  Up to one block of inputs is one
+// launch (no block sums).
 template <typename In, typename Out>
 void exclusive_scan(const In* in, Out* out, uint64_t n, Out* tmp, hipStream_t s) {
   if (n == 0) return;
   const uint64_t nb = scan_blocks(n);
+  if (nb == 1) {
+    hipLaunchKernelGGL((k_scan_apply<In, Out>), dim3(1), dim3(kScanThreads), 0, s, in, n, (const Out*)nullptr, out);
+    return;
+  }
   hipLaunchKernelGGL((k_scan_reduce<In, Out>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, tmp);
   hipLaunchKernelGGL((k_scan_blocks<Out>), dim3(1), dim3(kScanThreads), 0, s, tmp, nb);
   hipLaunchKernelGGL((k_scan_apply<In, Out>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n,

@@ -16,18 +16,16 @@ namespace {
 
 using T = RZ_TILE_T;
 
-// Persistent grid for one k_tile instantiation: enough workgroups to fill
-// every CU at the kernel's occupancy (no more than the units need), a
+// Persistent grid for one tile-kernel instantiation: enough workgroups to
+// fill every CU at the kernel's occupancy (no more than the units need), a
 // multiple of the 8 XCDs.
-template <int FUN, bool LINES>
-uint32_t tile_grid(uint64_t n_tiles) {
-  static uint32_t cached = 0;
+template <typename K>
+uint32_t tile_grid(K kernel, uint32_t& cached, uint64_t n_tiles) {
   if (!cached) {
     int dev = 0, cus = 256, per_cu = 1;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile<T, FUN, LINES>, 256, 0) != hipSuccess ||
-        per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
       per_cu = 1;
     cached = (uint32_t)(cus * per_cu);
   }
@@ -51,13 +49,15 @@ template <int FUN>
 void launch_tile(TileArgs a, bool lines, hipStream_t s) {
   const uint64_t tiles = (uint64_t)(a.t_hi - a.t_lo);
   if (lines) {
-    const uint32_t g = tile_grid<FUN, true>(tiles);
+    static uint32_t cached = 0;
+    const uint32_t g = tile_grid(k_tile_lp<T, FUN>, cached, tiles);
     a.chunk = queue_chunk(tiles, g, a.chunk);
-    hipLaunchKernelGGL((k_tile<T, FUN, true>), dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_tile_lp<T, FUN>), dim3(g), dim3(256), 0, s, a);
   } else {
-    const uint32_t g = tile_grid<FUN, false>(tiles);
+    static uint32_t cached = 0;
+    const uint32_t g = tile_grid(k_tile<T, FUN>, cached, tiles);
     a.chunk = queue_chunk(tiles, g, a.chunk);
-    hipLaunchKernelGGL((k_tile<T, FUN, false>), dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_tile<T, FUN>), dim3(g), dim3(256), 0, s, a);
   }
 }
 

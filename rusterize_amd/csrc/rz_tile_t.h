@@ -7,111 +7,55 @@
 
 namespace rzk {
 
-// Persistent work-queue tile kernel, one launch per pixel-function group of
-// a batch (a single call is a batch of one job).
-//
-// Scheduling: a unit of work is one 16-row strip of one output tile, and
-// every wave works alone (no workgroup barrier).  Workgroup b runs on XCD
-// b % 8; each XCD owns a contiguous eighth of the launch's tiles, so
-// neighbouring tiles share cache lines and items in that XCD's L2.  Waves
-// take blocks of tiles from per-(XCD, strip) counters and, when their XCD's
-// range is used up, from the other XCDs' counters, so jobs of very uneven
-// size still end together.  Every wave leaves when all eight ranges of its
-// strip are used up.
-//
-// Per unit (16 rows x TW columns of the job's output): the background fill
-// of RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and every write
-// of every item meeting the strip, folded in geometry order in registers,
-// then written to HBM once.
-//
-// Two lane layouts per wave:
-//  * coverage: lane (rl = lane/4, q = lane%4) takes every 4th active edge of
-//    row rl; the four lanes of a row XOR-combine their crossing toggles and
-//    publish the row's coverage mask in LDS;
-//  * fold / store: lane l owns the 16-byte chunk l % LPR of rows
-//    l / LPR + k * RSTEP, so every store instruction writes whole tile rows
-//    (full 128-byte lines, contiguous across lanes).
-// The polygon path is branch-free per lane (predicated crossings and folds).
-// Waves per SIMD the register allocation targets: 7 for 4-byte outputs, 6
-// for 1- and 2-byte outputs (measured: the latency-bound strip loop gains more
-// from the extra wave than the few spilled registers cost), the compiler's
-// choice for 8-byte outputs and the line / point variants.
-template <typename T, bool LINES>
-struct TileOccupancy {
-  static constexpr int value = LINES ? 4 : (sizeof(T) == 4 ? 7 : (sizeof(T) == 8 ? 5 : 6));
-};
+// Persistent work queue of the tile kernels.  A unit of work is one 16-row
+// strip of one output tile, and wave w of every workgroup takes strip w, so
+// the waves working on one strip index walk neighbouring tiles together
+// (their stores fill neighbouring lines).  Workgroup b runs on XCD b % 8;
+// each XCD owns a contiguous eighth of the launch's tiles, so neighbouring
+// tiles share cache lines and items in that XCD's L2.  A wave's first tile
+// is its workgroup's slot in that range (no atomic); after the first `slots`
+// tiles, queue (x, w) hands out blocks of a.chunk consecutive tiles.  A grab
+// is issued one block ahead, so its latency overlaps the current block.  A
+// wave whose own XCD range is used up reads all eight counters of its strip
+// with one relaxed load (counters only grow: a range that reads as used up
+// is used up) and continues on the next XCD range with work left; jobs of
+// very uneven size still end together, and every wave leaves when all eight
+// ranges of its strip are used up.  The waves of a workgroup never
+// synchronise with each other.
+struct TileQueue {
+  uint32_t n_loc, xcd, slots, B, cur_x, pending, t_cur, t_end;
+  uint32_t* qbase;  // queue (x, w) at qbase + 4x * kQueueStride
+  int lane;
 
-template <typename T, int FUN, bool LINES>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile(TileArgs a) {
-  constexpr int TW = tile_w(sizeof(T));
-  constexpr int NWORD = TW / 64;                    // u64 words of a row mask
-  constexpr int PPC = 16 / (int)sizeof(T);          // pixels per 16-byte chunk
-  constexpr int LPR = TW / PPC;                     // lanes per row (store layout)
-  constexpr int RSTEP = 64 / LPR;                   // rows per store instruction
-  constexpr int RPL = WAVE_ROWS / RSTEP;            // rows per lane
-  constexpr int NPIX = RPL * PPC;                   // pixels per lane
-  // 1- and 2-byte outputs of the polygon variants keep their pixels packed,
-  // four (two) per 32-bit word, and fold them lane-wise (fold_word)
-  constexpr bool PACKED = !LINES && sizeof(T) <= 2;
-  constexpr int NW = PACKED ? NPIX * (int)sizeof(T) / 4 : 1;  // words per lane
-  constexpr int PPW = 4 / (int)sizeof(T);                     // pixels per word (packed)
-  constexpr int CNT_WORDS = LINES ? WAVE_ROWS * TW / 2 : 1;  // u16 counters, 2 per word
-  static_assert(NPIX <= 64, "dedup bits assume <= 64 pixels per lane");
-  __shared__ PolyEdge s_edge[4][64];
-  __shared__ uint64_t s_mask[4][2][WAVE_ROWS][NWORD];
-  __shared__ uint32_t s_cnt[4][CNT_WORDS];
-
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int rl = lane >> 2, q = lane & 3;            // coverage layout
-  const int srow = lane / LPR, ch = lane % LPR;      // store layout
-  const int ccol = ch * PPC;                         // first local column of the chunk
-  PolyEdge* stage = s_edge[wave];
-  uint32_t* cnt = s_cnt[wave];
-  if (LINES) {
-    for (int w = lane; w < CNT_WORDS; w += 64) cnt[w] = 0u;  // every apply re-zeroes
-    wave_sync();
+  __device__ TileQueue(const TileArgs& a, int wave, int lane_) : lane(lane_) {
+    n_loc = a.t_hi - a.t_lo;
+    xcd = blockIdx.x % NUM_XCD;
+    const uint32_t slot = blockIdx.x / NUM_XCD;
+    slots = gridDim.x / NUM_XCD;
+    B = a.chunk;
+    qbase = a.queue + (uint32_t)wave * kQueueStride;
+    cur_x = xcd;
+    pending = 0;  // lane 0: result of the grab in flight on cur_x
+    t_cur = xcd_unit_lo(xcd, n_loc) + slot;  // current block [t_cur, t_end)
+    t_end = t_cur + 1u;
   }
-  unsigned long long burns = 0;
-  int mbuf = 0;  // alternating row-mask buffer: one wave_sync per polygon item
-
-  // ---- work queue.  Wave w of every workgroup rasterises strip w (rows
-  // 16w..16w+15) of its tiles.  Each XCD owns a contiguous eighth of the
-  // launch's tiles.  A wave's first tile is its workgroup's slot in that
-  // range (no atomic); after the first `slots` tiles, queue (x, w) hands out
-  // blocks of a.chunk consecutive tiles, so the waves working on one strip
-  // index walk neighbouring tiles together (their stores fill neighbouring
-  // lines).  A grab is issued one block ahead, so its latency overlaps the
-  // current block.  A wave whose own XCD range is used up reads all eight
-  // counters of its strip with one relaxed load (counters only grow: a range
-  // that reads as used up is used up) and continues on the next XCD range
-  // with work left.
-  const uint32_t n_loc = a.t_hi - a.t_lo;
-  const uint32_t xcd = blockIdx.x % NUM_XCD, slot = blockIdx.x / NUM_XCD;
-  const uint32_t slots = gridDim.x / NUM_XCD;
-  const uint32_t B = a.chunk;
-  uint32_t* const qbase = a.queue + (uint32_t)wave * kQueueStride;  // queue (x, w) at qbase + 4x*stride
-  uint32_t cur_x = xcd;
-  uint32_t pending = 0;  // lane 0: result of the grab in flight on cur_x
-  auto issue = [&](uint32_t x) {
+  __device__ void issue(uint32_t x) {
     if (lane == 0) pending = atomicAdd(qbase + x * 4u * kQueueStride, 1u);
-  };
+  }
   // first tile handed out by queue x's grab number c
-  auto grab_tile = [&](uint32_t x, uint32_t c) -> uint64_t {
+  __device__ uint64_t grab_tile(uint32_t x, uint32_t c) const {
     return (uint64_t)xcd_unit_lo(x, n_loc) + slots + (uint64_t)c * B;
-  };
-  uint32_t t_cur = xcd_unit_lo(xcd, n_loc) + slot;  // current block [t_cur, t_end)
-  uint32_t t_end = t_cur + 1u;
-  auto take = [&](uint32_t x) -> bool {
+  }
+  __device__ bool take(uint32_t x) {
     const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)pending, 0);
     const uint64_t s = grab_tile(x, c), xhi = xcd_unit_lo(x + 1u, n_loc);
     if (s >= xhi) return false;
     t_cur = (uint32_t)s;
     t_end = (uint32_t)min(s + (uint64_t)B, xhi);
     return true;
-  };
+  }
   // (a grab on cur_x is always in flight when next_block runs)
-  auto next_block = [&]() -> bool {
+  __device__ bool next_block() {
     if (take(cur_x)) return true;
     for (;;) {
       bool avail = false;
@@ -130,16 +74,164 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
         return true;
       }
     }
-  };
-  const RZ_CONST JobDev* jobs_c = as_const(a.jobs);
-  const RZ_CONST int64_t* tend_c = as_const(a.job_tend);
+  }
+  // Whether the wave has a first tile (t_cur).
+  __device__ bool begin() {
+    bool have = t_cur < xcd_unit_lo(xcd + 1u, n_loc);
+    issue(xcd);  // the grab after the first tile (or the first block, if there is no first tile)
+    if (!have) {
+      have = next_block();
+      if (have) issue(cur_x);
+    }
+    return have;
+  }
+  // Local index of the current tile; moves to the next one.
+  __device__ uint32_t pop() { return t_cur++; }
+  // Whether there is a next tile (t_cur), taking the next block when needed.
+  __device__ bool advance() {
+    if (t_cur != t_end) return true;
+    const bool have = next_block();
+    if (have) issue(cur_x);
+    return have;
+  }
+};
+
+// Geometry of one unit: job, band, tile and strip window.
+struct StripUnit {
+  const RZ_CONST JobDev* J;
+  uint32_t layer;
+  int32_t nrows, ncols, C0, Rw, strip_r1, tile_c1;
+};
+
+__device__ inline StripUnit strip_unit(const TileArgs& a, uint32_t t, int32_t strip, int tw) {
+  StripUnit u;
+  u.J = &as_const(a.jobs)[find_job(as_const(a.job_tend), a.n_jobs, (int64_t)t)];
+  u.nrows = u.J->nrows;
+  u.ncols = u.J->ncols;
+  const uint32_t tj = t - u.J->tile_base;
+  u.layer = tj / u.J->tiles_per_layer;
+  const uint32_t tl = tj - u.layer * u.J->tiles_per_layer;
+  const int32_t ty = (int32_t)(tl / (uint32_t)u.J->ntx), tx = (int32_t)(tl % (uint32_t)u.J->ntx);
+  u.C0 = tx * tw;
+  u.Rw = ty * TILE_H + strip * WAVE_ROWS;  // first row of the strip
+  u.strip_r1 = min(u.Rw + WAVE_ROWS, u.nrows);
+  u.tile_c1 = min(u.C0 + tw, u.ncols);
+  return u;
+}
+
+// A polygon item's coverage mask word for this lane's row / quarter (lane
+// rl = lane / 4, q = lane % 4): precomputed (wide item, heavy tile) or
+// computed from its edges (banded: only the strip's band).
+template <int NWORD>
+__device__ inline uint64_t polygon_mask(const TileArgs& a, const ItemDesc& d, const StripUnit& u,
+                                        bool hb_ok, uint64_t hbase, uint32_t pb, uint32_t pe, uint32_t pi,
+                                        int32_t strip, PolyEdge* stage, int lane) {
+  const int rl = lane >> 2, q = lane & 3;
+  uint64_t cov = 0;
+  if (d.flags & kItemWide) {  // the wide pre-pass wrote the item's masks
+    const int32_t nw = wide_row_words(d.c0, d.c1), wi = (u.C0 >> 6) + q - (d.c0 >> 6);
+    if (q < NWORD && wi >= 0 && wi < nw)
+      cov = as_const(a.wide_masks)[(uint64_t)d.wide * 16u +
+                                   (uint64_t)(((u.Rw >> 4) - (d.r0 >> 4)) * 16 + rl) * (uint64_t)nw + (uint64_t)wi];
+  } else if (hb_ok) {  // heavy tile: the coverage pre-pass wrote this mask
+    if (q < NWORD)
+      cov = as_const(a.masks)[(hbase * 4u + (uint64_t)strip * (pe - pb) + (pi - pb)) * a.slot_words +
+                              (uint32_t)(rl * NWORD + q)];
+  } else {
+    // the item's edges, or for a banded item only those of this strip
+    const PolyEdge* ebase = a.edges + d.vbeg;
+    uint32_t n_e = (uint32_t)d.nv;
+    if (d.band) {
+      const uint32_t bi = d.band - 1u + (uint32_t)(u.Rw / kBandRows - d.r0 / kBandRows);
+      const uint64_t b0 = as_const(a.band_off)[bi];
+      ebase = a.band_edges + b0;
+      n_e = (uint32_t)(as_const(a.band_off)[bi + 1u] - b0);
+    }
+    cov = strip_coverage<NWORD>(ebase, n_e, u.Rw, u.strip_r1, u.C0, (double)u.ncols, stage, lane);
+  }
+  return cov;
+}
+
+// Writes a strip once: lane l's chunk of PPC pixels in rows
+// srow + k * RSTEP (k < RPL) starting at column C0 + ccol, from `chunk(k)`
+// (16 bytes); each store instruction covers RSTEP whole tile rows.
+template <typename T, int PPC, int RPL, int RSTEP, typename F>
+__device__ inline void store_strip(const StripUnit& u, int srow, int ccol, F&& chunk) {
+  const int32_t c = u.C0 + ccol;
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int32_t row = u.Rw + srow + k * RSTEP;
+    if (row < u.nrows) {
+      T* orow = (T*)u.J->out + ((uint64_t)u.layer * (uint64_t)u.nrows + (uint64_t)row) * (uint64_t)u.ncols;
+      const u32x4 vec = chunk(k);
+      if (c + PPC <= u.ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
+        __builtin_nontemporal_store(vec, (u32x4*)(orow + c));
+      } else {
+        T px[PPC];
+        __builtin_memcpy(px, &vec, 16);
+#pragma unroll
+        for (int j = 0; j < PPC; ++j)
+          if (c + j < u.ncols) orow[c + j] = px[j];
+      }
+    }
+  }
+}
+
+// Waves per SIMD the register allocation of k_tile targets: 7 for 4-byte
+// outputs, 6 for 1- and 2-byte outputs (measured: the latency-bound strip
+// loop gains more from the extra wave than the few spilled registers cost),
+// 5 for 8-byte outputs.
+template <typename T>
+struct TileOccupancy {
+  static constexpr int value = sizeof(T) == 4 ? 7 : (sizeof(T) == 8 ? 5 : 6);
+};
+
+// Tile kernel of polygon-only groups, one launch per pixel-function group of
+// a batch (a single call is a batch of one job).
+//
+// Per unit (16 rows x TW columns of the job's output): the background fill
+// of RasterInfo::build_raster (rust/src/geo/raster.rs:23-28) and every write
+// of every item meeting the strip, folded in geometry order in registers,
+// then written to HBM once.
+//
+// Two lane layouts per wave:
+//  * coverage: lane (rl = lane/4, q = lane%4) takes every 4th active edge of
+//    row rl; the four lanes of a row XOR-combine their crossing toggles and
+//    publish the row's coverage mask in LDS;
+//  * fold / store: lane l owns the 16-byte chunk l % LPR of rows
+//    l / LPR + k * RSTEP, so every store instruction writes whole tile rows
+//    (full 128-byte lines, contiguous across lanes).
+// The polygon path is branch-free per lane (predicated crossings and folds).
+template <typename T, int FUN>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(TileOccupancy<T>::value))) void k_tile(TileArgs a) {
+  constexpr int TW = tile_w(sizeof(T));
+  constexpr int NWORD = TW / 64;                    // u64 words of a row mask
+  constexpr int PPC = 16 / (int)sizeof(T);          // pixels per 16-byte chunk
+  constexpr int LPR = TW / PPC;                     // lanes per row (store layout)
+  constexpr int RSTEP = 64 / LPR;                   // rows per store instruction
+  constexpr int RPL = WAVE_ROWS / RSTEP;            // rows per lane
+  constexpr int NPIX = RPL * PPC;                   // pixels per lane
+  // 1- and 2-byte outputs keep their pixels packed, four (two) per 32-bit
+  // word, and fold them lane-wise (fold_word)
+  constexpr bool PACKED = sizeof(T) <= 2;
+  constexpr int NW = PACKED ? NPIX * (int)sizeof(T) / 4 : 1;  // words per lane
+  constexpr int PPW = 4 / (int)sizeof(T);                     // pixels per word (packed)
+  __shared__ PolyEdge s_edge[4][64];
+  __shared__ uint64_t s_mask[4][2][WAVE_ROWS][NWORD];
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rl = lane >> 2, q = lane & 3;            // coverage layout
+  const int srow = lane / LPR, ch = lane % LPR;      // store layout
+  const int ccol = ch * PPC;                         // first local column of the chunk
+  PolyEdge* stage = s_edge[wave];
+  unsigned long long burns = 0;
+  int mbuf = 0;  // alternating row-mask buffer: one wave_sync per polygon item
+
+  TileQueue tq(a, wave, lane);
   const RZ_CONST uint32_t* toff_c = as_const(a.tile_off);
   const RZ_CONST ItemDesc* pdesc_c = as_const(a.pair_desc);
   const RZ_CONST ItemDesc* desc_c = as_const(a.desc);
-  const RZ_CONST uint64_t* band_c = as_const(a.band_off);
-  const RZ_CONST uint64_t* heavy_c = as_const(a.heavy_base);
-  const RZ_CONST uint64_t* mask_c = as_const(a.masks);
-  const RZ_CONST uint64_t* wide_c = as_const(a.wide_masks);
   uint32_t pb = 0, pe = 0;
   auto fetch_range = [&](uint32_t tl) {
     if (a.tile_off) {
@@ -147,36 +239,21 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
       pe = toff_c[a.t_lo + tl + 1u];
     }
   };
-  bool have = t_cur < xcd_unit_lo(xcd + 1u, n_loc);
-  issue(xcd);  // the grab after the first tile (or the first block, if there is no first tile)
-  if (!have) {
-    have = next_block();
-    if (have) issue(cur_x);
-  }
-  if (have) fetch_range(t_cur);
+  bool have = tq.begin();
+  if (have) fetch_range(tq.t_cur);
   const int32_t strip = wave;
 
   while (have) {
-    const uint32_t t = a.t_lo + t_cur++;
-    const RZ_CONST JobDev& J = jobs_c[find_job(tend_c, a.n_jobs, (int64_t)t)];
-    const int32_t nrows = J.nrows, ncols = J.ncols;
-    const double ncols_f = (double)ncols, nrows_f = (double)nrows;
+    const uint32_t t = a.t_lo + tq.pop();
+    const StripUnit u = strip_unit(a, t, strip, TW);
+    const RZ_CONST JobDev& J = *u.J;
     const T bg = from_bits<T>(J.bg_bits);
-    const uint32_t tj = t - J.tile_base;
-    const uint32_t layer = tj / J.tiles_per_layer;
-    const uint32_t tl = tj - layer * J.tiles_per_layer;
-    const int32_t ty = (int32_t)(tl / (uint32_t)J.ntx), tx = (int32_t)(tl % (uint32_t)J.ntx);
-    const int32_t C0 = tx * TW;
-    const int32_t Rw = ty * TILE_H + strip * WAVE_ROWS;  // first row of the strip
-    const int32_t strip_r1 = min(Rw + WAVE_ROWS, nrows);
-    const int32_t tile_c1 = min(C0 + TW, ncols);
-    const bool dedup_job = J.dedup_lines != 0;
     // heavy tile (many items): every polygon mask comes from the pre-pass
     // (a tile is heavy iff its pair range holds at least heavy_min items, so
     // the slot base is only loaded for heavy tiles)
     uint64_t hbase = 0;
     const bool hb_ok = a.heavy_base && !J.scan && pe - pb >= a.heavy_min;
-    if (hb_ok) hbase = heavy_c[t];
+    if (hb_ok) hbase = as_const(a.heavy_base)[t];
 
     T acc[PACKED ? 1 : NPIX];  // acc[k * PPC + j]: row srow + k*RSTEP, column ccol + j
     uint32_t accw[NW];         // packed: pixel k * PPC + j in word (k * PPC + j) / PPW
@@ -191,135 +268,33 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
 
     auto process_item = [&](const ItemDesc d, uint32_t pi) {
       // wave-uniform skip: the item box misses this strip
-      if (d.r1 <= Rw || d.r0 >= strip_r1 || d.c1 <= C0 || d.c0 >= tile_c1) return;
-      const int32_t kind = (int32_t)(d.kind_layer & 3u);
+      if (d.r1 <= u.Rw || d.r0 >= u.strip_r1 || d.c1 <= u.C0 || d.c0 >= u.tile_c1) return;
       const T v = from_bits<T>(d.value);
       const uint64_t vbits = d.value;
-      const int64_t vb = d.vbeg, ve = d.vbeg + d.nv;
-      if (kind == 0) {
-        uint64_t cov = 0;
-        if (d.flags & kItemWide) {  // the wide pre-pass wrote the item's masks
-          const int32_t nw = wide_row_words(d.c0, d.c1), wi = (C0 >> 6) + q - (d.c0 >> 6);
-          if (q < NWORD && wi >= 0 && wi < nw)
-            cov = wide_c[(uint64_t)d.wide * 16u +
-                         (uint64_t)(((Rw >> 4) - (d.r0 >> 4)) * 16 + rl) * (uint64_t)nw + (uint64_t)wi];
-        } else if (hb_ok) {  // heavy tile: the coverage pre-pass wrote this mask
-          if (q < NWORD)
-            cov = mask_c[(hbase * 4u + (uint64_t)strip * (pe - pb) + (pi - pb)) * a.slot_words +
-                         (uint32_t)(rl * NWORD + q)];
-        } else {
-          // the item's edges, or for a banded item only those of this tile row
-          const PolyEdge* ebase = a.edges + vb;
-          uint32_t n_e = (uint32_t)d.nv;
-          if (d.band) {
-            const uint32_t bi = d.band - 1u + (uint32_t)(Rw / kBandRows - d.r0 / kBandRows);
-            const uint64_t b0 = band_c[bi];
-            ebase = a.band_edges + b0;
-            n_e = (uint32_t)(band_c[bi + 1u] - b0);
-          }
-          cov = strip_coverage<NWORD>(ebase, n_e, Rw, strip_r1, C0, ncols_f, stage, lane);
-        }
-        if (q < NWORD) s_mask[wave][mbuf][rl][q] = cov;
-        wave_sync();
+      const uint64_t cov = polygon_mask<NWORD>(a, d, u, hb_ok, hbase, pb, pe, pi, strip, stage, lane);
+      if (q < NWORD) s_mask[wave][mbuf][rl][q] = cov;
+      wave_sync();
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-          const uint64_t row_word = s_mask[wave][mbuf][srow + k * RSTEP][ccol >> 6];
-          const uint32_t bits = (uint32_t)(row_word >> (ccol & 63)) & ((1u << PPC) - 1u);
-          if (__any(bits != 0u)) {  // wave-uniform skip of uncovered row groups
-            burns += (unsigned long long)__popc(bits);
-            if constexpr (PACKED) {
-              const uint32_t vw = lane_splat<T>(vbits);
+      for (int k = 0; k < RPL; ++k) {
+        const uint64_t row_word = s_mask[wave][mbuf][srow + k * RSTEP][ccol >> 6];
+        const uint32_t bits = (uint32_t)(row_word >> (ccol & 63)) & ((1u << PPC) - 1u);
+        if (__any(bits != 0u)) {  // wave-uniform skip of uncovered row groups
+          burns += (unsigned long long)__popc(bits);
+          if constexpr (PACKED) {
+            const uint32_t vw = lane_splat<T>(vbits);
 #pragma unroll
-              for (int i = 0; i < PPC / PPW; ++i)
-                accw[k * (PPC / PPW) + i] = fold_word<T, FUN>(
-                    accw[k * (PPC / PPW) + i], vw, bgw,
-                    lane_mask<(int)sizeof(T)>((bits >> (i * PPW)) & ((1u << PPW) - 1u)));
-            } else {
-#pragma unroll
-              for (int j = 0; j < PPC; ++j)
-                acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
-            }
-          }
-        }
-        mbuf ^= 1;  // the next polygon writes the other buffer
-        return;
-      }
-      if (!LINES) return;  // the host only picks this variant without lines / points
-      // ---- lines and points: multiplicities in wave-private u16 counters
-      bool hit = false;  // this lane counted a cell of the strip
-      auto emit = [&](int64_t cx, int64_t cy) {
-        if (cx < C0 || cx >= tile_c1 || cy < Rw || cy >= strip_r1) return;
-        const uint32_t idx = (uint32_t)(cy - Rw) * TW + (uint32_t)(cx - C0);
-        atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
-        hit = true;
-      };
-      const bool dedup = kind == 1 && dedup_job;
-      uint64_t seen = 0u;  // PixelCache bits of this lane's pixels
-      for (int64_t cs = vb; cs < ve; cs += 65535) {  // <= 65535 writes per pixel per chunk
-        const int64_t ce = min(ve, cs + 65535);
-        for (int64_t s = cs + lane; s < ce; s += 64) {
-          const double2 p0 = a.px[s];
-          if (kind == 2) {  // point: extract_point + burn_point
-            if (p0.x >= 0.0 && p0.x < ncols_f && p0.y >= 0.0 && p0.y < nrows_f)
-              emit((int64_t)p0.x, (int64_t)p0.y);
-            continue;
-          }
-          const uint8_t f = a.vflags[s];
-          if (!(f & VF_HAS_NEXT)) continue;
-          const double2 p1 = a.px[s + 1];
-          if (!(fmin(p0.x, p1.x) < ncols_f && fmax(p0.x, p1.x) >= 0.0 &&
-                fmin(p0.y, p1.y) < nrows_f && fmax(p0.y, p1.y) >= 0.0))
-            continue;  // extract_line bbox filter, edges.rs:130
-          const int64_t ix0 = sat_isize(floor(p0.x)), iy0 = sat_isize(floor(p0.y));
-          const int64_t ix1 = sat_isize(floor(p1.x)), iy1 = sat_isize(floor(p1.y));
-          const int64_t lim = (int64_t)1 << 30, lim2 = (int64_t)1 << 60;
-          if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
-              iy1 > -lim && iy1 < lim) {
-            walk_window<int64_t>(ix0, iy0, ix1, iy1, C0, tile_c1, Rw, strip_r1, emit);
-          } else if (ix0 > -lim2 && ix0 < lim2 && ix1 > -lim2 && ix1 < lim2 && iy0 > -lim2 &&
-                     iy0 < lim2 && iy1 > -lim2 && iy1 < lim2) {
-            walk_window<__int128>(ix0, iy0, ix1, iy1, C0, tile_c1, Rw, strip_r1, emit);
+            for (int i = 0; i < PPC / PPW; ++i)
+              accw[k * (PPC / PPW) + i] = fold_word<T, FUN>(
+                  accw[k * (PPC / PPW) + i], vw, bgw,
+                  lane_mask<(int)sizeof(T)>((bits >> (i * PPW)) & ((1u << PPW) - 1u)));
           } else {
-            atomicOr(a.status, 1u);  // pixel coordinates beyond +-2^60: unsupported
-          }
-          // end cell of the last kept edge of an open line (burners.rs:86-89)
-          if ((uint64_t)(s + 1) == d.last && !(f & VF_CLOSED)) emit(ix1, iy1);
-        }
-        // nothing counted in this strip: the counters are still zero
-        if (!__any(hit)) continue;
-        hit = false;
-        wave_sync();
-        // apply the multiplicities of this lane's pixels, then clear them
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-          uint32_t* crow = cnt + ((srow + k * RSTEP) * TW + ccol) / 2;
-#pragma unroll
-          for (int j2 = 0; j2 < PPC / 2; ++j2) {
-            const uint32_t wv = crow[j2];
-            if (wv) {
-              crow[j2] = 0u;
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                uint32_t kk = (wv >> (16 * h)) & 0xFFFFu;
-                const int j = k * PPC + 2 * j2 + h;
-                if (kk && dedup) {  // LineWriter + PixelCache: one write per pixel per item
-                  const uint64_t bit = 1ull << j;
-                  if (seen & bit) kk = 0;
-                  else {
-                    seen |= bit;
-                    kk = 1;
-                  }
-                }
-                if (kk) {
-                  burns += kk;
-                  fold<T, FUN>(acc[j], v, bg, kk);
-                }
-              }
-            }
+            for (int j = 0; j < PPC; ++j)
+              acc[k * PPC + j] = fold_select<T, FUN>(acc[k * PPC + j], v, bg, (bits >> j) & 1u);
           }
         }
-        wave_sync();
       }
+      mbuf ^= 1;  // the next polygon writes the other buffer
     };
 
     if (!J.scan) {
@@ -328,16 +303,17 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
       const uint32_t pend = (hb_ok && a.heavy_split) ? pb : pe;
       for (uint32_t i = pb; i < pend; ++i) process_item(load_desc(pdesc_c + i), i);
     } else {
-      // scan job: test every item's box in order (small item counts)
+      // scan job: test every item's box in order (small item counts), 64
+      // boxes at a time, the next 64 loaded while this batch's hits are
+      // processed
       const int64_t ib = J.i_base, ie = J.i_base + J.ni;
+      BoxLayer nbx = ib + lane < ie ? load_box(desc_c + ib + lane) : BoxLayer{0, 0, 0, 0, 0u};
       for (int64_t base = ib; base < ie; base += 64) {
-        const int64_t it = base + lane;
-        bool hit = false;
-        if (it < ie) {
-          const ItemDesc d = load_desc(desc_c + it);
-          hit = (d.kind_layer >> 2) == layer && d.r0 < strip_r1 && d.r1 > Rw && d.c0 < tile_c1 &&
-                d.c1 > C0 && d.r0 < d.r1 && d.c0 < d.c1;
-        }
+        const BoxLayer bx = nbx;
+        const bool valid = base + lane < ie;
+        if (base + 64 + lane < ie) nbx = load_box(desc_c + base + 64 + lane);
+        const bool hit = valid && (bx.kind_layer >> 2) == u.layer && bx.r0 < u.strip_r1 && bx.r1 > u.Rw &&
+                         bx.c0 < u.tile_c1 && bx.c1 > u.C0 && bx.r0 < bx.r1 && bx.c0 < bx.c1;
         uint64_t m = __ballot(hit);
         while (m) {
           const int bit = __builtin_ctzll(m);
@@ -348,40 +324,390 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T, LINES>::value))) void k_tile
     }
 
     // take the next tile (and its item range) before this strip's stores
-    if (t_cur == t_end) {
-      have = next_block();
-      if (have) issue(cur_x);
-    }
-    if (have) fetch_range(t_cur);
+    have = tq.advance();
+    if (have) fetch_range(tq.t_cur);
 
-    // ---- write the strip once (background + burns): each store instruction
-    // covers RSTEP whole rows of the tile
-    const int32_t c = C0 + ccol;
+    // ---- write the strip once (background + burns)
+    store_strip<T, PPC, RPL, RSTEP>(u, srow, ccol, [&](int k) {
+      u32x4 vec;
+      if constexpr (PACKED) vec = u32x4{accw[k * 4], accw[k * 4 + 1], accw[k * 4 + 2], accw[k * 4 + 3]};
+      else __builtin_memcpy(&vec, &acc[k * PPC], 16);
+      return vec;
+    });
+  }
+  if (a.burns) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) burns += __shfl_xor(burns, off, 64);
+    if (lane == 0 && burns) atomicAdd(&a.burns[blockIdx.x % kBurnSlots], burns);
+  }
+}
+
+// Tile kernel of groups with lines or points (polygon items may be mixed
+// in), one launch per pixel-function group.  Same work queue, strips, item
+// order, background fill and write-once store as k_tile, but the strip's
+// accumulator lives in wave-private LDS (16 x TW pixels), so a line or
+// point write is folded where it lands instead of through a per-item pass
+// over the whole strip:
+//  * first / last / min / max / any are idempotent for repeated writes of one
+//    value, so a line's cells are folded directly by the lane that emits
+//    them; two lanes landing on one pixel in the same instruction store the
+//    same value, and later instructions see it (a wave's LDS accesses are
+//    performed in order);
+//  * sum and count depend on the number of writes: a line's cells are
+//    counted in 16-bit LDS counters; a cell's first count appends it to the
+//    touched list, and after the line the touched pixels are folded k times
+//    (one lane per touched pixel, no collisions).  With xres != yres
+//    (LineWriter + PixelCache, burn_geometry.rs:179-185, 202-208) a pixel is
+//    written once per line: its first touch (an LDS bit) folds it directly;
+//  * runs of one-vertex point items are folded in geometry order by one
+//    lane each, from descriptors loaded 64 at a time (the point's cell is its
+//    box, so its vertex is not read);
+//  * polygon items fold their coverage masks into the LDS pixels.
+// Segments come as 16-byte records with floored end points (k_prep); a line
+// item of many segments over several strips reads only its band of the strip.
+template <typename T, int FUN>
+__global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
+  constexpr int TW = tile_w(sizeof(T));
+  constexpr int NWORD = TW / 64;
+  constexpr int PPC = 16 / (int)sizeof(T);
+  constexpr int LPR = TW / PPC;
+  constexpr int RSTEP = 64 / LPR;
+  constexpr int RPL = WAVE_ROWS / RSTEP;
+  constexpr int NPIXS = WAVE_ROWS * TW;           // pixels of a strip
+  constexpr bool COUNTED = FUN == 0 || FUN == 5;  // sum, count: the number of writes matters
+  constexpr int CNT_WORDS = COUNTED ? NPIXS / 2 : 1;   // u16 counters, 2 per word
+  constexpr int PPL = NPIXS / 64;                      // pixels per lane in an apply (16 or 64)
+  __shared__ PolyEdge s_edge[4][64];
+  __shared__ uint64_t s_mask[4][2][WAVE_ROWS][NWORD];
+  __shared__ __align__(16) T s_acc[4][NPIXS];
+  __shared__ __align__(16) uint32_t s_cnt[4][CNT_WORDS];
+  __shared__ uint32_t s_mark[4][NPIXS / 32];
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rl = lane >> 2, q = lane & 3;
+  const int srow = lane / LPR, ch = lane % LPR;
+  const int ccol = ch * PPC;
+  PolyEdge* stage = s_edge[wave];
+  T* acc = s_acc[wave];
+  uint32_t* cnt = s_cnt[wave];
+  uint32_t* mark = s_mark[wave];
+  // (every apply leaves its counters and marks at zero)
+  if (COUNTED)
+    for (int w = lane; w < CNT_WORDS; w += 64) cnt[w] = 0u;
+  for (int w = lane; w < NPIXS / 32; w += 64) mark[w] = 0u;
+  unsigned long long burns = 0;
+  int mbuf = 0;
+
+  TileQueue tq(a, wave, lane);
+  const RZ_CONST uint32_t* toff_c = as_const(a.tile_off);
+  const RZ_CONST ItemDesc* pdesc_c = as_const(a.pair_desc);
+  const RZ_CONST ItemDesc* desc_c = as_const(a.desc);
+  uint32_t pb = 0, pe = 0;
+  auto fetch_range = [&](uint32_t tl) {
+    if (a.tile_off) {
+      pb = toff_c[a.t_lo + tl];
+      pe = toff_c[a.t_lo + tl + 1u];
+    }
+  };
+  bool have = tq.begin();
+  if (have) fetch_range(tq.t_cur);
+  const int32_t strip = wave;
+
+  while (have) {
+    const uint32_t t = a.t_lo + tq.pop();
+    const StripUnit u = strip_unit(a, t, strip, TW);
+    const RZ_CONST JobDev& J = *u.J;
+    const T bg = from_bits<T>(J.bg_bits);
+    uint64_t hbase = 0;
+    const bool hb_ok = a.heavy_base && !J.scan && pe - pb >= a.heavy_min;
+    if (hb_ok) hbase = as_const(a.heavy_base)[t];
+    const bool dedup_job = J.dedup_lines != 0;
+
+    // the strip starts as the background
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
-      const int32_t row = Rw + srow + k * RSTEP;
-      if (row < nrows) {
-        T* orow = (T*)J.out + ((uint64_t)layer * (uint64_t)nrows + (uint64_t)row) * (uint64_t)ncols;
-        if (c + PPC <= ncols && ((uintptr_t)(orow + c) & 15u) == 0) {
-          u32x4 vec;
-          if constexpr (PACKED) vec = u32x4{accw[k * 4], accw[k * 4 + 1], accw[k * 4 + 2], accw[k * 4 + 3]};
-          else __builtin_memcpy(&vec, &acc[k * PPC], 16);
-          __builtin_nontemporal_store(vec, (u32x4*)(orow + c));
-        } else {
+      T* p = acc + (srow + k * RSTEP) * TW + ccol;
 #pragma unroll
-          for (int j = 0; j < PPC; ++j)
-            if (c + j < ncols) {
-              if constexpr (PACKED) {
-                T x;
-                __builtin_memcpy(&x, reinterpret_cast<const char*>(accw) + (k * PPC + j) * sizeof(T), sizeof(T));
-                orow[c + j] = x;
-              } else {
-                orow[c + j] = acc[k * PPC + j];
+      for (int j = 0; j < PPC; ++j) p[j] = bg;
+    }
+    wave_sync();
+
+    // one write of value v at strip pixel idx, by this lane alone
+    auto fold1 = [&](uint32_t idx, T v) { fold<T, FUN>(acc[idx], v, bg, 1u); };
+
+    auto polygon = [&](const ItemDesc& d, uint32_t pi) {
+      const T v = from_bits<T>(d.value);
+      const uint64_t cov = polygon_mask<NWORD>(a, d, u, hb_ok, hbase, pb, pe, pi, strip, stage, lane);
+      if (q < NWORD) s_mask[wave][mbuf][rl][q] = cov;
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = srow + k * RSTEP;
+        const uint64_t row_word = s_mask[wave][mbuf][r][ccol >> 6];
+        const uint32_t bits = (uint32_t)(row_word >> (ccol & 63)) & ((1u << PPC) - 1u);
+        if (__any(bits != 0u)) {
+          burns += (unsigned long long)__popc(bits);
+          T* p = acc + r * TW + ccol;
+          T px[PPC];
+          __builtin_memcpy(px, p, 16);
+#pragma unroll
+          for (int j = 0; j < PPC; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (bits >> j) & 1u);
+          __builtin_memcpy(p, px, 16);
+        }
+      }
+      wave_sync();
+      mbuf ^= 1;
+    };
+
+    // a line or a multi-point item: its segment / cell records, lanes over
+    // records (chunks of 65,535 records keep a pixel's count in 16 bits)
+    // a line or a multi-point item: its records [recs, recs + n_r) (a line's
+    // band of this strip, or the item's vertices), lanes over records; `pre`
+    // is this lane's record of the first 64, already loaded (chunks of
+    // 65,535 records keep a pixel's count in 16 bits)
+    auto lines = [&](const ItemDesc& d, const uint4* recs, uint32_t n_r, bool banded, uint4 pre) {
+      const int32_t kind = (int32_t)(d.kind_layer & 3u);
+      const T v = from_bits<T>(d.value);
+      const bool dedup = kind == 1 && dedup_job;
+      // a write is a mark (one fold per pixel: idempotent functions, or a
+      // line's first write per pixel with xres != yres) or a count; both are
+      // LDS atomics whose result is not waited for
+      const bool marked = !COUNTED || dedup;
+      auto emit = [&](auto cx, auto cy) {
+        if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+        const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
+        if (marked) atomicOr(&mark[idx >> 5], 1u << (idx & 31u));
+        else atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
+      };
+      // end cell of the last kept edge of an open line (burners.rs:86-89)
+      auto end_cell = [&](const uint4& r, uint32_t e) {
+        return banded ? (r.w & kLsFlag) != 0
+                      : ((uint64_t)(d.vbeg + (int64_t)e + 1) == d.last && !(r.w & kLsFlag));
+      };
+      for (uint32_t cs = 0; cs < n_r; cs += 65535u) {
+        const uint32_t ce = min(n_r, cs + 65535u);
+        bool far = false;
+        for (uint32_t e = cs + (uint32_t)lane; e < ce; e += 64u) {
+          const uint4 r = e < 64u ? pre : recs[e];
+          if (r.x == kLsNone) continue;
+          if (kind == 2) {  // a vertex of a multi-point: its cell
+            emit((int32_t)r.x, (int32_t)r.y);
+            continue;
+          }
+          if (r.x == kLsFar) {
+            far = true;
+            continue;
+          }
+          const int32_t ix0 = (int32_t)((int64_t)r.x - kLsBias), iy0 = (int32_t)((int64_t)r.y - kLsBias);
+          const int32_t ix1 = (int32_t)((int64_t)r.z - kLsBias);
+          const int32_t iy1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
+          walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit);
+          if (end_cell(r, e)) emit(ix1, iy1);
+        }
+        if (__any(far)) {
+          // segments beyond +-2^30 pixels (never banded): the end points again
+          // from the world coordinates (edges.rs:119-122), walked in 128 bits
+          for (uint32_t e = cs + (uint32_t)lane; e < ce; e += 64u) {
+            const uint4 r = recs[e];
+            if (r.x != kLsFar) continue;
+            const int64_t lv = d.vbeg + (int64_t)e - J.v_base;
+            auto px_x = [&](double X) { return isnan(X) ? X : (X - J.xmin) / J.xres; };
+            auto px_y = [&](double Y) { return isnan(Y) ? Y : (J.ymax - Y) / J.yres; };
+            const int64_t ix0 = sat_isize(floor(px_x(J.xy[2 * lv]))), iy0 = sat_isize(floor(px_y(J.xy[2 * lv + 1])));
+            const int64_t ix1 = sat_isize(floor(px_x(J.xy[2 * lv + 2]))), iy1 = sat_isize(floor(px_y(J.xy[2 * lv + 3])));
+            const int64_t lim = (int64_t)1 << 60;
+            if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
+                iy1 > -lim && iy1 < lim)
+              walk_window<__int128>(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit);
+            else
+              atomicOr(a.status, 1u);  // pixel coordinates beyond +-2^60: unsupported
+            if (end_cell(r, e)) emit(ix1, iy1);
+          }
+        }
+        if (COUNTED && !marked) {
+          // each lane folds its PPL pixels k times each and clears their counters
+          wave_sync();
+#pragma unroll
+          for (int c0 = 0; c0 < PPL; c0 += 16) {
+            const int base = lane * PPL + c0;
+            uint32_t w[8];
+            __builtin_memcpy(w, cnt + base / 2, 32);
+            if (w[0] | w[1] | w[2] | w[3] | w[4] | w[5] | w[6] | w[7]) {
+              T px[16];
+              __builtin_memcpy(px, acc + base, sizeof(px));
+#pragma unroll
+              for (int j = 0; j < 16; ++j) {
+                const uint32_t k = (w[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+                burns += k;
+                fold<T, FUN>(px[j], v, bg, k);
+              }
+              __builtin_memcpy(acc + base, px, sizeof(px));
+              const uint32_t z[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+              __builtin_memcpy(cnt + base / 2, z, 32);
+            }
+          }
+          wave_sync();
+        }
+      }
+      if (marked) {
+        // each lane folds its marked pixels once and clears their marks
+        wave_sync();
+#pragma unroll
+        for (int c0 = 0; c0 < PPL; c0 += 16) {
+          const int base = lane * PPL + c0;
+          const uint32_t bits = (mark[base >> 5] >> (base & 31)) & 0xFFFFu;
+          if (bits) {
+            burns += (unsigned long long)__popc(bits);
+            T px[16];
+            __builtin_memcpy(px, acc + base, sizeof(px));
+#pragma unroll
+            for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (bits >> j) & 1u);
+            __builtin_memcpy(acc + base, px, sizeof(px));
+          }
+        }
+        wave_sync();
+        for (int w = lane; w < NPIXS / 32; w += 64) mark[w] = 0u;
+        wave_sync();
+      }
+    };
+
+    // the records of a line or multi-point item in this strip: base, count,
+    // whether they are the line's band (each lane may ask for its own item)
+    auto records = [&](const ItemDesc& d, const uint4*& recs, uint32_t& n_r) -> bool {
+      if ((d.kind_layer & 3u) == 1u && d.band) {
+        const uint32_t bi = d.band - 1u + (uint32_t)(u.Rw / kBandRows - d.r0 / kBandRows);
+        const uint64_t b0 = a.band_off[bi];
+        recs = a.band_segs + b0;
+        n_r = (uint32_t)(a.band_off[bi + 1u] - b0);
+        return true;
+      }
+      recs = a.lseg + d.vbeg;
+      n_r = (uint32_t)d.nv;
+      return false;
+    };
+    auto first_chunk = [&](const uint4* recs, uint32_t n_r) {
+      return (uint32_t)lane < n_r ? recs[lane] : make_uint4(kLsNone, 0u, 0u, 0u);
+    };
+
+    auto process_item = [&](const ItemDesc& d, uint32_t pi) {
+      if (d.r1 <= u.Rw || d.r0 >= u.strip_r1 || d.c1 <= u.C0 || d.c0 >= u.tile_c1) return;
+      if ((d.kind_layer & 3u) == 0u) {
+        polygon(d, pi);
+      } else {
+        const uint4* recs;
+        uint32_t n_r;
+        const bool banded = records(d, recs, n_r);
+        lines(d, recs, n_r, banded, first_chunk(recs, n_r));
+      }
+    };
+
+    if (!J.scan) {
+      // descriptors 64 at a time, one per lane: which items meet the strip,
+      // the cells and values of one-vertex points (their box), and the
+      // record range of every line (its band bounds, loaded by all lanes at
+      // once).  Items are then taken in order: a run of points is folded by
+      // one lane from the lanes' copies, a polygon or line is reloaded
+      // uniformly and processed by the wave, and the next line's first
+      // records are loaded while the current one is walked.
+      for (uint32_t i0 = pb; i0 < pe; i0 += 64u) {
+        const uint32_t nb = min(64u, pe - i0);
+        bool pt = false, ln = false, po = false, bnd = false;
+        uint32_t pidx = 0, rn = 0;
+        uint64_t pv = 0;
+        const uint4* rp = nullptr;
+        if ((uint32_t)lane < nb) {
+          const ItemDesc dl = load_desc(pdesc_c + i0 + (uint32_t)lane);
+          const uint32_t kind = dl.kind_layer & 3u;
+          const bool vis = !(dl.r1 <= u.Rw || dl.r0 >= u.strip_r1 || dl.c1 <= u.C0 || dl.c0 >= u.tile_c1);
+          pt = vis && kind == 2u && dl.nv == 1;
+          po = vis && kind == 0u;
+          ln = vis && !pt && !po;
+          if (pt) {
+            pidx = (uint32_t)(dl.r0 - u.Rw) * TW + (uint32_t)(dl.c0 - u.C0);
+            pv = dl.value;
+          }
+          if (ln) bnd = records(dl, rp, rn);
+        }
+        const uint64_t ptm = __ballot(pt), lnm = __ballot(ln);
+        uint64_t todo = ptm | lnm | __ballot(po);
+        // the first records of line item q (a uniform lane index)
+        auto prefetch = [&](int q) {
+          const uint4* r = (const uint4*)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)((uint64_t)rp >> 32), q) << 32) |
+                                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)rp, q));
+          return first_chunk(r, (uint32_t)__builtin_amdgcn_readlane((int)rn, q));
+        };
+        uint64_t lrest = lnm;
+        uint4 pre = make_uint4(kLsNone, 0u, 0u, 0u);
+        if (lrest) pre = prefetch(__builtin_ctzll(lrest));
+        while (todo) {
+          const int p = __builtin_ctzll(todo);
+          if ((ptm >> p) & 1ull) {
+            // the run of points up to the next line or polygon, in order
+            const uint64_t other = todo & ~ptm & (~0ull << p);
+            const uint64_t run = todo & ptm & (other ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull);
+            uint64_t m = run;
+            while (m) {
+              const int b = __builtin_ctzll(m);
+              m &= m - 1;
+              const uint32_t idx = (uint32_t)__builtin_amdgcn_readlane((int)pidx, b);
+              const uint64_t vb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), b) << 32) |
+                                  (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pv, b);
+              if (lane == 0) {
+                ++burns;
+                fold1(idx, from_bits<T>(vb));
               }
             }
+            todo &= ~run;
+            continue;
+          }
+          todo &= todo - 1;
+          const ItemDesc d = load_desc(pdesc_c + i0 + (uint32_t)p);
+          if ((lnm >> p) & 1ull) {
+            const uint4 cur = pre;
+            lrest &= lrest - 1;
+            if (lrest) pre = prefetch(__builtin_ctzll(lrest));
+            const uint4* recs = (const uint4*)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)((uint64_t)rp >> 32), p) << 32) |
+                                               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)rp, p));
+            const uint32_t n_r = (uint32_t)__builtin_amdgcn_readlane((int)rn, p);
+            const bool banded = __builtin_amdgcn_readlane((int)bnd, p) != 0;
+            lines(d, recs, n_r, banded, cur);
+          } else {
+            polygon(d, i0 + (uint32_t)p);
+          }
+        }
+      }
+    } else {
+      // scan job: test every item's box in order (small item counts), 64
+      // boxes at a time, the next 64 loaded while this batch's hits are
+      // processed
+      const int64_t ib = J.i_base, ie = J.i_base + J.ni;
+      BoxLayer nbx = ib + lane < ie ? load_box(desc_c + ib + lane) : BoxLayer{0, 0, 0, 0, 0u};
+      for (int64_t base = ib; base < ie; base += 64) {
+        const BoxLayer bx = nbx;
+        const bool valid = base + lane < ie;
+        if (base + 64 + lane < ie) nbx = load_box(desc_c + base + 64 + lane);
+        const bool hit = valid && (bx.kind_layer >> 2) == u.layer && bx.r0 < u.strip_r1 && bx.r1 > u.Rw &&
+                         bx.c0 < u.tile_c1 && bx.c1 > u.C0 && bx.r0 < bx.r1 && bx.c0 < bx.c1;
+        uint64_t m = __ballot(hit);
+        while (m) {
+          const int bit = __builtin_ctzll(m);
+          m &= m - 1;
+          process_item(load_desc(desc_c + base + bit), 0u);
         }
       }
     }
+
+    have = tq.advance();
+    if (have) fetch_range(tq.t_cur);
+
+    // ---- write the strip once from LDS
+    wave_sync();
+    store_strip<T, PPC, RPL, RSTEP>(u, srow, ccol, [&](int k) {
+      u32x4 vec;
+      __builtin_memcpy(&vec, acc + (srow + k * RSTEP) * TW + ccol, 16);
+      return vec;
+    });
+    wave_sync();  // the next strip's background overwrites these pixels
   }
   if (a.burns) {
 #pragma unroll

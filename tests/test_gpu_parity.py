@@ -213,6 +213,65 @@ def test_config3_reduced():
         assert_same(got, ref, f"config3 {fun}")
 
 
+def zigzag(x0, y0, w, h, n, seed):
+    rng = np.random.default_rng(seed)
+    xs = x0 + rng.uniform(0, w, n)
+    ys = y0 + rng.uniform(0, h, n)
+    return "LINESTRING (" + ", ".join(f"{float(x)!r} {float(y)!r}" for x, y in zip(xs, ys)) + ")"
+
+
+LINE_PATH_CACHE = {}
+
+
+def line_path_geometry():
+    """Geometry for the line / point tile kernel's paths: zigzag lines whose
+    cells in one strip overflow the touched-pixel list (the dense apply),
+    banded lines over many strips, long runs of one-vertex points with
+    repeated cells, multi-points with repeated vertices, and polygons in
+    between (each of them in geometry order)."""
+    if "g" not in LINE_PATH_CACHE:
+        rng = np.random.default_rng(21)
+        wkts = []
+        wkts.append(zigzag(3.0, 100.0, 60.0, 14.0, 600, 1))      # one strip, > 256 cells
+        wkts.append(zigzag(70.0, 10.0, 120.0, 110.0, 400, 2))    # banded over many strips
+        wkts += [f"POINT ({x:.3f} {y:.3f})" for x, y in rng.uniform(0, 64, (300, 2)) // 4 * 4 + 0.5]
+        wkts.append(sq(20, 20, 50, 45))
+        wkts += [f"POINT ({x:.3f} {y:.3f})" for x, y in rng.uniform(0, 64, (200, 2)) // 4 * 4 + 0.5]
+        wkts.append("MULTIPOINT ((5.5 5.5), (5.5 5.5), (6.5 5.5), (5.5 5.5))")
+        wkts.append(zigzag(0.0, 0.0, 250.0, 120.0, 3000, 3))    # banded, heavy overlap
+        wkts.append(zigzag(100.0, 60.0, 30.0, 6.0, 800, 4))
+        LINE_PATH_CACHE["g"] = Geometry.from_wkt(wkts)
+    return LINE_PATH_CACHE["g"]
+
+
+@pytest.mark.parametrize("binning", [None, "always", "never"])
+def test_line_point_kernel_paths(binning, monkeypatch):
+    if binning:
+        monkeypatch.setenv("RUSTERIZE_BINNING", binning)
+    g = line_path_geometry()
+    for dtype, fun, bg in [("float64", "sum", float("nan")), ("int32", "count", 0), ("uint8", "sum", 0),
+                           ("float32", "first", float("nan")), ("int16", "last", 0), ("float64", "max", 0),
+                           ("uint16", "min", 0), ("int64", "any", 0)]:
+        field = field_for(dtype, g.n_geoms, seed=31)
+        field[::7] = 0
+        for xres in (1.0, 0.75):  # 0.75: non-square pixels, one write per pixel per line
+            info = _lib.raster_info(None, resolution=(xres, 1.0), extent=(0, 0, 250, 125))
+            got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+            assert_same(got, ref, f"line paths {binning} {dtype} {fun} xres={xres}")
+
+
+def test_far_line_segments():
+    # a segment whose floored end point lies beyond 2^30 pixels is walked in
+    # 128 bits from its world coordinates; its line is never banded
+    wkts = ["LINESTRING (-1100000000.25 40.5, 240.5 44.5, 100.5 100.5)", "POINT (3.5 80.5)",
+            zigzag(0.0, 0.0, 250.0, 120.0, 200, 5)]
+    g = Geometry.from_wkt(wkts)
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 250, 125))
+    field = field_for("float64", g.n_geoms, seed=41)
+    got, ref = run_both(g, info, "float64", "sum", field=field, bg=float("nan"))
+    assert_same(got, ref, "far segment")
+
+
 @pytest.mark.parametrize("binning", [None, "always", "never"])
 def test_long_rings_edge_bands(binning, monkeypatch):
     # many-vertex rings spanning many tile rows take the banded-edge path

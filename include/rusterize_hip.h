@@ -51,11 +51,16 @@ typedef enum rz_fun {
   RZ_SUM = 0, RZ_FIRST = 1, RZ_LAST = 2, RZ_MIN = 3, RZ_MAX = 4, RZ_COUNT = 5, RZ_ANY = 6
 } rz_fun;
 
-/* How `sum` is reduced on the device.  ORDERED folds every pixel in
- * geometry order and is bit-identical to the reference.  UNORDERED permits
- * any reduction order (float sums may then differ from the reference by
- * rounding, within 1e-12 relative); this build serves it with the ordered
- * fold, which costs the same here, so both modes are bit-exact. */
+/* How `sum` is reduced on the device.  ORDERED folds every pixel's writes
+ * in geometry order and is bit-identical to the reference
+ * (pixel_functions.rs:56-65).  UNORDERED adds the writes of lines and
+ * points with LDS atomics as they land, in no fixed order, with no per-line
+ * pass; a pixel is the sum of its writes' values (the background where it
+ * has none).  For float outputs that differs from the reference by rounding
+ * only: |got - ref| <= 8 n eps sum|v| for n writes of values v (eps of the
+ * output type), provided no value is NaN and no partial sum equals the
+ * background (the reference restarts a pixel there).  Integer outputs and
+ * polygon-only launches always take the ordered fold. */
 typedef enum rz_sum_mode { RZ_SUM_ORDERED = 0, RZ_SUM_UNORDERED = 1 } rz_sum_mode;
 
 /* Item kinds of the flattened geometry model (see rz_geometry). */
@@ -306,6 +311,8 @@ typedef struct rz_job {
   uint32_t n_layers;
   uint64_t background_bits;
   void* out_dev;             /* [n_layers][nrows][ncols] of dtype, disjoint per job */
+  rz_sum_mode sum_mode;      /* fun = sum with a float dtype: ordered (exact) or unordered */
+  int all_touched;           /* AllTouched strategy (burners.rs:94-247; not in captured graphs) */
 } rz_job;
 
 /* Rasterises n_jobs independent jobs (any mix of sizes, dtypes and pixel
@@ -404,12 +411,19 @@ rz_status rz_context_last_counters(rz_context* ctx, uint64_t* burns, uint64_t* p
  * returns the count and stores up to cap cells.
  * rz_debug_row_coverage: which of the columns [C0, C0+width) the spans of
  * burn_polygon (burners.rs:302-316) cover for the crossings xs[0..n) of one
- * row; width is 64, 128 or 256. */
+ * row; width is 64, 128 or 256.
+ * rz_debug_walk_all_touched: the cells AllTouchedBase::burn_line
+ * (burners.rs:115-246) writes for one edge in pixel coordinates on an
+ * ncols x nrows raster, in order, restricted to [X0,X1) x [Y0,Y1); returns
+ * the count (or -1 past the walk's step limit) and stores up to cap cells. */
 int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, int64_t iy1, int64_t X0,
                             int64_t X1, int64_t Y0, int64_t Y1, int64_t* xs, int64_t* ys,
                             int64_t cap);
 int rz_debug_row_coverage(const double* xs, int64_t n, int64_t ncols, int32_t C0, int32_t width,
                           uint8_t* out);
+int64_t rz_debug_walk_all_touched(double x0, double y0, double x1, double y1, int64_t ncols, int64_t nrows,
+                                  int64_t X0, int64_t X1, int64_t Y0, int64_t Y1, int64_t* xs, int64_t* ys,
+                                  int64_t cap);
 
 #ifdef __cplusplus
 }

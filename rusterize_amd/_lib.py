@@ -44,7 +44,7 @@ class RzJob(ctypes.Structure):
     _fields_ = [
         ("geometry", c_void_p), ("info", RzRasterInfo), ("dtype", c_int), ("fun", c_int),
         ("field_dev", c_void_p), ("scalar_bits", c_uint64), ("layer_dev", c_void_p), ("n_layers", c_uint32),
-        ("background_bits", c_uint64), ("out_dev", c_void_p),
+        ("background_bits", c_uint64), ("out_dev", c_void_p), ("sum_mode", c_int), ("all_touched", c_int),
     ]
 
 
@@ -131,6 +131,8 @@ _SIGNATURES = [
     ("rz_context_last_counters", c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     ("rz_debug_walk_cells", c_int64, [c_int64] * 8 + [POINTER(c_int64), POINTER(c_int64), c_int64]),
     ("rz_debug_row_coverage", c_int, [POINTER(c_double), c_int64, c_int64, c_int32, c_int32, POINTER(c_uint8)]),
+    ("rz_debug_walk_all_touched", c_int64, [c_double] * 4 + [c_int64] * 6 + [POINTER(c_int64), POINTER(c_int64),
+                                                                            c_int64]),
 ]
 
 EXPORTED = tuple(name for name, _, _ in _SIGNATURES)

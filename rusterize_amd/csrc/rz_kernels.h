@@ -95,6 +95,8 @@ struct JobDev {
   int32_t fun;               // pixel function (rz_fun)
   int32_t dedup_lines;       // xres != yres (burn_geometry.rs:179, 202)
   int32_t scan;              // 1: no bins, the tile kernel tests every item of the job
+  int32_t sum_unordered;     // fun = sum, float output: lines and points add with LDS atomics
+  int32_t all_touched;       // AllTouched strategy: lines walked cell by cell, polygons with their outline
   const void* field;         // [n_geoms] of the batch dtype, or null
   uint64_t scalar_bits, bg_bits;
   const uint32_t* layer_of_geom;  // may be null
@@ -145,7 +147,69 @@ struct PrepArgs {
   ItemBox* item_box;        // [n_items]
   unsigned long long* item_last;  // [n_items] lines: last kept segment + 1 (global; 0: none);
                                   // polygons: the sum over edges of the rows each crosses
+  unsigned long long* item_win;   // [8 n_items] all-touched sum / count polygons (or null): the f64
+                                  // box of their kept outline segments (min x, min y, max x, max y as
+                                  // ordered keys), then that of all their vertices (a NaN widens it
+                                  // to infinity)
 };
+
+// Order-preserving u64 key of an f64 (for atomic min / max): the sign bit
+// flipped for positives, every bit for negatives.
+__host__ __device__ inline uint64_t f64_key(double v) {
+  uint64_t b;
+  __builtin_memcpy(&b, &v, 8);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double f64_from_key(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  double v;
+  __builtin_memcpy(&v, &b, 8);
+  return v;
+}
+
+// The PixelCache of an all-touched polygon (rust/src/rasterization/
+// pixel_cache.rs:16-36) in integers: origin (xmin as isize, ymin as isize),
+// width and length in pixels, capacity width * length (usize arithmetic,
+// wrapping), from the f64 box of its kept outline segments.
+struct CacheWin {
+  int64_t x0, y0;
+  uint64_t width, cap;
+};
+__host__ __device__ inline uint64_t as_usize(double v) {
+  if (!(v > 0.0)) return 0;
+  if (v >= 18446744073709551616.0) return ~0ull;
+  return (uint64_t)v;
+}
+__host__ __device__ inline CacheWin cache_window(double xmin, double ymin, double xmax, double ymax) {
+  CacheWin w;
+  w.width = as_usize(floor(xmax) - floor(xmin)) + 1u;
+  const uint64_t length = as_usize(floor(ymax) - floor(ymin)) + 1u;
+  auto isz = [](double v) -> int64_t {
+    if (v != v) return 0;
+    if (v <= -9223372036854775808.0) return INT64_MIN;
+    if (v >= 9223372036854775808.0) return INT64_MAX;
+    return (int64_t)v;
+  };
+  w.x0 = isz(xmin);
+  w.y0 = isz(ymin);
+  w.cap = w.width * length;
+  return w;
+}
+// unravel_index (pixel_cache.rs:39-44), wrapping
+__host__ __device__ inline uint64_t cache_index(const CacheWin& w, int64_t x, int64_t y) {
+  return ((uint64_t)y - (uint64_t)w.y0) * w.width + ((uint64_t)x - (uint64_t)w.x0);
+}
+// the cache window of item i from its outline box keys (item_win); false when
+// no kept segment gave both axes a finite extent (an empty cache: it contains
+// no pixel of the raster)
+__host__ __device__ inline bool item_cache(const unsigned long long* win, int64_t i, CacheWin& cw) {
+  const unsigned long long* w = win + 8 * i;
+  const double xmin = f64_from_key(w[0]), ymin = f64_from_key(w[1]);
+  const double xmax = f64_from_key(w[2]), ymax = f64_from_key(w[3]);
+  if (!(xmin <= xmax && ymin <= ymax)) return false;
+  cw = cache_window(xmin, ymin, xmax, ymax);
+  return true;
+}
 
 // Per-call item descriptor (64 B, one scalar load): everything the tile
 // kernel needs about an item, with the owning geometry's field value and
@@ -176,6 +240,10 @@ constexpr uint32_t kItemWide = 1u;
 // parity is already zero from there on.  (Rings are closed, so their
 // crossing counts are even otherwise.)
 constexpr uint32_t kItemOddRows = 2u;
+// kItemAtCache: an all-touched sum / count polygon whose fill may reach
+// pixels outside its PixelCache window; those pixels look the cache up in
+// global memory (at_bits), where k_at_cache built it.
+constexpr uint32_t kItemAtCache = 4u;
 // k_prep marks such items in bit 63 of their item_last word.
 constexpr unsigned long long kItemOddBit = 1ull << 63;
 // ... and line items with a kLsFar segment in bit 62 (they are never banded).
@@ -212,8 +280,9 @@ constexpr int kBandRows = WAVE_ROWS;
 struct InitArgs {
   ItemBox* box;             // [n_items]
   unsigned long long* last; // [n_items]
+  unsigned long long* win;  // [8 n_items] or null: reset to two empty f64 boxes
   int64_t n_items;
-  static constexpr int kRanges = 6;
+  static constexpr int kRanges = 7;
   uint32_t* zero[kRanges];  // word ranges cleared (n_zero[r] words from zero[r])
   uint64_t n_zero[kRanges];
   // Adds a range to clear; false when every slot is taken.
@@ -247,6 +316,21 @@ struct DescArgs {
   uint32_t* wide_units;     // [n_items] k_wide_cover units of a wide item (0: not wide), or null
   uint32_t* wide_words;     // [n_items] row-mask words of a wide item
   uint32_t* status;         // bit 4 set when a geometry's layer index is >= the job's n_layers
+  const unsigned long long* item_win;  // all-touched polygons' outline boxes (null: none)
+  uint32_t* at_words;       // [n_items] words of the PixelCache an all-touched dedup polygon needs
+                            // in global memory (its fill reaches beyond the cache box), or null
+};
+
+// k_at_cache: the PixelCache bits of the kItemAtCache items, one wave per
+// item, lanes over its outline segments.
+struct AtCacheArgs {
+  const JobDev* jobs;
+  const ItemDesc* desc;     // [n_items]
+  int64_t n_items;
+  const unsigned long long* item_win;
+  const uint64_t* at_base;  // [n_items + 1] exclusive scan of at_words
+  unsigned long long* at_bits;
+  uint32_t* status;         // bit 64: a walk did not end
 };
 
 // Wide items within the capacities get their flag, mask base and unit list.
@@ -351,6 +435,12 @@ struct TileArgs {
   const ItemDesc* desc;
   const PolyEdge* edges;
   const uint4* lseg;        // line / point vertex records (kLs*)
+  // all-touched polygons with sum / count: outline boxes (PixelCache
+  // windows) and, for those whose fill reaches beyond them, the PixelCache
+  // bits in global memory (at_bits + at_base[item] words)
+  const unsigned long long* item_win;
+  const uint64_t* at_base;
+  const unsigned long long* at_bits;
   unsigned long long* burns;  // [kBurnSlots] counters, or null
   uint32_t* status;         // error flags
 };
@@ -607,6 +697,136 @@ __host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t 
       rem += tm;
     }
   }
+}
+
+// `v as isize` for f64 (saturating, NaN -> 0), for host and device.
+__host__ __device__ inline int64_t as_isize(double v) {
+  if (v != v) return 0;
+  if (v <= -9223372036854775808.0) return INT64_MIN;
+  if (v >= 9223372036854775808.0) return INT64_MAX;
+  return (int64_t)v;
+}
+
+// AllTouchedBase::burn_line for one edge (rust/src/rasterization/
+// burners.rs:115-246, the GDAL all-touched walker), in its order, with its
+// f64 state stepped exactly as written (every expression compiled without
+// contraction): cells inside the raster and inside the window
+// [X0, X1) x [Y0, Y1) are emitted.  The walk moves left to right and, when
+// sloped, monotonically down (up), so it stops once its column passes X1 or
+// its row passes the window in that direction.  Each step reaches the next column or row
+// boundary (or moves 1e-9 off one), so a walk takes at most about
+// 2 (ncols + nrows) steps; past 4 (ncols + nrows) + 64 it stops and returns
+// false (the reference would not end).
+template <typename F>
+__host__ __device__ inline bool walk_all_touched(double dfx, double dfy, double dfxe, double dfye, int64_t ncols,
+                                                 int64_t nrows, int64_t X0, int64_t X1, int64_t Y0, int64_t Y1,
+                                                 F&& emit) {
+  const double EPSILON_INTERSECT = 1e-4, TOLERANCE = 1e-9;
+  const double ncols_f = (double)ncols, nrows_f = (double)nrows;
+  double t;
+  if (dfx > dfxe) {  // proceed left to right
+    t = dfx; dfx = dfxe; dfxe = t;
+    t = dfy; dfy = dfye; dfye = t;
+  }
+  if (fabs(dfx - dfxe) < 0.01) {  // vertical (burners.rs:124-148)
+    if (dfye < dfy) { t = dfy; dfy = dfye; dfye = t; }
+    const int64_t ix = as_isize(floor(dfxe));
+    int64_t iy = as_isize(floor(dfy));
+    int64_t iy_end = as_isize(floor(dfye - EPSILON_INTERSECT));
+    if (ix < 0 || ix >= ncols || ix < X0 || ix >= X1) return true;
+    iy = iy < 0 ? 0 : iy;
+    iy_end = iy_end > nrows - 1 ? nrows - 1 : iy_end;
+    iy = iy < Y0 ? Y0 : iy;
+    iy_end = iy_end > Y1 - 1 ? Y1 - 1 : iy_end;
+    for (int64_t y = iy; y <= iy_end; ++y) emit(ix, y);
+    return true;
+  }
+  if (fabs(dfy - dfye) < 0.01) {  // horizontal (burners.rs:150-176)
+    if (dfxe < dfx) { t = dfx; dfx = dfxe; dfxe = t; }
+    int64_t ix = as_isize(floor(dfx));
+    const int64_t iy = as_isize(floor(dfy));
+    int64_t ix_end = as_isize(floor(dfxe - EPSILON_INTERSECT));
+    if (iy < 0 || iy >= nrows || iy < Y0 || iy >= Y1) return true;
+    ix = ix < 0 ? 0 : ix;
+    ix_end = ix_end > ncols - 1 ? ncols - 1 : ix_end;
+    ix = ix < X0 ? X0 : ix;
+    ix_end = ix_end > X1 - 1 ? X1 - 1 : ix_end;
+    for (int64_t x = ix; x <= ix_end; ++x) emit(x, iy);
+    return true;
+  }
+  // sloped (burners.rs:178-245)
+  const double slope = (dfye - dfy) / (dfxe - dfx);
+  const double inv_slope = 1.0 / slope;
+  if (dfx < 0.0) {  // clip along x
+    dfy += (0.0 - dfx) * slope;
+    dfx = 0.0;
+  }
+  if (dfxe > ncols_f) {
+    dfye += (ncols_f - dfxe) * slope;
+    dfxe = ncols_f;
+  }
+  if (dfy < 0.0) {  // clip along y
+    dfx += (0.0 - dfy) * inv_slope;
+    dfy = 0.0;
+  } else if (dfy > nrows_f) {
+    dfx += (nrows_f - dfy) * inv_slope;
+    dfy = nrows_f;
+  }
+  if (dfye < 0.0) {
+    dfxe += (0.0 - dfye) * inv_slope;
+  } else if (dfye > nrows_f) {
+    dfxe += (nrows_f - dfye) * inv_slope;
+  }
+  // f64::clamp (a NaN stays NaN)
+  if (dfx < 0.0) dfx = 0.0;
+  if (dfx > ncols_f) dfx = ncols_f;
+  if (dfxe < 0.0) dfxe = 0.0;
+  if (dfxe > ncols_f) dfxe = ncols_f;
+  int64_t steps = 4 * (ncols + nrows) + 64;
+  while (dfx >= 0.0 && dfx < dfxe) {
+    const int64_t ix = as_isize(floor(dfx));
+    const int64_t iy = as_isize(floor(dfy));
+    if (ix >= X1) return true;  // every later cell lies further right
+    if (slope > 0.0 ? iy >= Y1 : iy < Y0) return true;  // ... and further down (up)
+    if (ix >= 0 && ix < ncols && iy >= 0 && iy < nrows && ix >= X0 && iy >= Y0 && iy < Y1) emit(ix, iy);
+    if (--steps < 0) return false;
+    double sx = floor(dfx + 1.0) - dfx;
+    double sy = sx * slope;
+    if (as_isize(floor(dfy + sy)) == iy) {
+      dfx += sx;
+      dfy += sy;
+    } else if (slope < 0.0) {
+      sy = (double)iy - dfy;
+      if (sy > -TOLERANCE) sy = -TOLERANCE;
+      sx = sy / slope;
+      dfx += sx;
+      dfy += sy;
+    } else {
+      sy = (double)(int64_t)((uint64_t)iy + 1u) - dfy;
+      if (sy < TOLERANCE) sy = TOLERANCE;
+      sx = sy / slope;
+      dfx += sx;
+      dfy += sy;
+    }
+  }
+  return true;
+}
+
+// The outline segment from local vertex lv to the next vertex of its ring
+// in pixel coordinates, when extract_line keeps it (edges.rs:112-134: its
+// box meets the raster).
+template <typename JobRef>
+__device__ inline bool kept_segment(const JobRef& J, int64_t lv, double& x0, double& y0, double& x1,
+                                    double& y1) {
+  if (!(J.vert_info[lv] & VI_HAS_NEXT)) return false;
+  auto px_x = [&](double X) { return isnan(X) ? X : (X - J.xmin) / J.xres; };
+  auto px_y = [&](double Y) { return isnan(Y) ? Y : (J.ymax - Y) / J.yres; };
+  x0 = px_x(J.xy[2 * lv]);
+  y0 = px_y(J.xy[2 * lv + 1]);
+  x1 = px_x(J.xy[2 * lv + 2]);
+  y1 = px_y(J.xy[2 * lv + 3]);
+  return fmin(x0, x1) < (double)J.ncols && fmax(x0, x1) >= 0.0 && fmin(y0, y1) < (double)J.nrows &&
+         fmax(y0, y1) >= 0.0;
 }
 
 }  // namespace rzk

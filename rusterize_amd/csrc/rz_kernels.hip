@@ -49,6 +49,16 @@ __global__ void k_init(InitArgs a) {
   if (i < a.n_items) {
     a.box[i] = ItemBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
     a.last[i] = 0ull;
+    if (a.win) {  // (f64::MAX, f64::MAX, f64::MIN, f64::MIN): the folds' start, pixel_cache.rs:18-29
+      const uint64_t hi = f64_key(1.7976931348623157e308), lo = f64_key(-1.7976931348623157e308);
+#pragma unroll
+      for (int k = 0; k < 8; k += 4) {
+        a.win[8 * i + k + 0] = hi;
+        a.win[8 * i + k + 1] = hi;
+        a.win[8 * i + k + 2] = lo;
+        a.win[8 * i + k + 3] = lo;
+      }
+    }
   }
   uint64_t t = (uint64_t)i;
 #pragma unroll
@@ -215,6 +225,48 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         }
       }
       a.edges[v] = e;
+      const bool at_cache = a.item_win && (J.fun == 0 || J.fun == 5);
+      if (J.all_touched && at_cache) {
+        // the box of every vertex, which bounds the fill (a NaN coordinate
+        // may pin a crossing anywhere: the box becomes infinite)
+        unsigned long long* w = a.item_win + 8 * item + 4;
+        const double inf = __builtin_inf();
+        atomicMin(&w[0], f64_key(isnan(x0) ? -inf : x0));
+        atomicMax(&w[2], f64_key(isnan(x0) ? inf : x0));
+        atomicMin(&w[1], f64_key(isnan(y0) ? -inf : y0));
+        atomicMax(&w[3], f64_key(isnan(y0) ? inf : y0));
+      }
+      if (J.all_touched && has_next) {
+        // the outline segment the all-touched polygon also burns as a line
+        // (burn_geometry.rs:89-106: extract_line of every ring, edges.rs:130)
+        const double x1 = px_x(X1);
+        const double y1 = px_y(Y1);
+        const double mnx = fmin(x0, x1), mxx = fmax(x0, x1);
+        const double mny = fmin(y0, y1), mxy = fmax(y0, y1);
+        if (mnx < cols && mxx >= 0.0 && mny < rows && mxy >= 0.0) {
+          // its cells lie within the floors of its end points (one pixel of
+          // margin for the walker's 1e-9 steps), merged into the item's box
+          const int32_t c0 = (int32_t)max((int64_t)0, min(sat_isize(floor(mnx)) - 1, (int64_t)J.ncols));
+          const int32_t c1 = (int32_t)max((int64_t)0, min(sat_isize(floor(mxx)) + 2, (int64_t)J.ncols));
+          const int32_t r0 = (int32_t)max((int64_t)0, min(sat_isize(floor(mny)) - 1, (int64_t)J.nrows));
+          const int32_t r1 = (int32_t)max((int64_t)0, min(sat_isize(floor(mxy)) + 2, (int64_t)J.nrows));
+          if (c0 < c1 && r0 < r1) {
+            box.r0 = min(box.r0, r0);
+            box.r1 = max(box.r1, r1);
+            box.c0 = min(box.c0, c0);
+            box.c1 = max(box.c1, c1);
+          }
+          // the PixelCache box: f64::min / max over both end points (a NaN
+          // leaves the fold unchanged)
+          if (at_cache) {
+            unsigned long long* w = a.item_win + 8 * item;
+            if (!isnan(x0)) { atomicMin(&w[0], f64_key(x0)); atomicMax(&w[2], f64_key(x0)); }
+            if (!isnan(x1)) { atomicMin(&w[0], f64_key(x1)); atomicMax(&w[2], f64_key(x1)); }
+            if (!isnan(y0)) { atomicMin(&w[1], f64_key(y0)); atomicMax(&w[3], f64_key(y0)); }
+            if (!isnan(y1)) { atomicMin(&w[1], f64_key(y1)); atomicMax(&w[3], f64_key(y1)); }
+          }
+        }
+      }
     } else if (kind == 1) {  // line rings: extract_line
       line = true;
       uint4 rec = make_uint4(kLsNone, 0u, 0u, 0u);
@@ -228,10 +280,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
           // Bresenham visits cells between the floored end points only.
           const int64_t fx0 = sat_isize(floor(mnx)), fx1 = sat_isize(floor(mxx));
           const int64_t fy0 = sat_isize(floor(mny)), fy1 = sat_isize(floor(mxy));
-          box.c0 = (int32_t)max((int64_t)0, min(fx0, (int64_t)J.ncols));
-          box.c1 = (int32_t)max((int64_t)0, min(fx1 + 1, (int64_t)J.ncols));
-          box.r0 = (int32_t)max((int64_t)0, min(fy0, (int64_t)J.nrows));
-          box.r1 = (int32_t)max((int64_t)0, min(fy1 + 1, (int64_t)J.nrows));
+          // (all-touched walks get one pixel of margin for their 1e-9 steps)
+          const int64_t m = J.all_touched ? 1 : 0;
+          box.c0 = (int32_t)max((int64_t)0, min(fx0 - m, (int64_t)J.ncols));
+          box.c1 = (int32_t)max((int64_t)0, min(fx1 + 1 + m, (int64_t)J.ncols));
+          box.r0 = (int32_t)max((int64_t)0, min(fy0 - m, (int64_t)J.nrows));
+          box.r1 = (int32_t)max((int64_t)0, min(fy1 + 1 + m, (int64_t)J.nrows));
           // the segment's end cells (burners.rs:54-57), biased into 31 bits
           const int64_t ix0 = sat_isize(floor(x0)), iy0 = sat_isize(floor(y0));
           const int64_t ix1 = sat_isize(floor(x1)), iy1 = sat_isize(floor(y1));
@@ -321,13 +375,43 @@ __global__ void k_item_desc(DescArgs a) {
   const bool tall = d.r0 < d.r1 && (d.r1 - 1) / kBandRows > d.r0 / kBandRows;
   const bool banded = a.band_n && tall &&
                       ((kind == 0 && d.nv > kBandMinEdges) ||
-                       (kind == 1 && d.nv > kLineBandMinSegs && !(S_raw & kItemFarBit)));
+                       (kind == 1 && d.nv > kLineBandMinSegs && !(S_raw & kItemFarBit) && !J.all_touched));
   d.last = kind == 1 ? (S_raw & ~kItemFarBit) : 0ull;
+  if (kind == 0 && J.all_touched) d.last = (uint64_t)i;  // (its outline box is found by item)
+  if (a.at_words) {
+    // sum / count dedup (prelude.rs:116-118): the fill skips pixels in the
+    // outline's PixelCache; a fill pixel outside the cache box may alias a
+    // cache bit (unravel_index wraps), so such an item's cache is kept in
+    // global memory, where the tile kernel looks those pixels up
+    uint32_t words = 0;
+    CacheWin cw;
+    if (kind == 0 && J.all_touched && (J.fun == 0 || J.fun == 5) && d.r0 < d.r1 && d.c0 < d.c1 &&
+        item_cache(a.item_win, i, cw)) {
+      // fill pixels lie in the floors of the vertex box (crossings are
+      // between edge end points; spans start at floor(x + 0.5) and end
+      // before it), inside the raster; the window holds columns
+      // [x0, x0 + width) and rows [y0, y0 + length)
+      const unsigned long long* w = a.item_win + 8 * i + 4;
+      auto lo = [](double v, int64_t n) { return v < 0.0 ? (int64_t)0 : (v >= (double)n ? n : (int64_t)v); };
+      const int64_t fc0 = lo(floor(f64_from_key(w[0])), J.ncols), fc1 = lo(floor(f64_from_key(w[2])), J.ncols - 1);
+      const int64_t fr0 = lo(floor(f64_from_key(w[1])), J.nrows), fr1 = lo(floor(f64_from_key(w[3])), J.nrows - 1);
+      const uint64_t length = cw.width ? cw.cap / cw.width : 0;
+      const bool inside = fc0 > fc1 || fr0 > fr1 ||
+                          (fc0 >= cw.x0 && (uint64_t)fc1 - (uint64_t)cw.x0 < cw.width && fr0 >= cw.y0 &&
+                           (uint64_t)fr1 - (uint64_t)cw.y0 < length);
+      if (!inside) {
+        if (cw.cap > ((uint64_t)1 << 34)) atomicOr(a.status, 32u);  // more than 2 GiB of cache bits
+        else words = (uint32_t)((cw.cap + 63) / 64);
+      }
+    }
+    a.at_words[i] = words;
+  }
   d.job = (uint32_t)j;
   d.band = 0;
   d.flags = 0;
   d.wide = 0;
   if (kind == 0 && (S_raw & kItemOddBit)) d.flags |= kItemOddRows;
+  if (a.at_words && a.at_words[i]) d.flags |= kItemAtCache;
   a.desc[i] = d;
   if (a.wide_units) {
     // Work of one strip, from the rows the edges cross (S) over the item's
@@ -403,6 +487,35 @@ __global__ void k_wide_assign(WideAssignArgs a) {
   a.desc[i].flags |= kItemWide;
   a.desc[i].wide = (uint32_t)(wb >> 4);  // every item's words are a multiple of 16
   for (uint32_t k = 0; k < nu; ++k) a.unit_item[ub + k] = (uint32_t)i;
+}
+
+// ------------------------------------------------- all-touched PixelCache --
+
+// The PixelCache of each kItemAtCache item (burn_geometry.rs:225-234 with
+// pixel_cache.rs:38-59): every cell its kept outline segments walk sets the
+// bit at the cell's cache index (cells inside the raster; an index past the
+// capacity, which the reference rejects, sets nothing).  One wave per item.
+__global__ __launch_bounds__(256) void k_at_cache(AtCacheArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = __builtin_amdgcn_readfirstlane(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= a.n_items) return;
+  const RZ_CONST ItemDesc& d = as_const(a.desc)[i];
+  if (!(d.flags & kItemAtCache)) return;
+  const RZ_CONST JobDev& J = as_const(a.jobs)[d.job];
+  CacheWin cw;
+  if (!item_cache(a.item_win, i, cw)) return;
+  unsigned long long* bits = a.at_bits + a.at_base[i];
+  bool ok = true;
+  for (int64_t e = lane; e < d.nv; e += 64) {
+    double x0, y0, x1, y1;
+    if (!kept_segment(J, d.vbeg + e - J.v_base, x0, y0, x1, y1)) continue;
+    ok &= walk_all_touched(x0, y0, x1, y1, J.ncols, J.nrows, INT64_MIN, INT64_MAX, INT64_MIN, INT64_MAX,
+                           [&](int64_t x, int64_t y) {
+                             const uint64_t idx = cache_index(cw, x, y);
+                             if (idx < cw.cap) atomicOr(&bits[idx >> 6], 1ull << (idx & 63));
+                           });
+  }
+  if (!ok) atomicOr(a.status, 64u);
 }
 
 // Count pass, one wave per banded item (of the list k_band_assign built): band_cnt of each of its strips,

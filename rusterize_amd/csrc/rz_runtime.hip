@@ -83,8 +83,10 @@ struct rz_context {
       vals_b, tile_off, temp, misc, jobtab, queue, band_n, band_base, band_cnt, band_off, band_edges,
       heavy_cnt, heavy_base, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
       sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
-      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7, wide_j, band_segs;
-  // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words
+      wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7, wide_j, band_segs,
+      item_win, at_words, at_base, at_bits, scan_tmp8;
+  // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words,
+  // [6] all-touched PixelCache words
   uint64_t* host_misc = nullptr;
   // job table: built in tab_build, staged in pinned host_tab (tab_ev marks
   // the end of its last upload); tab_dev / tab_dev_bytes say which device
@@ -112,7 +114,7 @@ struct rz_context {
             &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
             &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
             &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items, &heavy_list, &scan_tmp7,
-            &wide_j, &band_segs};
+            &wide_j, &band_segs, &item_win, &at_words, &at_base, &at_bits, &scan_tmp8};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -219,7 +221,7 @@ extern "C" rz_status rz_context_create(int device, rz_context** out) {
     return st;
   }
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIP_TRY(hipHostMalloc((void**)&c->host_misc, 6 * sizeof(uint64_t), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&c->host_misc, 8 * sizeof(uint64_t), hipHostMallocDefault));
   HIP_TRY(hipEventCreateWithFlags(&c->tab_ev, hipEventDisableTiming));
   *out = c;
   return RZ_OK;
@@ -263,6 +265,11 @@ static rz_status check_status_word(rz_context* c) {
     return rz::fail(RZ_CAPACITY_ERROR,
                     "line coordinates beyond +-2^60 pixels are outside the supported range");
   if (flags & 16u) return rz::fail(RZ_VALUE_ERROR, "layer index out of range");
+  if (flags & 32u)
+    return rz::fail(RZ_CAPACITY_ERROR,
+                    "an all-touched polygon's pixel cache would exceed 2^34 bits (its outline box is too large)");
+  if (flags & 64u)
+    return rz::fail(RZ_VALUE_ERROR, "an all-touched line walk did not end (non-finite coordinates)");
   if (flags & (4u | 8u))
     return rz::fail(RZ_CAPACITY_ERROR,
                     "the batch outgrew the capacities it was captured with; capture it again");
@@ -601,6 +608,7 @@ rz_status check_job_common(const rz_job& j, bool need_out) {
     return rz::fail(RZ_VALUE_ERROR, "NULL geometry or output of a job");
   if ((unsigned)j.dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
   if ((unsigned)j.fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
+  if ((unsigned)j.sum_mode > 1) return rz::fail(RZ_VALUE_ERROR, "unknown sum mode");
   const uint32_t nl = j.layer_dev ? j.n_layers : 1u;
   if (nl == 0 || nl >= (1u << 29)) return rz::fail(RZ_VALUE_ERROR, "n_layers must be in [1, 2^29)");
   if (j.info.nrows >= (uint64_t)INT32_MAX - TILE_H || j.info.ncols >= (uint64_t)INT32_MAX - 256)
@@ -656,6 +664,8 @@ static uint64_t job_record(const rz_job& j, JobDev& J, int64_t v_base, int64_t i
   J.n_tiles = (uint32_t)std::min<uint64_t>(jt, UINT32_MAX);
   J.fun = (int32_t)j.fun;
   J.dedup_lines = j.info.xres != j.info.yres;
+  J.sum_unordered = j.fun == RZ_SUM && j.sum_mode == RZ_SUM_UNORDERED && (j.dtype == RZ_F32 || j.dtype == RZ_F64);
+  J.all_touched = j.all_touched ? 1 : 0;
   J.field = j.field_dev;
   J.scalar_bits = j.scalar_bits;
   J.bg_bits = j.background_bits;
@@ -701,7 +711,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     else if (std::strcmp(wm, "always") == 0) wide_mode = 2;
   }
   if (masks_only) wide_mode = 2;
-  auto has_lines = [](const rz_job& j) { return j.geometry->has_line || j.geometry->has_point; };
+  // (all-touched jobs walk their polygons' outlines too: the lines kernel)
+  auto has_lines = [](const rz_job& j) { return j.geometry->has_line || j.geometry->has_point || j.all_touched; };
   std::vector<int64_t> order((size_t)n_in);
   for (int64_t k = 0; k < n_in; ++k) order[(size_t)k] = k;
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
@@ -725,6 +736,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   bool any_binned = false, any_poly = false, any_lp = false, any_band = false;
   bool any_poly_band = false, any_line_band = false;
   bool any_tw64 = false, any_tw256 = false;
+  bool any_at_cache = false;  // all-touched sum / count polygons: PixelCache windows
   int max_tile_w = 64;
   for (size_t k = 0; k < n; ++k) {
     const rz_job& j = jobs_in[order[k]];
@@ -755,6 +767,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     any_line_band |= dg->has_line && dg->max_line_nv > kLineBandMinSegs;
     any_band = any_poly_band || any_line_band;
     any_lp |= lines;
+    any_at_cache |= j.all_touched && dg->has_poly && (j.fun == RZ_SUM || j.fun == RZ_COUNT);
     if (groups.empty() || groups.back().dtype != j.dtype || groups.back().fun != (int)j.fun ||
         groups.back().lines != lines)
       groups.push_back(Group{j.dtype, (int)j.fun, lines, (uint32_t)T, (uint32_t)T});
@@ -793,6 +806,12 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   if (any_binned) {
     HIP_TRY(ensure(c->stream, c->counts, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->stream, c->offsets, (size_t)(I + 1) * 8));
+  }
+  if (any_at_cache) {
+    if (capture) return rz::fail(RZ_VALUE_ERROR, "all_touched jobs cannot be captured in a graph");
+    HIP_TRY(ensure(c->stream, c->item_win, (size_t)std::max<int64_t>(I, 1) * 64));
+    HIP_TRY(ensure(c->stream, c->at_words, (size_t)(I + 1) * 4));
+    HIP_TRY(ensure(c->stream, c->at_base, (size_t)(I + 1) * 8));
   }
   const bool any_wide = any_poly && wide_mode != 1 && I > 0;
   // A small eager call whose only pre-pass is the wide one sizes the wide
@@ -842,12 +861,14 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   // 1. item boxes reset and the call's counters cleared (one launch), edges /
   // pixel coordinates / item boxes, then the item descriptors
   {
-    InitArgs ia;
+    InitArgs ia{};
     std::memset(&ia, 0, sizeof(ia));
     ia.box = (ItemBox*)c->item_box.p;
     ia.last = (unsigned long long*)c->item_last.p;
     ia.n_items = I;
+    ia.win = any_at_cache ? (unsigned long long*)c->item_win.p : nullptr;
     ia.clear(c->misc.p, (c->profiling > 1 ? misc_bytes : 64) / 4);
+    if (any_at_cache) ia.clear((uint32_t*)c->at_words.p + I, 1);
     ia.clear(c->queue.p, queue_words);
     if (any_binned) ia.clear((uint32_t*)c->counts.p + I, 1);
     if (any_band) ia.clear((uint32_t*)c->band_n.p + I, 1);
@@ -859,7 +880,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     hipLaunchKernelGGL(k_init, dim3(blocks_for((int64_t)nz)), dim3(256), 0, s, ia);
   }
   if (V > 0) {
-    PrepArgs pa;
+    PrepArgs pa{};
     pa.jobs = d_jobs;
     pa.job_vend = d_vend;
     pa.n_jobs = (int)n;
@@ -868,10 +889,11 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     pa.lseg = (uint4*)c->lseg.p;
     pa.item_box = (ItemBox*)c->item_box.p;
     pa.item_last = (unsigned long long*)c->item_last.p;
+    pa.item_win = any_at_cache ? (unsigned long long*)c->item_win.p : nullptr;
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
   }
   if (I > 0) {
-    DescArgs da;
+    DescArgs da{};
     da.jobs = d_jobs;
     da.job_iend = d_iend;
     da.n_jobs = (int)n;
@@ -885,7 +907,38 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     da.wide_units = any_wide ? (uint32_t*)c->wide_units.p : nullptr;
     da.wide_words = any_wide ? (uint32_t*)c->wide_words.p : nullptr;
     da.status = d_status;
+    da.item_win = any_at_cache ? (const unsigned long long*)c->item_win.p : nullptr;
+    da.at_words = any_at_cache ? (uint32_t*)c->at_words.p : nullptr;
     hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
+  }
+  TileArgs ta{};
+  std::memset(&ta, 0, sizeof(ta));
+  // all-touched sum / count polygons whose fill reaches past their
+  // PixelCache window: their caches in global memory (one host round trip
+  // for the size)
+  if (any_at_cache && I > 0) {
+    HIP_TRY(ensure(c->stream, c->scan_tmp8, scan_blocks((uint64_t)I + 1) * 8));
+    exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->at_words.p, (uint64_t*)c->at_base.p,
+                                       (uint64_t)I + 1, (uint64_t*)c->scan_tmp8.p, s);
+    HIP_TRY(hipMemcpyAsync(&c->host_misc[6], (const uint64_t*)c->at_base.p + I, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const uint64_t words = c->host_misc[6];
+    if (words > 0) {
+      HIP_TRY(ensure(c->stream, c->at_bits, words * 8));
+      zero_words(c->at_bits.p, words * 2, s);
+      AtCacheArgs aa{};
+      aa.jobs = d_jobs;
+      aa.desc = (const ItemDesc*)c->desc.p;
+      aa.n_items = I;
+      aa.item_win = (const unsigned long long*)c->item_win.p;
+      aa.at_base = (const uint64_t*)c->at_base.p;
+      aa.at_bits = (unsigned long long*)c->at_bits.p;
+      aa.status = d_status;
+      hipLaunchKernelGGL(k_at_cache, dim3((unsigned)((I + 3) / 4)), dim3(256), 0, s, aa);
+    }
+    ta.item_win = (const unsigned long long*)c->item_win.p;
+    ta.at_base = (const uint64_t*)c->at_base.p;
+    ta.at_bits = words > 0 ? (const unsigned long long*)c->at_bits.p : nullptr;
   }
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
 
@@ -893,8 +946,6 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   // Eager calls read the pair, band-entry and band-edge totals back (one or
   // two host round trips) and size the buffers to them; a captured batch
   // (caps != null) uses fixed capacities instead and never leaves the device.
-  TileArgs ta;
-  std::memset(&ta, 0, sizeof(ta));
   uint64_t n_pairs = 0;
   if (any_binned) {
     HIP_TRY(ensure(c->stream, c->scan_tmp, scan_blocks((uint64_t)I + 1) * 8));
@@ -947,7 +998,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     if (wide_units > 0 && wide_words > 0) {
       HIP_TRY(ensure(c->stream, c->wide_item, wide_units * 4));
       HIP_TRY(ensure(c->stream, c->wide_masks, wide_words * 8));
-      WideAssignArgs wa;
+      WideAssignArgs wa{};
       wa.desc = (ItemDesc*)c->desc.p;
       wa.n_items = I;
       wa.wide_units = (const uint32_t*)c->wide_units.p;
@@ -978,7 +1029,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I)), dim3(256), 0, s, (ItemDesc*)c->desc.p,
                          (const uint32_t*)c->band_n.p, (const uint64_t*)c->band_base.p, I, nb,
                          (uint32_t*)c->band_items.p, n_list);
-      BandArgs bd;
+      BandArgs bd{};
       bd.desc = (const ItemDesc*)c->desc.p;
       bd.band_n = (const uint32_t*)c->band_n.p;
       bd.items = (const uint32_t*)c->band_items.p;
@@ -1028,7 +1079,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   auto launch_wide = [&](const uint32_t* h_cnt, const uint64_t* h_base, const uint32_t* w_j,
                          uint64_t* masks, uint32_t slot_words) {
     if (wide_units > 0) {
-      WideCoverArgs wc;
+      WideCoverArgs wc{};
       wc.jobs = d_jobs;
       wc.desc = (const ItemDesc*)c->desc.p;
       wc.n_items = I;
@@ -1065,7 +1116,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     HIP_TRY(ensure(c->stream, c->vals_b, cap * 4));
     HIP_TRY(ensure(c->stream, c->tile_off, ((size_t)n_tiles + 1) * 4));
     HIP_TRY(ensure(c->stream, c->pair_desc, cap * sizeof(ItemDesc)));
-    BinArgs ba;
+    BinArgs ba{};
     ba.jobs = d_jobs;
     ba.desc = (const ItemDesc*)c->desc.p;
     ba.n_items = I;
@@ -1148,7 +1199,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                          (const uint32_t*)c->tile_off.p, (const uint64_t*)c->heavy_base.p,
                          (const uint64_t*)c->offsets.p, (uint32_t*)c->wide_j.p, (uint32_t*)c->cover_list.p,
                          cover_count);
-      CoverArgs ca;
+      CoverArgs ca{};
       ca.jobs = d_jobs;
       ca.job_tend = d_tend;
       ca.n_jobs = (int)n;
@@ -1392,9 +1443,7 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
   if ((unsigned)dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
   if ((unsigned)fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
   if (!field_dev && !scalar_host) return rz::fail(RZ_VALUE_ERROR, "no field and no scalar value");
-  if (all_touched)
-    return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not implemented on the device path yet");
-  (void)sum_mode;  // the ordered fold is exact, so it also serves the unordered mode
+  if ((unsigned)sum_mode > 1) return rz::fail(RZ_VALUE_ERROR, "unknown sum mode");
   if (info->nrows == 0 || info->ncols == 0) return RZ_OK;  // empty raster: nothing to write
   const size_t esz = rz_dtype_size(dtype);
   rz_job j;
@@ -1409,6 +1458,8 @@ extern "C" rz_status rz_rasterize_device(rz_context* c, const rz_device_geometry
   j.n_layers = layer_dev ? n_layers : 1u;
   std::memcpy(&j.background_bits, bg_host, esz);
   j.out_dev = out_dev;
+  j.sum_mode = sum_mode;
+  j.all_touched = all_touched ? 1 : 0;
   return run_batch(c, &j, 1);
 }
 
@@ -1442,7 +1493,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
   if (!field_dev && !scalar_host) return rz::fail(RZ_VALUE_ERROR, "no field and no scalar value");
   if (all_touched)
     return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not available on the allocation-free path");
-  (void)sum_mode;  // the ordered fold is exact, so it also serves the unordered mode
+  if ((unsigned)sum_mode > 1) return rz::fail(RZ_VALUE_ERROR, "unknown sum mode");
   if (info->nrows == 0 || info->ncols == 0) return RZ_OK;  // empty raster: nothing to write
   if (c->device < 0 || c->device >= kMaxDevices || !g_scratch[c->device])
     return rz::fail(RZ_DEVICE_ERROR, "no allocation-free workspace on this device");
@@ -1459,6 +1510,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
   j.n_layers = layer_dev ? n_layers : 1u;
   std::memcpy(&j.background_bits, bg_host, esz);
   j.out_dev = out_dev;
+  j.sum_mode = sum_mode;
   JobDev J;
   const uint64_t jt = job_record(j, J, 0, 0, 0);
   J.scan = 1;  // no bins: every strip tests the item boxes itself
@@ -1502,7 +1554,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
   {
     // item boxes, the status word and the tile queue (the previous call's
     // contents stay in the array otherwise)
-    InitArgs ia;
+    InitArgs ia{};
     std::memset(&ia, 0, sizeof(ia));
     ia.box = d_box;
     ia.last = d_last;
@@ -1513,7 +1565,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
     hipLaunchKernelGGL(k_init, dim3(blocks_for((int64_t)nz)), dim3(256), 0, s, ia);
   }
   if (V > 0) {
-    PrepArgs pa;
+    PrepArgs pa{};
     pa.jobs = d_tab;
     pa.job_vend = d_ends;
     pa.n_jobs = 1;
@@ -1525,7 +1577,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(V)), dim3(256), 0, s, pa);
   }
   if (I > 0) {
-    DescArgs da;
+    DescArgs da{};
     std::memset(&da, 0, sizeof(da));
     da.jobs = d_tab;
     da.job_iend = d_ends + 1;
@@ -1538,7 +1590,7 @@ extern "C" rz_status rz_rasterize_device_noalloc(rz_context* c, const rz_device_
     da.status = d_status;
     hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
   }
-  TileArgs ta;
+  TileArgs ta{};
   std::memset(&ta, 0, sizeof(ta));
   ta.jobs = d_tab;
   ta.job_tend = d_ends + 2;
@@ -1661,6 +1713,22 @@ extern "C" int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, in
   else
     walk_window<__int128>(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, emit);
   return n;
+}
+
+// Host evaluation of the all-touched walker of the tile kernel
+// (walk_all_touched in rz_kernels.h), for checks against the oracle.
+extern "C" int64_t rz_debug_walk_all_touched(double x0, double y0, double x1, double y1, int64_t ncols,
+                                             int64_t nrows, int64_t X0, int64_t X1, int64_t Y0, int64_t Y1,
+                                             int64_t* xs, int64_t* ys, int64_t cap) {
+  int64_t n = 0;
+  auto emit = [&](int64_t x, int64_t y) {
+    if (n < cap) {
+      xs[n] = x;
+      ys[n] = y;
+    }
+    ++n;
+  };
+  return walk_all_touched(x0, y0, x1, y1, ncols, nrows, X0, X1, Y0, Y1, emit) ? n : -1;
 }
 
 // Host evaluation of the polygon row coverage used by the tile kernel: the

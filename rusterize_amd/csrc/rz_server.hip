@@ -5,15 +5,22 @@
 //
 // n_streams worker threads each own one context (one HIP stream, default
 // priority).  Submitted requests wait in a FIFO queue.  An idle worker admits
-// queued requests in arrival order while their output rasters and estimated
-// workspace fit the memory budget (a request larger than the whole budget is
-// failed), up to max_batch of them, allocates their outputs, runs them as one
-// batch on its context and waits for it.  Batches of different workers
-// overlap on the device, and one batch's host round trips (the eager
+// queued requests in arrival order, up to max_batch of them, while the
+// budget holds them (a request larger than the whole budget is failed), runs
+// them as one batch on its context and waits for it.  Batches of different
+// workers overlap on the device, and one batch's host round trips (the eager
 // pipeline reads a few totals back) do not hold up the others, so requests
 // arriving while others run are admitted as soon as a worker and memory free
-// up.  Outputs stay on the device until the caller releases them, which
-// returns their memory to the budget.
+// up.  Outputs stay on the device until the caller releases them; released
+// outputs are kept in a cache of size classes for later requests.
+//
+// Memory: the budget holds the outputs (admitted and cached) plus every
+// worker context's workspace as measured after its last batch, plus, while a
+// batch runs, the growth its workspace estimate may need beyond that.
+// Admission only does this bookkeeping under the server's lock; the worker
+// allocates fresh outputs and frees evicted cache entries afterwards, with
+// stream-ordered allocation on its own stream (no device-wide
+// synchronisation, and no worker waits on another's allocation).
 #pragma once
 
 #include <chrono>
@@ -35,18 +42,28 @@ struct Request {
   rz_job job{};
   size_t out_bytes = 0, ws_bytes = 0;
   int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  bool admitted = false;  // its output bytes are held in the budget until released
   rz_status status = RZ_OK;
   std::string error;
   Clock::time_point t_submit, t_start, t_done;
 };
 
-// Device memory a request may need besides its output: edge records and
-// coordinates per vertex, descriptors and bins per item, and the wide
-// pre-pass masks (capped at 2 bytes per output pixel).
+// Workspace a request may need (run_batch's buffers): per vertex its edge or
+// segment record and band copies (64 B); per item its box, statistics,
+// descriptor, scan entries and, per tile its box covers, a bin pair with its
+// sort copies, gathered descriptor and heavy-tile masks (4 x 2 KiB at most);
+// per output pixel at most 2 B of wide-pre-pass masks and the tile tables.
 size_t workspace_estimate(const rz_job& j) {
   const rz_device_geometry* g = j.geometry;
-  const size_t px = (size_t)j.info.nrows * (size_t)j.info.ncols * (j.layer_dev ? j.n_layers : 1u);
-  return (size_t)g->V * 64 + (size_t)g->I * 512 + 2 * px;
+  const size_t nl = j.layer_dev ? j.n_layers : 1u;
+  const size_t px = (size_t)j.info.nrows * (size_t)j.info.ncols * nl;
+  const size_t tw = tile_w((int)rz_dtype_size(j.dtype));
+  const size_t tiles = ((size_t)j.info.ncols + tw - 1) / tw * (((size_t)j.info.nrows + TILE_H - 1) / TILE_H) * nl;
+  // an item's box meets about 4 tiles (its tile rectangle, most items being
+  // smaller than a tile); pairs beyond that are covered by the per-pixel term
+  const size_t pairs = std::min<size_t>((size_t)g->I * 4, tiles * 64);
+  return (size_t)g->V * 64 + (size_t)g->I * 160 + pairs * (16 + 64 + 8 + 8 * 1024) + 2 * px + tiles * 40 +
+         (1u << 20);
 }
 
 }  // namespace
@@ -62,9 +79,17 @@ struct rz_server {
   std::unordered_map<uint64_t, Request> reqs;
   std::unordered_multimap<size_t, void*> free_bufs;  // released outputs, by size
   uint64_t next_ticket = 1;
-  uint64_t used = 0;       // outputs and workspace of admitted requests
-  uint64_t cached = 0;     // bytes in free_bufs
+  uint64_t out_used = 0;        // outputs of admitted, unreleased requests
+  uint64_t cached = 0;          // bytes in free_bufs
+  std::vector<uint64_t> ws;     // each context's workspace, measured after its last batch
+  std::vector<uint64_t> grow;   // each running batch's reserved workspace growth
   uint64_t n_batches = 0, n_done = 0, peak_used = 0, running = 0;
+
+  uint64_t used() const {
+    uint64_t u = out_used + cached;
+    for (size_t k = 0; k < ws.size(); ++k) u += ws[k] + grow[k];
+    return u;
+  }
   bool stop = false;
   std::vector<std::thread> workers;
 
@@ -78,33 +103,23 @@ struct rz_server {
     return c <= (1u << 20) ? c : ((bytes + q - 1) / q) * q;
   }
 
-  void* take_buffer(size_t bytes) {
-    auto it = free_bufs.find(bytes);
-    if (it != free_bufs.end()) {
-      void* p = it->second;
-      free_bufs.erase(it);
-      cached -= bytes;
-      return p;
-    }
-    // make room in the cache for the budget before a fresh allocation
-    while (cached > 0 && used + cached + bytes > budget) {
-      auto victim = free_bufs.begin();
-      cached -= victim->first;
-      (void)hipFree(victim->second);
-      free_bufs.erase(victim);
-    }
-    void* p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) return nullptr;
-    return p;
-  }
+  // What a worker must do outside the lock before its batch runs.
+  struct Admission {
+    std::vector<uint64_t> tickets;
+    std::vector<uint64_t> fresh;   // tickets whose output is allocated fresh
+    std::vector<void*> evict;      // cached outputs to free
+  };
 
-  // Admits queued requests in arrival order while they fit (lock held).
-  std::vector<uint64_t> admit() {
-    std::vector<uint64_t> batch;
-    while (!queue.empty() && (int)batch.size() < max_batch) {
+  // Admits queued requests in arrival order while the budget holds them
+  // (lock held; bookkeeping only).
+  Admission admit(int slot) {
+    Admission a;
+    uint64_t est = 0;
+    while (!queue.empty() && (int)a.tickets.size() < max_batch) {
       Request& r = reqs[queue.front()];
-      const uint64_t need = r.out_bytes + r.ws_bytes;
-      if (need > budget) {  // can never fit
+      const uint64_t est_r = est + r.ws_bytes;
+      const uint64_t g = est_r > ws[slot] ? est_r - ws[slot] : 0;  // this slot's growth with r
+      if (r.out_bytes + r.ws_bytes > budget) {  // can never fit
         r.state = 3;
         r.status = RZ_CAPACITY_ERROR;
         r.error = "request exceeds the server's memory budget";
@@ -113,18 +128,37 @@ struct rz_server {
         cv_done.notify_all();
         continue;
       }
-      if (used + need > budget) break;
-      void* out = take_buffer(r.out_bytes);
-      if (!out) break;  // device memory exhausted: wait for releases
-      r.job.out_dev = out;
-      used += need;
-      peak_used = std::max(peak_used, used);
+      // a cached output of r's size class is reused; otherwise cache
+      // entries are evicted while that makes r fit
+      auto hit = free_bufs.find(r.out_bytes);
+      const uint64_t fresh_bytes = hit == free_bufs.end() ? r.out_bytes : 0;
+      const uint64_t base = used() - grow[slot];
+      while (fresh_bytes && base + fresh_bytes + g > budget && !free_bufs.empty()) {
+        auto victim = free_bufs.begin();
+        cached -= victim->first;
+        a.evict.push_back(victim->second);
+        free_bufs.erase(victim);
+      }
+      if (used() - grow[slot] + fresh_bytes + g > budget) break;
+      if (hit != free_bufs.end()) {
+        r.job.out_dev = hit->second;
+        cached -= hit->first;
+        free_bufs.erase(hit);
+      } else {
+        r.job.out_dev = nullptr;
+        a.fresh.push_back(r.ticket);
+      }
+      out_used += r.out_bytes;
+      r.admitted = true;
+      grow[slot] = g;
+      est = est_r;
+      peak_used = std::max(peak_used, used());
       r.state = 1;
       r.t_start = Clock::now();
-      batch.push_back(r.ticket);
+      a.tickets.push_back(r.ticket);
       queue.pop_front();
     }
-    return batch;
+    return a;
   }
 
   // One worker per context: admit a batch, run it, wait for it on this
@@ -153,38 +187,55 @@ struct rz_server {
         cv_work.wait(lk, [&] { return stop || !queue.empty(); });
         continue;
       }
-      std::vector<uint64_t> batch = admit();
+      Admission adm = admit(slot);
+      std::vector<uint64_t>& batch = adm.tickets;
       if (batch.empty()) {  // the budget is full: wait for releases
         cv_work.wait_for(lk, std::chrono::microseconds(200));
         continue;
       }
-      std::vector<rz_job> jobs;
-      jobs.reserve(batch.size());
-      for (uint64_t t : batch) jobs.push_back(reqs[t].job);
+      std::vector<uint64_t> fresh_bytes;
+      for (uint64_t t : adm.fresh) fresh_bytes.push_back(reqs[t].out_bytes);
       ++running;
       lk.unlock();
-      rz_status st = run_batch(c, jobs.data(), (int64_t)jobs.size());
+      // evictions and fresh outputs, stream-ordered on this worker's stream
+      rz_status st = RZ_OK;
+      for (void* p : adm.evict)
+        if (hipFreeAsync(p, c->stream) != hipSuccess) st = rz::fail(RZ_DEVICE_ERROR, "freeing a cached output failed");
+      std::vector<void*> fresh_ptr(adm.fresh.size(), nullptr);
+      for (size_t k = 0; k < adm.fresh.size() && st == RZ_OK; ++k)
+        if (hipMallocAsync(&fresh_ptr[k], std::max<uint64_t>(fresh_bytes[k], 1), c->stream) != hipSuccess)
+          st = rz::fail(RZ_CAPACITY_ERROR, "device memory exhausted allocating an output");
+      std::vector<rz_job> jobs;
+      jobs.reserve(batch.size());
+      {
+        std::lock_guard<std::mutex> g(mu);
+        for (size_t k = 0; k < adm.fresh.size(); ++k) reqs[adm.fresh[k]].job.out_dev = fresh_ptr[k];
+        for (uint64_t t : batch) jobs.push_back(reqs[t].job);
+      }
+      if (st == RZ_OK) st = run_batch(c, jobs.data(), (int64_t)jobs.size());
       if (st == RZ_OK && !(have_ev && hipEventRecord(ev, c->stream) == hipSuccess &&
                            hipEventSynchronize(ev) == hipSuccess))
         st = rz::fail(RZ_DEVICE_ERROR, "waiting for a batch failed");
       if (st == RZ_OK) st = check_status_word(c);
       const std::string msg = st ? rz_last_error() : std::string();
+      const uint64_t ws_now = rz_context_workspace_bytes(c);
       const auto now = Clock::now();
       lk.lock();
+      ws[slot] = ws_now;  // the measured workspace replaces the reservation
+      grow[slot] = 0;
+      peak_used = std::max(peak_used, used());
       for (uint64_t t : batch) {
         Request& r = reqs[t];
         r.state = st == RZ_OK ? 2 : 3;
         r.status = st;
         r.error = msg;
         r.t_done = now;
-        used -= r.ws_bytes;  // the workspace estimate is released; the output stays
-        r.ws_bytes = 0;
         ++n_done;
       }
       --running;
       ++n_batches;
       cv_done.notify_all();
-      cv_work.notify_all();  // workspace returned to the budget
+      cv_work.notify_all();  // the reservation returned to the budget
     }
     lk.unlock();
     if (have_ev) (void)hipEventDestroy(ev);
@@ -210,6 +261,8 @@ extern "C" rz_status rz_server_create(int device, uint64_t memory_budget, int n_
     }
     s->ctx.push_back(c);
   }
+  s->ws.assign((size_t)n_streams, 0);
+  s->grow.assign((size_t)n_streams, 0);
   for (int k = 0; k < n_streams; ++k) s->workers.emplace_back([s, k] { s->work(k); });
   *out = s;
   return RZ_OK;
@@ -280,10 +333,10 @@ extern "C" rz_status rz_server_release(rz_server* s, uint64_t ticket) {
   if (it == s->reqs.end()) return rz::fail(RZ_VALUE_ERROR, "unknown ticket");
   if (it->second.state < 2) return rz::fail(RZ_VALUE_ERROR, "request still running");
   Request& r = it->second;
+  if (r.admitted) s->out_used -= r.out_bytes;
   if (r.job.out_dev) {
     s->free_bufs.emplace(r.out_bytes, r.job.out_dev);
     s->cached += r.out_bytes;
-    s->used -= r.out_bytes;
   }
   s->reqs.erase(it);
   s->cv_work.notify_one();
@@ -297,7 +350,7 @@ extern "C" rz_status rz_server_get_stats(rz_server* s, rz_server_stats* out) {
   out->in_flight_batches = s->running;
   out->batches = s->n_batches;
   out->completed = s->n_done;
-  out->memory_used = s->used;
+  out->memory_used = s->used();
   out->memory_peak = s->peak_used;
   out->memory_budget = s->budget;
   return RZ_OK;

@@ -424,7 +424,7 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
                                      (uint32_t*)c->sort_off.p, (uint32_t*)c->sort_tmp.p, s);
   const uint32_t* perm = (const uint32_t*)(in_b ? c->vals_b.p : c->vals_a.p);
   // 2. units of each position, placed by a scan (and the dedup lines' bits)
-  SparseArgs a;
+  SparseArgs a{};
   std::memset(&a, 0, sizeof(a));
   a.job = (const JobDev*)c->jobtab.p;
   a.desc = (const ItemDesc*)c->desc.p;
@@ -440,7 +440,7 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
   HIP_TRY(ensure(s, c->band_base, (size_t)(I + 1) * 8));  // dedup bits per position
   HIP_TRY(ensure(s, c->heavy_base, (size_t)(I + 1) * 8)); // their scan
   HIP_TRY(ensure(s, c->scan_tmp, scan_blocks((uint64_t)I + 1) * 8));
-  InitArgs ia;
+  InitArgs ia{};
   std::memset(&ia, 0, sizeof(ia));
   ia.clear((uint32_t*)c->counts.p + I, 1);
   ia.clear((uint64_t*)c->band_base.p + I, 2);
@@ -471,7 +471,7 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
     a.bits = (uint64_t*)c->wide_item.p;
     a.bit_base = (const uint64_t*)c->heavy_base.p;
   }
-  InitArgs ib;
+  InitArgs ib{};
   std::memset(&ib, 0, sizeof(ib));
   ib.clear(a.counts, U + 1);
   if (a.dedup) ib.clear(a.bits, 4 * n_bits_words);  // (32-bit words)

@@ -366,7 +366,7 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T>::value))) void k_tile(TileAr
 // Segments come as 16-byte records with floored end points (k_prep); a line
 // item of many segments over several strips reads only its band of the strip.
 template <typename T, int FUN>
-__global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_lp(TileArgs a) {
   constexpr int TW = tile_w(sizeof(T));
   constexpr int NWORD = TW / 64;
   constexpr int PPC = 16 / (int)sizeof(T);
@@ -382,6 +382,9 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
   __shared__ __align__(16) T s_acc[4][NPIXS];
   __shared__ __align__(16) uint32_t s_cnt[4][CNT_WORDS];
   __shared__ uint32_t s_mark[4][NPIXS / 32];
+  // unordered float sum: the strip holds plain sums and s_wr which pixels have any
+  constexpr bool FSUM = FUN == 0 && (sizeof(T) == 4 || sizeof(T) == 8) && (T)0.5 != (T)0;
+  __shared__ uint32_t s_wr[4][FSUM ? NPIXS / 32 : 1];
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rl = lane >> 2, q = lane & 3;
@@ -391,6 +394,7 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
   T* acc = s_acc[wave];
   uint32_t* cnt = s_cnt[wave];
   uint32_t* mark = s_mark[wave];
+  uint32_t* wr = s_wr[wave];
   // (every apply leaves its counters and marks at zero)
   if (COUNTED)
     for (int w = lane; w < CNT_WORDS; w += 64) cnt[w] = 0u;
@@ -422,18 +426,104 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
     const bool hb_ok = a.heavy_base && !J.scan && pe - pb >= a.heavy_min;
     if (hb_ok) hbase = as_const(a.heavy_base)[t];
     const bool dedup_job = J.dedup_lines != 0;
+    const bool at_job = J.all_touched != 0;
 
-    // the strip starts as the background
+    // unordered float sum (no dedup: a line's pixel counts once there):
+    // lines and points add as they land (RZ_SUM_UNORDERED)
+    const bool unord = FSUM && J.sum_unordered && !dedup_job && !at_job;
+    // the strip starts as the background (unordered: as empty sums)
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
       T* p = acc + (srow + k * RSTEP) * TW + ccol;
 #pragma unroll
-      for (int j = 0; j < PPC; ++j) p[j] = bg;
+      for (int j = 0; j < PPC; ++j) p[j] = unord ? (T)0 : bg;
     }
+    if (FSUM && unord)
+      for (int w = lane; w < NPIXS / 32; w += 64) wr[w] = 0u;
     wave_sync();
+    // an unordered write of v at strip pixel idx (any lane, any order)
+    auto add1 = [&](uint32_t idx, T v) {
+      if constexpr (FSUM) {
+        atomicAdd(&acc[idx], v);
+        atomicOr(&wr[idx >> 5], 1u << (idx & 31u));
+      }
+    };
 
     // one write of value v at strip pixel idx, by this lane alone
     auto fold1 = [&](uint32_t idx, T v) { fold<T, FUN>(acc[idx], v, bg, 1u); };
+
+    // each lane folds its marked pixels once and clears their marks
+    auto apply_marks = [&](T v) {
+      wave_sync();
+#pragma unroll
+      for (int c0 = 0; c0 < PPL; c0 += 16) {
+        const int base = lane * PPL + c0;
+        const uint32_t bits = (mark[base >> 5] >> (base & 31)) & 0xFFFFu;
+        if (bits) {
+          burns += (unsigned long long)__popc(bits);
+          T px[16];
+          __builtin_memcpy(px, acc + base, sizeof(px));
+#pragma unroll
+          for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (bits >> j) & 1u);
+          __builtin_memcpy(acc + base, px, sizeof(px));
+        }
+      }
+      wave_sync();
+      for (int w = lane; w < NPIXS / 32; w += 64) mark[w] = 0u;
+      wave_sync();
+    };
+
+    // all-touched: the cells of the kept segment from local vertex lv that
+    // fall in this strip, as marks (the walker stops past the strip)
+    auto walk_at = [&](int64_t lv) {
+      double x0, y0, x1, y1;
+      if (!kept_segment(J, lv, x0, y0, x1, y1)) return;
+      // (cells lie within a pixel of the floors of the end points)
+      if (floor(fmax(x0, x1)) + 1.0 < (double)u.C0 || floor(fmin(x0, x1)) - 1.0 >= (double)u.tile_c1 ||
+          floor(fmax(y0, y1)) + 1.0 < (double)u.Rw || floor(fmin(y0, y1)) - 1.0 >= (double)u.strip_r1)
+        return;
+      const bool ok = walk_all_touched(x0, y0, x1, y1, J.ncols, J.nrows, u.C0, u.tile_c1, u.Rw, u.strip_r1,
+                                       [&](int64_t x, int64_t y) {
+                                         const uint32_t idx = (uint32_t)(y - u.Rw) * TW + (uint32_t)(x - u.C0);
+                                         atomicOr(&mark[idx >> 5], 1u << (idx & 31u));
+                                       });
+      if (!ok) atomicOr(a.status, 64u);
+    };
+
+    // An all-touched polygon (burn_geometry.rs:219-238): its outline cells
+    // and its fill, each pixel folded once.  Idempotent functions fold the
+    // union; sum and count (LineWriter + FillWriter over the outline's
+    // PixelCache) fold the outline cells once and the fill pixels the cache
+    // does not contain: inside the cache window that is every fill pixel off
+    // the outline, again the union; a fill pixel outside the window is
+    // skipped when the cache bit its wrapped index lands on is set.
+    auto polygon_at = [&](const ItemDesc& d, uint32_t pi) {
+      const T v = from_bits<T>(d.value);
+      for (int64_t e = lane; e < d.nv; e += 64) walk_at(d.vbeg + e - J.v_base);
+      const uint64_t cov = polygon_mask<NWORD>(a, d, u, hb_ok, hbase, pb, pe, pi, strip, stage, lane);
+      wave_sync();
+      if (q < NWORD) {
+        uint64_t m = cov;
+        if (COUNTED && (d.flags & kItemAtCache) && m) {
+          CacheWin cw;
+          item_cache(a.item_win, (int64_t)d.last, cw);
+          const uint64_t length = cw.cap / cw.width;
+          const unsigned long long* bits = a.at_bits + a.at_base[d.last];
+          const int64_t y = (int64_t)u.Rw + rl, xb = (int64_t)u.C0 + q * 64;
+          for (uint64_t mm = m; mm; mm &= mm - 1) {
+            const int b = __builtin_ctzll(mm);
+            const int64_t x = xb + b;
+            if ((uint64_t)x - (uint64_t)cw.x0 < cw.width && (uint64_t)y - (uint64_t)cw.y0 < length) continue;
+            const uint64_t ci = cache_index(cw, x, y);
+            if (ci < cw.cap && ((bits[ci >> 6] >> (ci & 63)) & 1ull)) m &= ~(1ull << b);
+          }
+        }
+        const int idx = rl * TW + q * 64;
+        mark[idx >> 5] |= (uint32_t)m;
+        mark[(idx >> 5) + 1] |= (uint32_t)(m >> 32);
+      }
+      apply_marks(v);
+    };
 
     auto polygon = [&](const ItemDesc& d, uint32_t pi) {
       const T v = from_bits<T>(d.value);
@@ -450,8 +540,15 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
           T* p = acc + r * TW + ccol;
           T px[PPC];
           __builtin_memcpy(px, p, 16);
+          if (FSUM && unord) {
 #pragma unroll
-          for (int j = 0; j < PPC; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (bits >> j) & 1u);
+            for (int j = 0; j < PPC; ++j) px[j] = ((bits >> j) & 1u) ? px[j] + v : px[j];
+            const int idx = r * TW + ccol;
+            if (bits) atomicOr(&wr[idx >> 5], bits << (idx & 31));
+          } else {
+#pragma unroll
+            for (int j = 0; j < PPC; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (bits >> j) & 1u);
+          }
           __builtin_memcpy(p, px, 16);
         }
       }
@@ -468,16 +565,22 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
     auto lines = [&](const ItemDesc& d, const uint4* recs, uint32_t n_r, bool banded, uint4 pre) {
       const int32_t kind = (int32_t)(d.kind_layer & 3u);
       const T v = from_bits<T>(d.value);
-      const bool dedup = kind == 1 && dedup_job;
+      const bool dedup = kind == 1 && (dedup_job || at_job);
       // a write is a mark (one fold per pixel: idempotent functions, or a
-      // line's first write per pixel with xres != yres) or a count; both are
-      // LDS atomics whose result is not waited for
+      // line's first write per pixel with xres != yres or all-touched) or a
+      // count; both are LDS atomics whose result is not waited for
       const bool marked = !COUNTED || dedup;
       auto emit = [&](auto cx, auto cy) {
         if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
         const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
-        if (marked) atomicOr(&mark[idx >> 5], 1u << (idx & 31u));
-        else atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
+        if (FSUM && unord) {
+          ++burns;
+          add1(idx, v);
+        } else if (marked) {
+          atomicOr(&mark[idx >> 5], 1u << (idx & 31u));
+        } else {
+          atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
+        }
       };
       // end cell of the last kept edge of an open line (burners.rs:86-89)
       auto end_cell = [&](const uint4& r, uint32_t e) {
@@ -492,6 +595,10 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
           if (r.x == kLsNone) continue;
           if (kind == 2) {  // a vertex of a multi-point: its cell
             emit((int32_t)r.x, (int32_t)r.y);
+            continue;
+          }
+          if (at_job) {  // the all-touched walker, from the world coordinates
+            walk_at(d.vbeg + (int64_t)e - J.v_base);
             continue;
           }
           if (r.x == kLsFar) {
@@ -524,7 +631,7 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
             if (end_cell(r, e)) emit(ix1, iy1);
           }
         }
-        if (COUNTED && !marked) {
+        if (COUNTED && !marked && !unord) {
           // each lane folds its PPL pixels k times each and clears their counters
           wave_sync();
 #pragma unroll
@@ -549,26 +656,7 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
           wave_sync();
         }
       }
-      if (marked) {
-        // each lane folds its marked pixels once and clears their marks
-        wave_sync();
-#pragma unroll
-        for (int c0 = 0; c0 < PPL; c0 += 16) {
-          const int base = lane * PPL + c0;
-          const uint32_t bits = (mark[base >> 5] >> (base & 31)) & 0xFFFFu;
-          if (bits) {
-            burns += (unsigned long long)__popc(bits);
-            T px[16];
-            __builtin_memcpy(px, acc + base, sizeof(px));
-#pragma unroll
-            for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (bits >> j) & 1u);
-            __builtin_memcpy(acc + base, px, sizeof(px));
-          }
-        }
-        wave_sync();
-        for (int w = lane; w < NPIXS / 32; w += 64) mark[w] = 0u;
-        wave_sync();
-      }
+      if (marked && !unord) apply_marks(v);
     };
 
     // the records of a line or multi-point item in this strip: base, count,
@@ -592,7 +680,8 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
     auto process_item = [&](const ItemDesc& d, uint32_t pi) {
       if (d.r1 <= u.Rw || d.r0 >= u.strip_r1 || d.c1 <= u.C0 || d.c0 >= u.tile_c1) return;
       if ((d.kind_layer & 3u) == 0u) {
-        polygon(d, pi);
+        if (at_job) polygon_at(d, pi);
+        else polygon(d, pi);
       } else {
         const uint4* recs;
         uint32_t n_r;
@@ -615,8 +704,10 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
         uint32_t pidx = 0, rn = 0;
         uint64_t pv = 0;
         const uint4* rp = nullptr;
+        ItemDesc dl;
+        __builtin_memset(&dl, 0, sizeof(dl));
         if ((uint32_t)lane < nb) {
-          const ItemDesc dl = load_desc(pdesc_c + i0 + (uint32_t)lane);
+          dl = load_desc(pdesc_c + i0 + (uint32_t)lane);
           const uint32_t kind = dl.kind_layer & 3u;
           const bool vis = !(dl.r1 <= u.Rw || dl.r0 >= u.strip_r1 || dl.c1 <= u.C0 || dl.c0 >= u.tile_c1);
           pt = vis && kind == 2u && dl.nv == 1;
@@ -645,6 +736,14 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
             // the run of points up to the next line or polygon, in order
             const uint64_t other = todo & ~ptm & (~0ull << p);
             const uint64_t run = todo & ptm & (other ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull);
+            if (FSUM && unord) {
+              if ((run >> lane) & 1ull) {
+                ++burns;
+                add1(pidx, from_bits<T>(pv));
+              }
+              todo &= ~run;
+              continue;
+            }
             uint64_t m = run;
             while (m) {
               const int b = __builtin_ctzll(m);
@@ -661,7 +760,7 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
             continue;
           }
           todo &= todo - 1;
-          const ItemDesc d = load_desc(pdesc_c + i0 + (uint32_t)p);
+          const ItemDesc d = bcast_desc(dl, p);  // lane p's copy, no reload
           if ((lnm >> p) & 1ull) {
             const uint4 cur = pre;
             lrest &= lrest - 1;
@@ -671,6 +770,8 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
             const uint32_t n_r = (uint32_t)__builtin_amdgcn_readlane((int)rn, p);
             const bool banded = __builtin_amdgcn_readlane((int)bnd, p) != 0;
             lines(d, recs, n_r, banded, cur);
+          } else if (at_job) {
+            polygon_at(d, i0 + (uint32_t)p);
           } else {
             polygon(d, i0 + (uint32_t)p);
           }
@@ -703,8 +804,16 @@ __global__ __launch_bounds__(256) void k_tile_lp(TileArgs a) {
     // ---- write the strip once from LDS
     wave_sync();
     store_strip<T, PPC, RPL, RSTEP>(u, srow, ccol, [&](int k) {
+      const int idx = (srow + k * RSTEP) * TW + ccol;
+      T px[PPC];
+      __builtin_memcpy(px, acc + idx, 16);
+      if (FSUM && unord) {  // pixels with no write keep the background
+        const uint32_t w = wr[idx >> 5] >> (idx & 31);
+#pragma unroll
+        for (int j = 0; j < PPC; ++j) px[j] = ((w >> j) & 1u) ? px[j] : bg;
+      }
       u32x4 vec;
-      __builtin_memcpy(&vec, acc + (srow + k * RSTEP) * TW + ccol, 16);
+      __builtin_memcpy(&vec, px, 16);
       return vec;
     });
     wave_sync();  // the next strip's background overwrites these pixels

@@ -142,6 +142,8 @@ class Job:
     layers: DeviceBuffer | None = None
     n_layers: int = 1
     background: float = 0
+    sum_mode: int = 0  # rz_sum_mode: 0 ordered (exact), 1 unordered (float sums of lines and points)
+    all_touched: bool = False
 
     @property
     def shape(self) -> tuple[int, int, int]:
@@ -164,6 +166,8 @@ class Job:
         j.n_layers = self.n_layers if self.layers is not None else 1
         j.background_bits = value_bits(self.background, self.dtype)
         j.out_dev = self.out.ptr if self.out is not None else None
+        j.sum_mode = self.sum_mode
+        j.all_touched = int(bool(self.all_touched))
         return j
 
     def result(self) -> np.ndarray:
@@ -179,7 +183,7 @@ def rasterize_device(ctx: Context, job: Job) -> None:
         ctx.ptr, job.geometry.ptr, byref(job.info), _lib.dtype_code(job.dtype), _lib.fun_code(job.fun),
         job.field.ptr if job.field is not None else None, None if job.field is not None else scalar.ctypes.data,
         job.layers.ptr if job.layers is not None else None, job.n_layers if job.layers is not None else 1,
-        bg.ctypes.data, 0, 0, job.out.ptr))
+        bg.ctypes.data, int(bool(job.all_touched)), job.sum_mode, job.out.ptr))
 
 
 def batch_rasterize(ctx: Context, jobs: list[Job]) -> None:

@@ -17,6 +17,7 @@ import pytest
 
 from oracle import oracle
 from rusterize_amd import _lib
+from rusterize_amd._lib import Geometry
 
 POINTER = ctypes.POINTER
 
@@ -137,3 +138,52 @@ def test_row_coverage_matches_sorted_spans(width, seed):
             seg = ref[C0:C0 + width]
             want[:len(seg)] = seg
             np.testing.assert_array_equal(got[:len(seg)], want[:len(seg)], err_msg=f"xs={xs} C0={C0}")
+
+
+# ------------------------------------------------ all-touched walker (device form)
+
+def _at_cells(x0, y0, x1, y1, ncols, nrows, X0, X1, Y0, Y1, cap=1 << 16):
+    xs = np.empty(cap, dtype=np.int64)
+    ys = np.empty(cap, dtype=np.int64)
+    n = _lib.lib().rz_debug_walk_all_touched(x0, y0, x1, y1, ncols, nrows, X0, X1, Y0, Y1,
+                                              xs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                              ys.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap)
+    assert n >= 0
+    return list(zip(ys[:n].tolist(), xs[:n].tolist()))
+
+
+def test_all_touched_walker_equals_the_oracle():
+    # the tile kernel's walker (walk_all_touched) against the oracle's
+    # restatement of burners.rs:115-246, cell for cell and in order, on whole
+    # rasters and on windows of them
+    rng = np.random.default_rng(23)
+    ncols, nrows = 37, 29
+    info = oracle.make_info(ncols, nrows, 0.0, float(ncols), 0.0, float(nrows), 1.0, 1.0)
+    cases = []
+    for _ in range(400):
+        X0, X1 = rng.uniform(-8, ncols + 8, 2)
+        Y0, Y1 = rng.uniform(-8, nrows + 8, 2)
+        kind = rng.integers(0, 6)
+        if kind == 1:
+            X1 = X0 + rng.uniform(-0.009, 0.009)      # vertical branch
+        elif kind == 2:
+            Y1 = Y0 + rng.uniform(-0.009, 0.009)      # horizontal branch
+        elif kind == 3:
+            X0, Y0 = float(np.round(X0)), float(np.round(Y0))  # grid points
+            X1, Y1 = float(np.round(X1)), float(np.round(Y1))
+        elif kind == 4:
+            X1 = X0 + (Y1 - Y0)                       # 45 degrees
+        cases.append((float(X0), float(Y0), float(X1), float(Y1)))
+    for X0, Y0, X1, Y1 in cases:
+        g = Geometry.from_wkt([f"LINESTRING ({X0!r} {Y0!r}, {X1!r} {Y1!r})"])
+        want = [tuple(c) for c in oracle.trace(g, info, "last", all_touched=True)[:, :2].tolist()]
+        px0, py0, px1, py1 = X0, float(nrows) - Y0, X1, float(nrows) - Y1
+        if not (min(px0, px1) < ncols and max(px0, px1) >= 0 and min(py0, py1) < nrows and max(py0, py1) >= 0):
+            assert want == []  # extract_line's box test drops it (edges.rs:130)
+            continue
+        assert _at_cells(px0, py0, px1, py1, ncols, nrows, 0, ncols, 0, nrows) == want, (X0, Y0, X1, Y1)
+        for _ in range(3):
+            wx0, wy0 = int(rng.integers(0, ncols)), int(rng.integers(0, nrows))
+            wx1, wy1 = wx0 + int(rng.integers(1, 20)), wy0 + int(rng.integers(1, 20))
+            sub = [(r, c) for r, c in want if wx0 <= c < wx1 and wy0 <= r < wy1]
+            assert _at_cells(px0, py0, px1, py1, ncols, nrows, wx0, wx1, wy0, wy1) == sub, (X0, Y0, X1, Y1)

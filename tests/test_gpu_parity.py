@@ -260,6 +260,42 @@ def test_line_point_kernel_paths(binning, monkeypatch):
             assert_same(got, ref, f"line paths {binning} {dtype} {fun} xres={xres}")
 
 
+def sum_error_bound(g, info, dtype, field):
+    """8 n eps sum|v| per pixel: the rounding of n additions in any order,
+    from the oracle's count of writes and its sum of |v|."""
+    eps = float(np.finfo(np.dtype(dtype)).eps)
+    n = oracle.rasterize(g, oracle.copy_info(info), "float64", "count", scalar=1, background=0)
+    s = oracle.rasterize(g, oracle.copy_info(info), "float64", "sum", field=np.abs(field).astype(np.float64),
+                         background=0)
+    return 8.0 * n * eps * s
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_unordered_sum_mode_within_its_bound(dtype):
+    # RZ_SUM_UNORDERED on the lines / points / collections workload (C3
+    # reduced): every pixel within 8 n eps sum|v| of the reference's ordered
+    # fold (f64: below 1e-12 relative wherever the sum does not cancel), the
+    # same written / unwritten pixels; the ordered mode stays bit-exact
+    g = Geometry.synth(3, 3, 0, 2000.0, 2000.0, 5000, 20000, 500)
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 2000, 2000))
+    field = np.random.default_rng(17).standard_normal(g.n_geoms).astype(dtype)
+    ref = oracle.rasterize(g, oracle.copy_info(info), dtype, "sum", field=field, background=float("nan"))
+    ordered = _lib.rasterize(g, info, dtype, "sum", field, None, None, 1, float("nan"), sum_mode=0)
+    assert_same(ordered, ref, f"ordered sum {dtype}")
+    got = _lib.rasterize(g, info, dtype, "sum", field, None, None, 1, float("nan"), sum_mode=1)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    err = np.abs(got[ok].astype(np.float64) - ref[ok].astype(np.float64))
+    bound = sum_error_bound(g, info, dtype, field)[ok]
+    assert (err <= bound).all(), float((err - bound).max())
+    if dtype == "float64":
+        # where the sum keeps a tenth of sum|v| (no heavy cancellation): 1e-12 relative
+        absum = sum_error_bound(g, info, dtype, field)[ok] / (8.0 * np.finfo(np.float64).eps)
+        n = oracle.rasterize(g, oracle.copy_info(info), "float64", "count", scalar=1, background=0)[ok]
+        keep = np.abs(ref[ok]) >= 0.1 * absum / np.maximum(n, 1)
+        assert (err[keep] <= 1e-12 * np.abs(ref[ok][keep])).all()
+
+
 def test_far_line_segments():
     # a segment whose floored end point lies beyond 2^30 pixels is walked in
     # 128 bits from its world coordinates; its line is never banded

@@ -237,6 +237,12 @@ rz_status rz_device_geometry_upload(rz_context* ctx, const rz_geometry* host,
                                     rz_device_geometry** out);
 rz_status rz_device_geometry_free(rz_device_geometry* dg);
 int64_t rz_device_geometry_count(const rz_device_geometry* dg);
+/* Replaces the coordinates (n_vertices x,y pairs, the set's own count) of a
+ * resident set in place, keeping its topology; returns once the copy is
+ * done.  Captured batches over the set read the new coordinates on their
+ * next replay.  (No counterpart in the reference, whose geometries are
+ * parsed per call.) */
+rz_status rz_device_geometry_set_xy(rz_device_geometry* dg, const double* xy, int64_t n_vertices);
 /* Job k of the seeded job stream (see rz_synth_job_spec), generated on the
  * device into a resident geometry set. */
 rz_status rz_device_synth_job(rz_context* ctx, uint64_t seed, int64_t k, rz_device_geometry** out);

@@ -99,6 +99,7 @@ _SIGNATURES = [
     ("rz_device_geometry_upload", c_int, [c_void_p, POINTER(RzGeometry), POINTER(c_void_p)]),
     ("rz_device_geometry_free", c_int, [c_void_p]),
     ("rz_device_geometry_count", c_int64, [c_void_p]),
+    ("rz_device_geometry_set_xy", c_int, [c_void_p, c_void_p, c_int64]),
     ("rz_device_synth_job", c_int, [c_void_p, c_uint64, c_int64, POINTER(c_void_p)]),
     ("rz_rasterize_device", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),

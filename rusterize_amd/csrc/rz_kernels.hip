@@ -117,7 +117,11 @@ __device__ inline void reduce_box(int64_t item, ItemBox b, uint64_t stat, bool l
   }
   int64_t nit = __shfl_down(it, 1, 64);
   bool tail = (lane == 63) || (nit != it);
-  if (active && tail && b.r0 < b.r1 && b.c0 < b.c1) {
+  // A segment's box is merged whenever some edge set it, even when its own
+  // column range is empty: a polygon edge left or right of the raster
+  // clamps to an empty range, and the item's other edges, possibly in
+  // another wave, widen the merged box to the span between them.
+  if (active && tail && b.r0 < b.r1) {
     atomicMin(&out[item].r0, b.r0);
     atomicMax(&out[item].r1, b.r1);
     atomicMin(&out[item].c0, b.c0);
@@ -669,7 +673,14 @@ __global__ void k_bin_emit(BinArgs a) {
   if (!tile_rect(d.r0, d.r1, d.c0, d.c1, J.tile_w, tx0, tx1, ty0, ty1)) return;
   uint64_t o = a.offsets[i];
   if (a.offsets[i + 1] > a.cap) {  // capacity of a captured batch exceeded
+    // the one item whose range crosses the capacity fills its slots below it
+    // with keys past every tile, so every slot the fixed-capacity sort reads
+    // is written in this pass (no stale key reaches the later kernels)
     atomicOr(a.status, 4u);
+    for (; o < a.cap; ++o) {
+      a.keys[o] = 0xFFFFFFFFu;
+      a.vals[o] = 0xFFFFFFFFu;
+    }
     return;
   }
   const uint32_t base = J.tile_base + (d.kind_layer >> 2) * J.tiles_per_layer;

@@ -371,6 +371,20 @@ extern "C" rz_status rz_device_geometry_free(rz_device_geometry* dg) {
 
 extern "C" int64_t rz_device_geometry_count(const rz_device_geometry* dg) { return dg ? dg->G : 0; }
 
+// New coordinates for a resident set of the same topology (its derived maps
+// depend on the ring and item structure only), copied on the stream of the
+// context that uploaded it and waited for; captured batches over the set
+// read them on their next replay.
+extern "C" rz_status rz_device_geometry_set_xy(rz_device_geometry* dg, const double* xy, int64_t n_vertices) {
+  if (!dg || (!xy && n_vertices > 0)) return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_device_geometry_set_xy");
+  if (n_vertices != dg->V) return rz::fail(RZ_VALUE_ERROR, "vertex count differs from the resident geometry's");
+  if (n_vertices == 0) return RZ_OK;
+  HIP_TRY(hipSetDevice(dg->ctx->device));
+  HIP_TRY(hipMemcpyAsync(dg->xy, xy, (size_t)n_vertices * 16, hipMemcpyHostToDevice, dg->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(dg->ctx->stream));
+  return RZ_OK;
+}
+
 static rz_status finish_geometry(rz_context* c, rz_device_geometry* dg, rz_device_geometry** out);
 
 extern "C" rz_status rz_device_geometry_upload(rz_context* c, const rz_geometry* g,
@@ -1198,7 +1212,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                          sorted_keys, sorted_vals, (const ItemDesc*)c->pair_desc.p, d_total, n_pairs,
                          (const uint32_t*)c->tile_off.p, (const uint64_t*)c->heavy_base.p,
                          (const uint64_t*)c->offsets.p, (uint32_t*)c->wide_j.p, (uint32_t*)c->cover_list.p,
-                         cover_count);
+                         cover_count, n_tiles);
       CoverArgs ca{};
       ca.jobs = d_jobs;
       ca.job_tend = d_tend;

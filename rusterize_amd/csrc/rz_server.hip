@@ -94,8 +94,8 @@ struct rz_server {
   std::vector<std::thread> workers;
 
   // Output buffers come in size classes (four per power of two) so released
-  // outputs are reused by later requests of similar size: fresh allocations,
-  // and the hipFree of cached ones (which synchronises the device), stay rare.
+  // outputs are reused by later requests of similar size: fresh allocations
+  // and the eviction of cached ones stay rare.
   static size_t size_class_bytes(size_t bytes) {
     size_t c = 1u << 20;
     while (c < bytes) c <<= 1;  // next power of two >= 1 MiB
@@ -263,6 +263,20 @@ extern "C" rz_status rz_server_create(int device, uint64_t memory_budget, int n_
   }
   s->ws.assign((size_t)n_streams, 0);
   s->grow.assign((size_t)n_streams, 0);
+  // Outputs and workspaces come from the device's default memory pool in
+  // stream order.  A pool returns freed memory to the system at every
+  // synchronisation by default, so the next large allocation maps it again
+  // (a stall of many milliseconds under load): keep up to the budget
+  // reserved in the pool instead.
+  hipMemPool_t pool = nullptr;
+  if (hipSetDevice(device) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+    uint64_t thr = 0;
+    (void)hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    if (thr < memory_budget) {
+      thr = memory_budget;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+  }
   for (int k = 0; k < n_streams; ++k) s->workers.emplace_back([s, k] { s->work(k); });
   *out = s;
   return RZ_OK;

@@ -45,13 +45,15 @@ __global__ void k_heavy_list(const uint32_t* heavy_flag, const uint32_t* heavy_r
 __global__ void k_heavy_pairs(const JobDev* jobs, const uint32_t* pair_tile, const uint32_t* pair_item,
                               const ItemDesc* pair_desc, const uint64_t* n_pairs, uint64_t cap,
                               const uint32_t* tile_off, const uint64_t* heavy_base, const uint64_t* pair_off,
-                              uint32_t* wide_j, uint32_t* cover_list, uint32_t* cover_count) {
+                              uint32_t* wide_j, uint32_t* cover_list, uint32_t* cover_count, uint32_t n_tiles) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool poly = false;
   if (i < min(*n_pairs, cap)) {
     const uint32_t t = pair_tile[i];
-    if (heavy_base[t + 1] > heavy_base[t]) {
+    // (a key past every tile: a slot of a captured batch whose pairs outgrew
+    // its capacity, whose tile lists are then all empty)
+    if (t < n_tiles && heavy_base[t + 1] > heavy_base[t]) {
       const ItemDesc& d = pair_desc[i];
       poly = (d.kind_layer & 3u) == 0u && !(d.flags & kItemWide);  // k_wide_cover writes those
       int32_t tx0, tx1, ty0, ty1;

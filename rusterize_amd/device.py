@@ -123,6 +123,13 @@ class DeviceGeometry:
         check(ctx.L.rz_device_synth_job(ctx.ptr, seed, k, byref(self.ptr)))
         return self
 
+    def set_xy(self, xy: np.ndarray) -> None:
+        """New coordinates for the same topology (rz_device_geometry_set_xy)."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1)
+        if xy.size != 2 * self.n_vertices:
+            raise ValueError("set_xy needs one (x, y) pair per resident vertex")
+        check(self.ctx.L.rz_device_geometry_set_xy(self.ptr, xy.ctypes.data, self.n_vertices))
+
     def free(self) -> None:
         if self.ptr:
             self.ctx.L.rz_device_geometry_free(self.ptr)

@@ -342,3 +342,72 @@ def test_out_of_range_layer_fails_the_batch_and_spares_other_jobs(ctx):
         ctx.synchronize()  # the error was reported once
     finally:
         b.free()
+
+
+def _star(cx, cy, rx, ry, n=200):
+    """A closed star ring of n + 1 vertices in world coordinates (the grid
+    below maps pixel row y to world y = 256 - y)."""
+    k = np.arange(n)
+    r = np.where(k % 2 == 0, 1.0, 0.45)
+    ring = np.stack([cx + rx * r * np.cos(2 * np.pi * k / n), 256.0 - (cy + ry * r * np.sin(2 * np.pi * k / n))], 1)
+    return np.vstack([ring, ring[:1]])
+
+
+def _grown_sets(b_cy, b_ry):
+    """Forty long rings (151 vertices) at rows b_cy +- b_ry, then one ring of
+    10,001 vertices over rows 68..124; the same topology whatever b_cy, b_ry."""
+    from tests.test_gpu_parity import polygon_rings
+    rings = [[_star(12 + 12 * k, b_cy, 10, b_ry, 150)] for k in range(40)]
+    rings.append([_star(256, 96, 200, 28, 10000)])
+    return polygon_rings(rings)
+
+
+@pytest.mark.parametrize("wide", [None, "always"])
+def test_graph_replay_past_band_and_mask_capacities(ctx, wide, monkeypatch):
+    # a batch captured while forty long rings were flat (one 16-row strip:
+    # no edge bands, one strip of wide masks) and one large ring was banded,
+    # replayed after the forty grew to two strips inside the same tiles:
+    # their band entries outgrow the captured capacity, so the rings past it
+    # (the large one among them) fall back to their whole edge list, and with
+    # every polygon on the wide pre-pass the masks outgrow theirs too, so
+    # the items past it take per-tile coverage; the replay still equals the
+    # oracle.  Growing into more tiles outgrows the pairs and fails the
+    # replay with a capacity error.
+    from rusterize_amd.device import BatchGraph
+    if wide:
+        monkeypatch.setenv("RUSTERIZE_WIDE", wide)
+    monkeypatch.setenv("RUSTERIZE_BINNING", "always")  # 41 items would be scanned, with no pairs
+    flat, tall, taller = _grown_sets(8.0, 3.0), _grown_sets(16.0, 12.0), _grown_sets(100.0, 95.0)
+    info = grid(512, 256, 0, 0)
+    dg = DeviceGeometry(ctx, flat)
+    field = field_for("float64", flat.n_geoms, seed=77)
+    d_field = DeviceBuffer.from_array(ctx, field)
+    out = DeviceBuffer(ctx, 512 * 256 * 8)
+    jobs = [Job(dg, info, "float64", fun, out, field=d_field, background=float("nan")) for fun in ("sum",)]
+    g = None
+    try:
+        g = BatchGraph(jobs, headroom=1.0)
+        g.launch()
+        g.synchronize()
+        ref = oracle.rasterize(flat, oracle.copy_info(info), "float64", "sum", field=field, background=float("nan"))
+        assert_same(jobs[0].result(), ref, "captured (flat)")
+        dg.set_xy(tall.xy)
+        for rep in range(2):
+            g.launch()
+            g.synchronize()
+            ref = oracle.rasterize(tall, oracle.copy_info(info), "float64", "sum", field=field,
+                                   background=float("nan"))
+            assert_same(jobs[0].result(), ref, f"replay {rep} past the band / mask capacities")
+            rasterize_device(ctx, jobs[0])  # eager work between replays
+            ctx.synchronize()
+            assert_same(jobs[0].result(), ref, "eager on the grown rings")
+        dg.set_xy(taller.xy)
+        g.launch()
+        with pytest.raises(Exception, match="capture it again"):
+            g.synchronize()
+    finally:
+        if g is not None:
+            g.close()
+        d_field.free()
+        out.free()
+        dg.free()

@@ -415,3 +415,44 @@ def test_coverage_paths_agree(wide, layout, monkeypatch):
                 field = field_for(dtype, g.n_geoms, seed=11)
                 got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
                 assert_same(got, ref, f"{wide} {layout} {binning} {g.n_geoms} geoms {dtype} {fun}")
+
+
+@pytest.mark.parametrize("binning", [None, "always"])
+def test_wide_masks_past_their_cap(binning, monkeypatch):
+    # 300 polygons over a 512 x 512 raster, every one on the wide pre-pass:
+    # their row masks (about 1.2M words) exceed the cap (the larger of 2 bytes
+    # per output pixel and 2^20 words), so the items past it compute their
+    # coverage per tile; the raster is still the oracle's (a single call
+    # sizes the masks for the cap without reading totals back; binned, from
+    # the totals)
+    monkeypatch.setenv("RUSTERIZE_WIDE", "always")
+    if binning:
+        monkeypatch.setenv("RUSTERIZE_BINNING", binning)
+    rng = np.random.default_rng(12)
+    wkts = []
+    for _ in range(300):
+        x0, y0 = rng.uniform(-20, 40, 2)
+        x1, y1 = rng.uniform(470, 530, 2)
+        wkts.append(f"POLYGON (({x0} {y0}, {x1} {y0 + rng.uniform(-10, 10)}, {x1} {y1}, "
+                    f"{x0 + rng.uniform(-10, 10)} {y1}, {x0} {y0}))")
+    g = Geometry.from_wkt(wkts)
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 512, 512))
+    for dtype, fun, bg in (("float64", "sum", float("nan")), ("uint8", "count", 0)):
+        field = field_for(dtype, g.n_geoms, seed=19)
+        got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+        assert_same(got, ref, f"wide past cap {binning} {dtype} {fun}")
+
+
+def test_item_box_merged_across_waves():
+    # a polygon whose left and right edges lie outside the raster, its
+    # vertices split between two waves of the vertex pass (items before it
+    # shift it across a 64-vertex boundary): each wave's part of its box has
+    # an empty column range, and only merged do they span the raster
+    big = "POLYGON ((-12.4 0.8, 517.2 0.7, 517.2 497.3, -6.7 497.3, -12.4 0.8))"
+    for shift in (0, 12, 24, 25, 26, 38):  # 25, 38: the right edge in one wave, the left edge in the next
+        wkts = [sq(10 + k, 10, 11 + k, 11) for k in range(shift)] + [big]
+        g = Geometry.from_wkt(wkts)
+        info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 512, 512))
+        field = np.arange(1, g.n_geoms + 1, dtype=np.float64)
+        got, ref = run_both(g, info, "float64", "count", field=field, bg=0)
+        assert_same(got, ref, f"shift {shift}")

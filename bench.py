@@ -276,9 +276,10 @@ def measure(wl: Workload, steps: int, warmup: int, with_cpu: bool, only: str | N
         release()
         return res
 
-    # HIP events around every stage of every call, recorded on the context's
-    # stream and resolved after the timed region (no added synchronisation)
-    check(L.rz_context_set_profiling(ctx, 1))
+    # HIP events around the tile launches of every call (profiling level 3:
+    # two events per call on the context's stream, resolved after the timed
+    # region; no added synchronisation)
+    check(L.rz_context_set_profiling(ctx, 3))
     for _ in range(warmup):
         step()
     check(L.rz_context_synchronize(ctx))
@@ -291,8 +292,14 @@ def measure(wl: Workload, steps: int, warmup: int, with_cpu: bool, only: str | N
     check(L.rz_context_synchronize(ctx))
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / steps
-    stages = stage_times(steps)  # ms per step, summed over the step's calls
-    tile_ms = stages["tile"]
+    tile_ms = stage_times(steps)["tile"]  # ms per step, summed over the step's calls
+    # the stage breakdown: events around every stage, on more passes (untimed)
+    check(L.rz_context_set_profiling(ctx, 1))
+    L.rz_context_reset_stage_times(ctx)
+    n_break = min(steps, 20)
+    for _ in range(n_break):
+        step()
+    stages = stage_times(n_break)
 
     # per-dtype breakdown (untimed)
     by_dtype = {}
@@ -337,7 +344,7 @@ def measure(wl: Workload, steps: int, warmup: int, with_cpu: bool, only: str | N
                 by_dtype[dt]["traffic_over_alg"] = round(traffic[dt] / wl.b_alg_tile(np.dtype(dt).itemsize), 4)
     tile_bytes = sum(wl.b_alg_tile(np.dtype(dt).itemsize) for dt, _ in wl.calls)
     call_bytes = sum(wl.b_alg(np.dtype(dt).itemsize) for dt, _ in wl.calls)
-    call_ms = sum(stages.values())
+    call_ms = ms_per_step  # the whole calls, host launch included (wall clock over the timed steps)
     achieved = tile_bytes / (tile_ms * 1e-3)
 
     cpu, parity = None, None

@@ -391,7 +391,8 @@ rz_status rz_server_get_stats(rz_server* server, rz_server_stats* out);
 
 /* Profiling level: 0 off; 1 records HIP events around every stage of every
  * rz_rasterize_device call on the context's stream (no added
- * synchronisation); 2 also counts exact pixel-burns.
+ * synchronisation); 2 also counts exact pixel-burns; 3 records only the two
+ * events around the tile launches (the other stages read 0).
  * rz_context_stage_times waits for the recorded calls and writes up to n
  * per-stage sums (milliseconds since the last reset) in the order given by
  * rz_stage_name(i); returns the number of stages. */

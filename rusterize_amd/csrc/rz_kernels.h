@@ -97,6 +97,8 @@ struct JobDev {
   int32_t scan;              // 1: no bins, the tile kernel tests every item of the job
   int32_t sum_unordered;     // fun = sum, float output: lines and points add with LDS atomics
   int32_t all_touched;       // AllTouched strategy: lines walked cell by cell, polygons with their outline
+  int32_t scan_compact;      // scan job of one band whose strip and tile indices fit 16 bits: its strips
+                             // test the 8-byte item boxes of scan_box
   const void* field;         // [n_geoms] of the batch dtype, or null
   uint64_t scalar_bits, bg_bits;
   const uint32_t* layer_of_geom;  // may be null
@@ -317,6 +319,7 @@ struct DescArgs {
   uint32_t* wide_words;     // [n_items] row-mask words of a wide item
   uint32_t* status;         // bit 4 set when a geometry's layer index is >= the job's n_layers
   const unsigned long long* item_win;  // all-touched polygons' outline boxes (null: none)
+  uint2* scan_box;          // [n_items] boxes of items of scan_compact jobs (or null), see ScanBox
   uint32_t* at_words;       // [n_items] words of the PixelCache an all-touched dedup polygon needs
                             // in global memory (its fill reaches beyond the cache box), or null
 };
@@ -403,6 +406,21 @@ struct BinArgs {
 constexpr int kQueueStride = 32;  // u32 words between counters (one 128-B line each)
 constexpr int kQueueWords = NUM_XCD * 4 * kQueueStride;
 
+// An item box in units of 16-row strips and tiles, both ends inclusive, for
+// the box tests of scan jobs: x = first strip | last strip << 16, y = first
+// tile column | last tile column << 16 (an empty box: first strip 0xFFFF,
+// last 0).  A strip meets the item iff its strip and tile-column indices lie
+// in both ranges, which is the pixel-box test (rows [r0, r1), columns
+// [c0, c1)) in those units.
+__host__ __device__ inline uint2 scan_box_of(int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t tw) {
+  if (!(r0 < r1 && c0 < c1)) return make_uint2(0xFFFFu, 0u);
+  return make_uint2((uint32_t)(r0 / WAVE_ROWS) | ((uint32_t)((r1 - 1) / WAVE_ROWS) << 16),
+                    (uint32_t)(c0 / tw) | ((uint32_t)((c1 - 1) / tw) << 16));
+}
+__host__ __device__ inline bool scan_box_meets(uint2 b, uint32_t strip, uint32_t tx) {
+  return (b.x & 0xFFFFu) <= strip && strip <= (b.x >> 16) && (b.y & 0xFFFFu) <= tx && tx <= (b.y >> 16);
+}
+
 struct TileArgs {
   const JobDev* jobs;
   const int64_t* job_tend;  // [n_jobs] prefix ends of the tile ranges
@@ -413,6 +431,7 @@ struct TileArgs {
   // binned jobs: pairs sorted by tile, tile_off[t]..tile_off[t+1]
   const uint32_t* tile_off;  // [n_tiles + 1], or null when no job is binned
   const ItemDesc* pair_desc;  // descriptors in pair order
+  const uint2* scan_box;      // [n_items] compact boxes of scan_compact jobs' items
   const uint64_t* band_off;   // edge bands of banded polygon and line items
   const PolyEdge* band_edges;
   const uint4* band_segs;

@@ -414,6 +414,7 @@ __global__ void k_item_desc(DescArgs a) {
   d.band = 0;
   d.flags = 0;
   d.wide = 0;
+  if (a.scan_box && J.scan_compact) a.scan_box[i] = scan_box_of(d.r0, d.r1, d.c0, d.c1, J.tile_w);
   if (kind == 0 && (S_raw & kItemOddBit)) d.flags |= kItemOddRows;
   if (a.at_words && a.at_words[i]) d.flags |= kItemAtCache;
   a.desc[i] = d;

@@ -73,6 +73,7 @@ inline unsigned blocks_for(int64_t n, int bs = 256) {
 
 struct EventSet {
   hipEvent_t ev[kStages + 1] = {};
+  bool tile_only = false;  // only the tile stage's two events were recorded
 };
 
 struct rz_context {
@@ -84,7 +85,7 @@ struct rz_context {
       heavy_cnt, heavy_base, masks, scan_tmp, scan_tmp2, scan_tmp3, scan_tmp4,
       sort_hist, sort_off, sort_tmp, cover_list, wide_units, wide_words, wide_ubase, wide_wbase,
       wide_item, wide_masks, scan_tmp5, scan_tmp6, band_items, heavy_list, scan_tmp7, wide_j, band_segs,
-      item_win, at_words, at_base, at_bits, scan_tmp8;
+      item_win, at_words, at_base, at_bits, scan_tmp8, scan_box;
   // pinned: [0] pairs, [1] status, [2] band entries, [3] banded edges, [4] wide units, [5] wide words,
   // [6] all-touched PixelCache words
   uint64_t* host_misc = nullptr;
@@ -98,7 +99,8 @@ struct rz_context {
   bool tab_pending = false;
   const void* tab_dev = nullptr;
   size_t tab_dev_bytes = 0;
-  // profiling: 0 off, 1 stage events, 2 stage events + exact burn counts
+  // profiling: 0 off, 1 stage events, 2 stage events + exact burn counts,
+  // 3 the tile stage's two events only (the least added to a timed call)
   int profiling = 0;
   hipEvent_t ev[kStages + 1] = {};  // the current call's events (taken from the pool)
   std::vector<EventSet> ev_pool, ev_pending;
@@ -114,7 +116,7 @@ struct rz_context {
             &masks, &scan_tmp, &scan_tmp2, &scan_tmp3, &scan_tmp4,
             &sort_hist, &sort_off, &sort_tmp, &cover_list, &wide_units, &wide_words, &wide_ubase,
             &wide_wbase, &wide_item, &wide_masks, &scan_tmp5, &scan_tmp6, &band_items, &heavy_list, &scan_tmp7,
-            &wide_j, &band_segs, &item_win, &at_words, &at_base, &at_bits, &scan_tmp8};
+            &wide_j, &band_segs, &item_win, &at_words, &at_base, &at_bits, &scan_tmp8, &scan_box};
   }
   // Takes an event set for the next call (events are recorded on the stream
   // and resolved later, so timing adds no synchronisation to the call).
@@ -129,6 +131,7 @@ struct rz_context {
     }
     cur = ev_pool.back();
     ev_pool.pop_back();
+    cur.tile_only = profiling == 3;
     for (int i = 0; i <= kStages; ++i) ev[i] = cur.ev[i];
     return hipSuccess;
   }
@@ -136,7 +139,7 @@ struct rz_context {
   void resolve_times() {
     for (auto& s : ev_pending) {
       if (hipEventSynchronize(s.ev[kStages]) == hipSuccess)
-        for (int i = 0; i < kStages; ++i) {
+        for (int i = s.tile_only ? kStages - 1 : 0; i < kStages; ++i) {
           float t = 0.f;
           if (hipEventElapsedTime(&t, s.ev[i], s.ev[i + 1]) == hipSuccess) acc_ms[i] += (double)t;
         }
@@ -284,7 +287,7 @@ extern "C" rz_status rz_context_synchronize(rz_context* c) {
 }
 
 extern "C" rz_status rz_context_set_profiling(rz_context* c, int level) {
-  c->profiling = level < 0 ? 0 : (level > 2 ? 2 : level);
+  c->profiling = level < 0 ? 0 : (level > 3 ? 3 : level);
   return RZ_OK;
 }
 
@@ -751,6 +754,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   bool any_poly_band = false, any_line_band = false;
   bool any_tw64 = false, any_tw256 = false;
   bool any_at_cache = false;  // all-touched sum / count polygons: PixelCache windows
+  bool any_wide_job = false;  // jobs whose polygons may take the wide pre-pass
+  bool any_scan_compact = false;
   int max_tile_w = 64;
   for (size_t k = 0; k < n; ++k) {
     const rz_job& j = jobs_in[order[k]];
@@ -771,12 +776,18 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     else if (binning == 1) scan = ni == 0;
     else if (binning == -1) scan = true;
     J.scan = scan ? 1 : 0;
+    J.scan_compact = scan && nl == 1 && (nrows + WAVE_ROWS - 1) / WAVE_ROWS < 0xFFFF && J.ntx < 0xFFFF;
+    any_scan_compact |= J.scan_compact != 0;
     any_binned |= !scan && ni > 0;
     const bool lines = has_lines(j);
     any_tw64 |= TW == 64;
     any_tw256 |= TW == 256;
     max_tile_w = std::max(max_tile_w, TW);
     any_poly |= dg->has_poly;
+    // A small (scanned) job whose rings fit two 64-edge stages gains nothing
+    // from the wide pre-pass: each of its strips loads a ring's edges once
+    // either way, and the pass would add three launches to the call.
+    any_wide_job |= dg->has_poly && !(scan && dg->max_poly_nv <= kBandMinEdges);
     any_poly_band |= dg->has_poly && dg->max_poly_nv > kBandMinEdges;
     any_line_band |= dg->has_line && dg->max_line_nv > kLineBandMinSegs;
     any_band = any_poly_band || any_line_band;
@@ -821,13 +832,14 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     HIP_TRY(ensure(c->stream, c->counts, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->stream, c->offsets, (size_t)(I + 1) * 8));
   }
+  if (any_scan_compact) HIP_TRY(ensure(c->stream, c->scan_box, (size_t)std::max<int64_t>(I, 1) * 8));
   if (any_at_cache) {
     if (capture) return rz::fail(RZ_VALUE_ERROR, "all_touched jobs cannot be captured in a graph");
     HIP_TRY(ensure(c->stream, c->item_win, (size_t)std::max<int64_t>(I, 1) * 64));
     HIP_TRY(ensure(c->stream, c->at_words, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->stream, c->at_base, (size_t)(I + 1) * 8));
   }
-  const bool any_wide = any_poly && wide_mode != 1 && I > 0;
+  const bool any_wide = any_poly && wide_mode != 1 && I > 0 && (wide_mode == 2 || any_wide_job);
   // A small eager call whose only pre-pass is the wide one sizes the wide
   // buffers for the most they can hold (2 bytes per output pixel, every
   // unit at least 16 words) instead of reading the totals back: no host
@@ -845,10 +857,10 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   const int64_t* d_iend = d_vend + n;
   const int64_t* d_tend = d_iend + n;
 
-  if (c->profiling) {
-    HIP_TRY(c->take_events());
-    HIP_TRY(hipEventRecord(c->ev[0], s));
-  }
+  // (stage events: every stage at levels 1-2, only the tile stage at 3)
+  const bool stage_ev = c->profiling == 1 || c->profiling == 2;
+  if (c->profiling) HIP_TRY(c->take_events());
+  if (stage_ev) HIP_TRY(hipEventRecord(c->ev[0], s));
   // upload the job table unless the device already holds these exact bytes
   // (repeated calls with the same arguments skip the copy)
   if (!(c->tab_dev == c->jobtab.p && c->tab_dev_bytes == tab_bytes &&
@@ -881,7 +893,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     ia.last = (unsigned long long*)c->item_last.p;
     ia.n_items = I;
     ia.win = any_at_cache ? (unsigned long long*)c->item_win.p : nullptr;
-    ia.clear(c->misc.p, (c->profiling > 1 ? misc_bytes : 64) / 4);
+    ia.clear(c->misc.p, (c->profiling == 2 ? misc_bytes : 64) / 4);
     if (any_at_cache) ia.clear((uint32_t*)c->at_words.p + I, 1);
     ia.clear(c->queue.p, queue_words);
     if (any_binned) ia.clear((uint32_t*)c->counts.p + I, 1);
@@ -923,10 +935,11 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     da.status = d_status;
     da.item_win = any_at_cache ? (const unsigned long long*)c->item_win.p : nullptr;
     da.at_words = any_at_cache ? (uint32_t*)c->at_words.p : nullptr;
+    da.scan_box = any_scan_compact ? (uint2*)c->scan_box.p : nullptr;
     hipLaunchKernelGGL(k_item_desc, dim3(blocks_for(I)), dim3(256), 0, s, da);
   }
   TileArgs ta{};
-  std::memset(&ta, 0, sizeof(ta));
+  ta.scan_box = any_scan_compact ? (const uint2*)c->scan_box.p : nullptr;
   // all-touched sum / count polygons whose fill reaches past their
   // PixelCache window: their caches in global memory (one host round trip
   // for the size)
@@ -954,7 +967,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     ta.at_base = (const uint64_t*)c->at_base.p;
     ta.at_bits = words > 0 ? (const unsigned long long*)c->at_bits.p : nullptr;
   }
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], s));
+  if (stage_ev) HIP_TRY(hipEventRecord(c->ev[1], s));
 
   // 2. binning: stable sort-middle item lists per tile for binned jobs.
   // Eager calls read the pair, band-entry and band-edge totals back (one or
@@ -1145,7 +1158,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     if (capture && n_pairs > 0)
       hipLaunchKernelGGL(k_fill_keys, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s,
                          (uint32_t*)c->keys_a.p, d_total, n_pairs);
-    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], s));
+    if (stage_ev) HIP_TRY(hipEventRecord(c->ev[2], s));
     // stable LSD radix sort by tile key keeps geometry order inside each tile
     int end_bit = 1;
     while (end_bit < 32 && ((uint64_t)1 << end_bit) <= (uint64_t)n_tiles) ++end_bit;
@@ -1174,7 +1187,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
                          (ItemDesc*)c->pair_desc.p);
     ta.tile_off = (const uint32_t*)c->tile_off.p;
     ta.pair_desc = (const ItemDesc*)c->pair_desc.p;
-    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], s));
+    if (stage_ev) HIP_TRY(hipEventRecord(c->ev[3], s));
 
     // heavy tiles (kHeavyItems items or more): coverage masks precomputed in
     // parallel, one wave per (item, strip); the buffers are sized for every
@@ -1242,7 +1255,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       launch_wide(nullptr, nullptr, nullptr, nullptr, 0);
     }
   } else {
-    if (c->profiling) {
+    if (stage_ev) {
       HIP_TRY(hipEventRecord(c->ev[2], s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
     }
@@ -1258,7 +1271,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   ta.desc = (const ItemDesc*)c->desc.p;
   ta.edges = (const PolyEdge*)c->edges.p;
   ta.lseg = (const uint4*)c->lseg.p;
-  ta.burns = c->profiling > 1 ? d_burns : nullptr;
+  ta.burns = c->profiling == 2 ? d_burns : nullptr;
   ta.status = d_status;
   for (size_t g = 0; g < groups.size(); ++g) {
     if (groups[g].t_hi == groups[g].t_lo) continue;
@@ -1277,7 +1290,7 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   HIP_TRY(hipGetLastError());
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[5], s));
   if (c->profiling) c->accumulate_times();
-  c->burns_pending = c->profiling > 1;
+  c->burns_pending = c->profiling == 2;
   c->last_pairs = n_pairs;
   c->last_tiles = n_tiles;
   return RZ_OK;

@@ -119,38 +119,6 @@ __device__ inline StripUnit strip_unit(const TileArgs& a, uint32_t t, int32_t st
   return u;
 }
 
-// The items [ib, ie) of a scan job whose boxes meet the strip, in order:
-// kScanBatches batches of 64 boxes per round, their loads all issued before
-// the first test (a round costs one load latency instead of one per batch),
-// then visit(item index) for the round's hits in item order.
-constexpr int kScanBatches = 4;
-template <typename F>
-__device__ inline void scan_items(const RZ_CONST ItemDesc* desc_c, int64_t ib, int64_t ie, const StripUnit& u,
-                                  int lane, F&& visit) {
-  for (int64_t base = ib; base < ie; base += 64 * kScanBatches) {
-    BoxLayer bx[kScanBatches];
-#pragma unroll
-    for (int b = 0; b < kScanBatches; ++b) {
-      const int64_t it = base + b * 64 + lane;
-      bx[b] = it < ie ? load_box(desc_c + it) : BoxLayer{0, 0, 0, 0, 0u};
-    }
-    uint64_t hits[kScanBatches];
-#pragma unroll
-    for (int b = 0; b < kScanBatches; ++b)
-      hits[b] = __ballot((bx[b].kind_layer >> 2) == u.layer && bx[b].r0 < u.strip_r1 && bx[b].r1 > u.Rw &&
-                         bx[b].c0 < u.tile_c1 && bx[b].c1 > u.C0 && bx[b].r0 < bx[b].r1 && bx[b].c0 < bx[b].c1);
-#pragma unroll
-    for (int b = 0; b < kScanBatches; ++b) {
-      uint64_t m = hits[b];
-      while (m) {
-        const int bit = __builtin_ctzll(m);
-        m &= m - 1;
-        visit(base + b * 64 + bit);
-      }
-    }
-  }
-}
-
 // A polygon item's coverage mask word for this lane's row / quarter (lane
 // rl = lane / 4, q = lane % 4): precomputed (wide item, heavy tile) or
 // computed from its edges (banded: only the strip's band).
@@ -334,12 +302,44 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T>::value))) void k_tile(TileAr
       // which runs after this kernel and stores over this strip)
       const uint32_t pend = (hb_ok && a.heavy_split) ? pb : pe;
       for (uint32_t i = pb; i < pend; ++i) process_item(load_desc(pdesc_c + i), i);
+    } else if (J.scan_compact && a.scan_box) {
+      // scan job of one band: the 8-byte item boxes in strip and tile units,
+      // 64 at a time, the next 64 loaded while this batch's hits are
+      // processed
+      const int64_t ib = J.i_base, ie = J.i_base + J.ni;
+      const uint32_t sidx = (uint32_t)u.Rw / WAVE_ROWS, tx = (uint32_t)u.C0 / (uint32_t)TW;
+      const uint2* sb = a.scan_box;
+      uint2 nbx = ib + lane < ie ? sb[ib + lane] : make_uint2(0xFFFFu, 0u);
+      for (int64_t base = ib; base < ie; base += 64) {
+        const uint2 bx = nbx;
+        const bool valid = base + lane < ie;
+        if (base + 64 + lane < ie) nbx = sb[base + 64 + lane];
+        uint64_t m = __ballot(valid && scan_box_meets(bx, sidx, tx));
+        while (m) {
+          const int bit = __builtin_ctzll(m);
+          m &= m - 1;
+          process_item(load_desc(desc_c + base + bit), 0u);
+        }
+      }
     } else {
-      // scan job: test every item's box in order (small item counts),
-      // kScanBatches x 64 boxes per round with all their loads in flight at
-      // once, then the round's hits in order
-      scan_items(desc_c, J.i_base, J.i_base + J.ni, u, lane,
-                 [&](int64_t it) { process_item(load_desc(desc_c + it), 0u); });
+      // scan job: test every item's box in order (small item counts), 64
+      // boxes at a time, the next 64 loaded while this batch's hits are
+      // processed
+      const int64_t ib = J.i_base, ie = J.i_base + J.ni;
+      BoxLayer nbx = ib + lane < ie ? load_box(desc_c + ib + lane) : BoxLayer{0, 0, 0, 0, 0u};
+      for (int64_t base = ib; base < ie; base += 64) {
+        const BoxLayer bx = nbx;
+        const bool valid = base + lane < ie;
+        if (base + 64 + lane < ie) nbx = load_box(desc_c + base + 64 + lane);
+        const bool hit = valid && (bx.kind_layer >> 2) == u.layer && bx.r0 < u.strip_r1 && bx.r1 > u.Rw &&
+                         bx.c0 < u.tile_c1 && bx.c1 > u.C0 && bx.r0 < bx.r1 && bx.c0 < bx.c1;
+        uint64_t m = __ballot(hit);
+        while (m) {
+          const int bit = __builtin_ctzll(m);
+          m &= m - 1;
+          process_item(load_desc(desc_c + base + bit), 0u);
+        }
+      }
     }
 
     // take the next tile (and its item range) before this strip's stores
@@ -365,19 +365,17 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T>::value))) void k_tile(TileAr
 // in), one launch per pixel-function group.  Same work queue, strips, item
 // order, background fill and write-once store as k_tile, but the strip's
 // accumulator lives in wave-private LDS (16 x TW pixels), so a line or
-// point write is folded where it lands instead of through a per-item pass
-// over the whole strip:
-//  * first / last / min / max / any are idempotent for repeated writes of one
-//    value, so a line's cells are folded directly by the lane that emits
-//    them; two lanes landing on one pixel in the same instruction store the
-//    same value, and later instructions see it (a wave's LDS accesses are
-//    performed in order);
+// point write lands where it belongs instead of going through a per-item
+// pass over the whole strip:
+//  * first / last / min / max / any give the same result for one or several
+//    writes of one value, so a line's cells set LDS mark bits (atomic OR,
+//    not waited for) and after the line each lane folds its marked pixels
+//    once;
 //  * sum and count depend on the number of writes: a line's cells are
 //    counted in 16-bit LDS counters, and after the line each lane folds its
-//    pixels of the strip k times (skipped when the line left no count in the
-//    strip).  With xres != yres (LineWriter + PixelCache, burn_geometry.rs:
-//    179-185, 202-208) or all-touched a pixel is written once per line: an
-//    LDS mark bit, folded once after the line;
+//    pixels k times.  With xres != yres (LineWriter + PixelCache,
+//    burn_geometry.rs:179-185, 202-208) or all-touched a pixel is written
+//    once per line: a mark bit, folded once;
 //  * the unordered float sum adds every write with an LDS atomic;
 //  * runs of one-vertex point items are folded in geometry order by one
 //    lane each, from descriptors loaded 64 at a time (the point's cell is its
@@ -386,6 +384,8 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T>::value))) void k_tile(TileAr
 //    all-touched polygon folds its outline cells and its fill once each.
 // Segments come as 16-byte records with floored end points (k_prep); a line
 // item of many segments over several strips reads only its band of the strip.
+// (Folding idempotent writes directly, lane by lane, measured 13 % slower
+// than the marks: the read-modify-write per cell serialises the walk.)
 template <typename T, int FUN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_lp(TileArgs a) {
   constexpr int TW = tile_w(sizeof(T));
@@ -495,9 +495,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     };
 
     // all-touched: the cells of the kept segment from local vertex lv that
-    // fall in this strip, each given to put(strip pixel index) (the walker
-    // stops past the strip)
-    auto walk_at = [&](int64_t lv, auto&& put) {
+    // fall in this strip, as marks (the walker stops past the strip)
+    auto walk_at = [&](int64_t lv) {
       double x0, y0, x1, y1;
       if (!kept_segment(J, lv, x0, y0, x1, y1)) return;
       // (cells lie within a pixel of the floors of the end points)
@@ -506,7 +505,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         return;
       const bool ok = walk_all_touched(x0, y0, x1, y1, J.ncols, J.nrows, u.C0, u.tile_c1, u.Rw, u.strip_r1,
                                        [&](int64_t x, int64_t y) {
-                                         put((uint32_t)(y - u.Rw) * TW + (uint32_t)(x - u.C0));
+                                         const uint32_t idx = (uint32_t)(y - u.Rw) * TW + (uint32_t)(x - u.C0);
+                                         atomicOr(&mark[idx >> 5], 1u << (idx & 31u));
                                        });
       if (!ok) atomicOr(a.status, 64u);
     };
@@ -520,8 +520,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // skipped when the cache bit its wrapped index lands on is set.
     auto polygon_at = [&](const ItemDesc& d, uint32_t pi) {
       const T v = from_bits<T>(d.value);
-      for (int64_t e = lane; e < d.nv; e += 64)
-        walk_at(d.vbeg + e - J.v_base, [&](uint32_t idx) { atomicOr(&mark[idx >> 5], 1u << (idx & 31u)); });
+      for (int64_t e = lane; e < d.nv; e += 64) walk_at(d.vbeg + e - J.v_base);
       const uint64_t cov = polygon_mask<NWORD>(a, d, u, hb_ok, hbase, pb, pe, pi, strip, stage, lane);
       wave_sync();
       if (q < NWORD) {
@@ -588,31 +587,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       const int32_t kind = (int32_t)(d.kind_layer & 3u);
       const T v = from_bits<T>(d.value);
       const bool dedup = kind == 1 && (dedup_job || at_job);
-      // a write of an idempotent function is folded by the lane that emits
-      // it; sum and count take a mark (a line's first write per pixel with
-      // xres != yres or all-touched) or a count, LDS atomics whose result is
-      // not waited for, folded after the item
-      constexpr bool direct = !COUNTED;
-      const bool marked = dedup;
-      bool hit = false;  // this lane marked or counted a pixel
-      auto put = [&](uint32_t idx) {
+      // a write is a mark (one fold per pixel: idempotent functions, or a
+      // line's first write per pixel with xres != yres or all-touched) or a
+      // count; both are LDS atomics whose result is not waited for
+      const bool marked = !COUNTED || dedup;
+      auto emit = [&](auto cx, auto cy) {
+        if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+        const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
         if (FSUM && unord) {
           ++burns;
           add1(idx, v);
-        } else if (direct) {
-          ++burns;
-          fold1(idx, v);
         } else if (marked) {
-          hit = true;
           atomicOr(&mark[idx >> 5], 1u << (idx & 31u));
         } else {
-          hit = true;
           atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
         }
-      };
-      auto emit = [&](auto cx, auto cy) {
-        if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
-        put((uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0));
       };
       // end cell of the last kept edge of an open line (burners.rs:86-89)
       auto end_cell = [&](const uint4& r, uint32_t e) {
@@ -630,7 +619,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             continue;
           }
           if (at_job) {  // the all-touched walker, from the world coordinates
-            walk_at(d.vbeg + (int64_t)e - J.v_base, put);
+            walk_at(d.vbeg + (int64_t)e - J.v_base);
             continue;
           }
           if (r.x == kLsFar) {
@@ -663,9 +652,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (end_cell(r, e)) emit(ix1, iy1);
           }
         }
-        if (COUNTED && !marked && !unord && __any(hit)) {
+        if (COUNTED && !marked && !unord) {
           // each lane folds its PPL pixels k times each and clears their counters
-          hit = false;
           wave_sync();
 #pragma unroll
           for (int c0 = 0; c0 < PPL; c0 += 16) {
@@ -689,7 +677,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           wave_sync();
         }
       }
-      if (COUNTED && marked && !unord && __any(hit)) apply_marks(v);
+      if (marked && !unord) apply_marks(v);
     };
 
     // the records of a line or multi-point item in this strip: base, count,
@@ -810,12 +798,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           }
         }
       }
+    } else if (J.scan_compact && a.scan_box) {
+      // scan job of one band: the 8-byte item boxes in strip and tile units,
+      // 64 at a time, the next 64 loaded while this batch's hits are
+      // processed
+      const int64_t ib = J.i_base, ie = J.i_base + J.ni;
+      const uint32_t sidx = (uint32_t)u.Rw / WAVE_ROWS, tx = (uint32_t)u.C0 / (uint32_t)TW;
+      const uint2* sb = a.scan_box;
+      uint2 nbx = ib + lane < ie ? sb[ib + lane] : make_uint2(0xFFFFu, 0u);
+      for (int64_t base = ib; base < ie; base += 64) {
+        const uint2 bx = nbx;
+        const bool valid = base + lane < ie;
+        if (base + 64 + lane < ie) nbx = sb[base + 64 + lane];
+        uint64_t m = __ballot(valid && scan_box_meets(bx, sidx, tx));
+        while (m) {
+          const int bit = __builtin_ctzll(m);
+          m &= m - 1;
+          process_item(load_desc(desc_c + base + bit), 0u);
+        }
+      }
     } else {
-      // scan job: test every item's box in order (small item counts),
-      // kScanBatches x 64 boxes per round with all their loads in flight at
-      // once, then the round's hits in order
-      scan_items(desc_c, J.i_base, J.i_base + J.ni, u, lane,
-                 [&](int64_t it) { process_item(load_desc(desc_c + it), 0u); });
+      // scan job: test every item's box in order (small item counts), 64
+      // boxes at a time, the next 64 loaded while this batch's hits are
+      // processed
+      const int64_t ib = J.i_base, ie = J.i_base + J.ni;
+      BoxLayer nbx = ib + lane < ie ? load_box(desc_c + ib + lane) : BoxLayer{0, 0, 0, 0, 0u};
+      for (int64_t base = ib; base < ie; base += 64) {
+        const BoxLayer bx = nbx;
+        const bool valid = base + lane < ie;
+        if (base + 64 + lane < ie) nbx = load_box(desc_c + base + 64 + lane);
+        const bool hit = valid && (bx.kind_layer >> 2) == u.layer && bx.r0 < u.strip_r1 && bx.r1 > u.Rw &&
+                         bx.c0 < u.tile_c1 && bx.c1 > u.C0 && bx.r0 < bx.r1 && bx.c0 < bx.c1;
+        uint64_t m = __ballot(hit);
+        while (m) {
+          const int bit = __builtin_ctzll(m);
+          m &= m - 1;
+          process_item(load_desc(desc_c + base + bit), 0u);
+        }
+      }
     }
 
     have = tq.advance();

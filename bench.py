@@ -496,9 +496,11 @@ def batch_bench(args) -> int:
     generated on the device per wave (untimed).  Each wave runs (a) as one
     plain rz_rasterize_device call per job, (b) as one rz_batch_rasterize
     pipeline and (c) as a captured hipGraph replay; each leg is timed from
-    device synchronisation to device synchronisation and summed over the
-    waves.  The autotuner runs first, on jobs of the first wave, and every
-    leg uses its table."""
+    device synchronisation to device synchronisation, with no profiling
+    events, after one untimed pass of the same leg on the same wave (which
+    takes any workspace growth and records the stage breakdown), and summed
+    over the waves.  The autotuner runs first, on jobs of the first wave,
+    and every leg uses its table."""
     from rusterize_amd.device import (BatchGraph, Context, DeviceBuffer, DeviceGeometry, Job, autotune,
                                       autotune_table, batch_rasterize, grid, rasterize_device)
 
@@ -548,18 +550,19 @@ def batch_bench(args) -> int:
             batch_rasterize(ctx, jobs)
 
         for name, fn in (("plain", plain), ("batched", batched)):
-            if wi == 0:
-                fn()  # warm-up (workspace)
-                ctx.synchronize()
+            # an untimed pass first: any workspace growth for this wave's
+            # sizes, and the stage breakdown (events around every stage)
             ctx.set_profiling(1)
             ctx.reset_stage_times()
+            fn()
+            ctx.synchronize()
+            for key, v in ctx.stage_times().items():
+                stages[name][key] = stages[name].get(key, 0.0) + v
+            ctx.set_profiling(0)
             t0 = time.perf_counter()
             fn()
             ctx.synchronize()
             times[name] += time.perf_counter() - t0
-            for key, v in ctx.stage_times().items():
-                stages[name][key] = stages[name].get(key, 0.0) + v
-            ctx.set_profiling(0)
         if not args.no_graph:
             g = BatchGraph(jobs)
             g.launch()
@@ -638,8 +641,9 @@ def serve_bench(args) -> int:
     field and function.  Served = rz_server with continuous batching over
     --streams worker streams (batches of up to --max-batch) under a 200 GB
     budget; plain = the same arrivals served one call at a time (1 stream,
-    batches of 1).  Both servers are warmed up once on the stream's first
-    --pool jobs and then serve every rate in turn.  Latency is submission to
+    batches of 1).  Both servers are warmed up on the stream's first --pool
+    jobs and on one untimed run of the highest rate's arrivals (other jobs
+    of the stream), then serve every rate in turn.  Latency is submission to
     completion, measured by the server."""
     import threading
 
@@ -740,10 +744,24 @@ def serve_bench(args) -> int:
         return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "jobs_per_s": round(len(reqs) / wall, 1), "burns_per_s": n_burns / wall,
                 "wall_s": round(wall, 3), "batches": st["batches"] - st0["batches"],
-                "peak_memory_GB": round(st["memory_peak"] / 1e9, 2)}
+                "peak_memory_GB": round(st["memory_peak"] / 1e9, 2),
+                # host time of the batches' phases in this run (sums), and the
+                # largest single phase so far
+                "phase_ms": {k: round((st[k + "_us_sum"] - st0[k + "_us_sum"]) / 1e3, 1) for k in ("alloc", "run", "wait")},
+                "phase_max_ms": {k: round(st[k + "_us_max"] / 1e3, 1) for k in ("alloc", "run", "wait")}}
 
     served_srv = make_server(args.streams, args.max_batch)
     plain_srv = make_server(1, 1)
+    # one untimed run of the highest rate's arrivals (other stream jobs) on
+    # both servers: their workspaces and output cache grow to the sizes the
+    # timed rates need, as in a server that has been running under load
+    reqs_w, sets_w = schedule(max(rates), 1, 977 * (len(rates) + 1))
+    run(served_srv, reqs_w, 0)
+    run(plain_srv, reqs_w, 0)
+    for _, dg, f0, alt in sets_w:
+        f0.free()
+        alt.free()
+        dg.free()
     table = {}
     for i, rate in enumerate(rates):
         reqs, sets = schedule(rate, int(rate), 977 * (i + 1))

@@ -372,6 +372,11 @@ typedef struct rz_server rz_server;
 typedef struct rz_server_stats {
   uint64_t queued, in_flight_batches, batches, completed;
   uint64_t memory_used, memory_peak, memory_budget;
+  /* host time of a batch's phases, microseconds, summed over batches and
+   * the largest single one: output allocation and eviction, run_batch
+   * (launches and its host round trips), the wait for the device */
+  uint64_t alloc_us_sum, run_us_sum, wait_us_sum;
+  uint64_t alloc_us_max, run_us_max, wait_us_max;
 } rz_server_stats;
 
 rz_status rz_server_create(int device, uint64_t memory_budget, int n_streams, int max_batch,

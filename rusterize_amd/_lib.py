@@ -50,7 +50,9 @@ class RzJob(ctypes.Structure):
 
 class RzServerStats(ctypes.Structure):
     _fields_ = [(name, c_uint64) for name in ("queued", "in_flight_batches", "batches", "completed",
-                                              "memory_used", "memory_peak", "memory_budget")]
+                                              "memory_used", "memory_peak", "memory_budget",
+                                              "alloc_us_sum", "run_us_sum", "wait_us_sum",
+                                              "alloc_us_max", "run_us_max", "wait_us_max")]
 
 
 class RusterizeNativeError(RuntimeError):

@@ -84,6 +84,7 @@ struct rz_server {
   std::vector<uint64_t> ws;     // each context's workspace, measured after its last batch
   std::vector<uint64_t> grow;   // each running batch's reserved workspace growth
   uint64_t n_batches = 0, n_done = 0, peak_used = 0, running = 0;
+  uint64_t phase_sum[3] = {0, 0, 0}, phase_max[3] = {0, 0, 0};  // alloc, run, wait (us)
 
   uint64_t used() const {
     uint64_t u = out_used + cached;
@@ -198,6 +199,7 @@ struct rz_server {
       ++running;
       lk.unlock();
       // evictions and fresh outputs, stream-ordered on this worker's stream
+      const auto t_alloc = Clock::now();
       rz_status st = RZ_OK;
       for (void* p : adm.evict)
         if (hipFreeAsync(p, c->stream) != hipSuccess) st = rz::fail(RZ_DEVICE_ERROR, "freeing a cached output failed");
@@ -212,7 +214,9 @@ struct rz_server {
         for (size_t k = 0; k < adm.fresh.size(); ++k) reqs[adm.fresh[k]].job.out_dev = fresh_ptr[k];
         for (uint64_t t : batch) jobs.push_back(reqs[t].job);
       }
+      const auto t_run = Clock::now();
       if (st == RZ_OK) st = run_batch(c, jobs.data(), (int64_t)jobs.size());
+      const auto t_wait = Clock::now();
       if (st == RZ_OK && !(have_ev && hipEventRecord(ev, c->stream) == hipSuccess &&
                            hipEventSynchronize(ev) == hipSuccess))
         st = rz::fail(RZ_DEVICE_ERROR, "waiting for a batch failed");
@@ -220,7 +224,13 @@ struct rz_server {
       const std::string msg = st ? rz_last_error() : std::string();
       const uint64_t ws_now = rz_context_workspace_bytes(c);
       const auto now = Clock::now();
+      const uint64_t us[3] = {(uint64_t)(ms_between(t_alloc, t_run) * 1e3), (uint64_t)(ms_between(t_run, t_wait) * 1e3),
+                              (uint64_t)(ms_between(t_wait, now) * 1e3)};
       lk.lock();
+      for (int k = 0; k < 3; ++k) {
+        phase_sum[k] += us[k];
+        phase_max[k] = std::max(phase_max[k], us[k]);
+      }
       ws[slot] = ws_now;  // the measured workspace replaces the reservation
       grow[slot] = 0;
       peak_used = std::max(peak_used, used());
@@ -367,5 +377,11 @@ extern "C" rz_status rz_server_get_stats(rz_server* s, rz_server_stats* out) {
   out->memory_used = s->used();
   out->memory_peak = s->peak_used;
   out->memory_budget = s->budget;
+  out->alloc_us_sum = s->phase_sum[0];
+  out->run_us_sum = s->phase_sum[1];
+  out->wait_us_sum = s->phase_sum[2];
+  out->alloc_us_max = s->phase_max[0];
+  out->run_us_max = s->phase_max[1];
+  out->wait_us_max = s->phase_max[2];
   return RZ_OK;
 }

@@ -285,10 +285,15 @@ rz_status rz_rasterize_device_noalloc(rz_context* ctx, const rz_device_geometry*
  * bottom and each row's columns left to right; a line's kept segments in
  * order and each segment's cells in Bresenham order, the end cell last;
  * points in order.  value is the geometry's field value (or the scalar).
+ * With all_touched, a line's segments are walked by the all-touched walker,
+ * and a polygon writes its outline that way before its rows; fun decides,
+ * as in the reference's context (prelude.rs:116-118), whether sum / count
+ * write each outline pixel once and skip fill pixels the outline's
+ * PixelCache contains (LineWriter / FillWriter, writers.rs:14-60).
  * The triplets stay on the device until rz_sparse_free. */
 typedef struct rz_sparse rz_sparse;
 rz_status rz_rasterize_sparse(const rz_geometry* geoms, const rz_raster_info* info, rz_dtype dtype,
-                              const void* field, const void* scalar, const uint32_t* layer_of_geom,
+                              rz_fun fun, const void* field, const void* scalar, const uint32_t* layer_of_geom,
                               uint32_t n_layers, int all_touched, rz_sparse** out);
 /* Triplets of each band (n_layers values; SparseArray offsets). */
 rz_status rz_sparse_counts(const rz_sparse* sparse, uint64_t* per_band);

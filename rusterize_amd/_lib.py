@@ -107,7 +107,7 @@ _SIGNATURES = [
                                     c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
     ("rz_rasterize_device_noalloc", c_int, [c_void_p, c_void_p, POINTER(RzRasterInfo), c_int, c_int, c_void_p,
                                             c_void_p, c_void_p, c_uint32, c_void_p, c_int, c_int, c_void_p]),
-    ("rz_rasterize_sparse", c_int, [POINTER(RzGeometry), POINTER(RzRasterInfo), c_int, c_void_p, c_void_p,
+    ("rz_rasterize_sparse", c_int, [POINTER(RzGeometry), POINTER(RzRasterInfo), c_int, c_int, c_void_p, c_void_p,
                                     POINTER(c_uint32), c_uint32, c_int, POINTER(c_void_p)]),
     ("rz_sparse_counts", c_int, [c_void_p, POINTER(c_uint64)]),
     ("rz_sparse_copy", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -49,10 +49,25 @@ struct SparseArgs {
   uint8_t* values;             // [triplets] of esz bytes
   int esz;
   int dedup;                   // xres != yres: lines are walked whole
-  uint64_t* bits;              // first-visit bits of the dedup lines
-  const uint64_t* bit_base;    // [n_items + 1] bit offsets of the dedup lines (by position)
+  int at;                      // all-touched: the GDAL walker, polygons burn their outline first
+  int at_dedup;                // all-touched sum / count: LineWriter + FillWriter over each item's PixelCache
+  const unsigned long long* item_win;  // all-touched dedup polygons' cache windows (by item)
+  uint64_t* bits;              // first-visit bits of the whole-walked items
+  const uint64_t* bit_base;    // [n_items + 1] bit offsets of those items (by position)
   uint32_t* status;
 };
+
+// A line item walked whole (one unit, first visit per pixel): xres != yres,
+// or all-touched sum / count (burn_geometry.rs:168-210).
+__device__ inline bool sp_line_whole(const SparseArgs& a) { return a.dedup || a.at_dedup; }
+
+// Outline units of a polygon item before its strips: all-touched polygons
+// burn their outline first, segment by segment (one unit per vertex), or
+// as one unit walked whole through the PixelCache (sum / count).
+__device__ inline uint32_t sp_outline_units(const SparseArgs& a, const ItemDesc& d) {
+  if (!a.at) return 0u;
+  return a.at_dedup ? 1u : (uint32_t)d.nv;
+}
 
 // Units of the item at position p: a polygon's strips, a line's or a point
 // item's vertices (1 for a line walked whole).
@@ -66,7 +81,13 @@ __global__ void k_sp_units(SparseArgs a, uint32_t* n_units, uint64_t* nbits) {
   uint64_t b = 0;
   if (kind == 0) {
     n = boxed && (d.flags & kItemWide) ? (uint32_t)wide_strip_count(d.r0, d.r1) : 0u;
-  } else if (kind == 1 && a.dedup) {
+    if (boxed) n += sp_outline_units(a, d);
+    CacheWin cw;
+    if (boxed && a.at && a.at_dedup && item_cache(a.item_win, (int64_t)d.last, cw)) {
+      if (cw.cap > ((uint64_t)1 << 34)) atomicOr(a.status, 32u);  // more than 2 GiB of cache bits
+      else b = cw.cap;
+    }
+  } else if (kind == 1 && sp_line_whole(a)) {
     n = boxed ? 1u : 0u;
     b = boxed ? (uint64_t)(d.r1 - d.r0) * (uint64_t)(d.c1 - d.c0) : 0u;
   } else {
@@ -119,9 +140,11 @@ __global__ void k_sp_lines(SparseArgs a, uint64_t n_units) {
   const uint32_t p = sp_unit_pos(a.ubase, (uint32_t)a.n_items, u);
   const ItemDesc d = a.desc[a.perm[p]];
   const uint32_t kind = d.kind_layer & 3u;
-  if (kind == 0 || (kind == 1 && a.dedup)) return;
+  if (kind == 1 && sp_line_whole(a)) return;
+  const uint64_t local = u - a.ubase[p];
+  if (kind == 0 && (a.at_dedup || local >= (uint64_t)sp_outline_units(a, d))) return;  // strips, whole outlines
   const RZ_CONST JobDev& J = *as_const(a.job);
-  const int64_t s = d.vbeg + (int64_t)(u - a.ubase[p]);
+  const int64_t s = d.vbeg + (int64_t)local;
   uint64_t n = 0;
   const uint64_t o = EMIT ? a.toff[u] : 0;
   auto emit = [&](auto cx, auto cy) {
@@ -132,6 +155,12 @@ __global__ void k_sp_lines(SparseArgs a, uint64_t n_units) {
   if (kind == 2) {  // a point: its cell (extract_point, edges.rs:79-88)
     const uint4 r = a.lseg[s];
     if (r.x != kLsNone) emit((int32_t)r.x, (int32_t)r.y);
+  } else if (a.at) {  // an all-touched segment (burners.rs:115-246): a line's, or a polygon's outline
+    double x0, y0, x1, y1;
+    if (kept_segment(J, s - J.v_base, x0, y0, x1, y1) &&
+        !walk_all_touched(x0, y0, x1, y1, J.ncols, J.nrows, 0, J.ncols, 0, J.nrows,
+                          [&](int64_t x, int64_t y) { emit(x, y); }))
+      atomicOr(a.status, 64u);
   } else {
     sp_segment(a, J, d, s, emit);
   }
@@ -141,28 +170,53 @@ __global__ void k_sp_lines(SparseArgs a, uint64_t n_units) {
 // Lines walked whole (xres != yres), one thread each: a pixel is written at
 // its first visit only (PixelCache over the line's pixel box).  The count
 // pass leaves the bits set; the emit pass runs on cleared bits.
+// An all-touched sum / count polygon's outline is walked whole the same way
+// (LineWriter over its PixelCache, burn_geometry.rs:225-234): a cell is
+// written at the first insert of its cache index, and the bits then hold the
+// cache its fill is filtered by.
 template <bool EMIT>
 __global__ void k_sp_dedup(SparseArgs a) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n_items) return;
   const ItemDesc d = a.desc[a.perm[p]];
-  if ((d.kind_layer & 3u) != 1u || !a.dedup || !(d.r0 < d.r1 && d.c0 < d.c1)) return;
+  const uint32_t kind = d.kind_layer & 3u;
+  if (!(d.r0 < d.r1 && d.c0 < d.c1)) return;
+  const bool outline = kind == 0 && a.at && a.at_dedup;
+  if (!(kind == 1 && sp_line_whole(a)) && !outline) return;
   const RZ_CONST JobDev& J = *as_const(a.job);
   uint64_t* bits = a.bits + a.bit_base[p];
   const int64_t w = d.c1 - d.c0;
+  CacheWin cw{0, 0, 0, 0};
+  // (no kept segment: no cache; one past 2^34 bits was refused by k_sp_units)
+  if (outline && (!item_cache(a.item_win, (int64_t)d.last, cw) || cw.cap > ((uint64_t)1 << 34))) return;
   const uint64_t u = a.ubase[p];
   uint64_t n = 0;
   const uint64_t o = EMIT ? a.toff[u] : 0;
   auto emit = [&](auto cx, auto cy) {
     if (cx < 0 || cx >= J.ncols || cy < 0 || cy >= J.nrows) return;
-    const uint64_t k = (uint64_t)((int64_t)cy - d.r0) * (uint64_t)w + (uint64_t)((int64_t)cx - d.c0);
+    if (!outline && (cy < d.r0 || cy >= d.r1 || cx < d.c0 || cx >= d.c1)) return;  // (cells lie in the box)
+    // a line's first visits over its box; an outline's PixelCache index
+    // (one past the capacity, which the reference rejects, is skipped)
+    const uint64_t k = outline ? cache_index(cw, (int64_t)cx, (int64_t)cy)
+                               : (uint64_t)((int64_t)cy - d.r0) * (uint64_t)w + (uint64_t)((int64_t)cx - d.c0);
+    if (outline && k >= cw.cap) return;
     const uint64_t bit = 1ull << (k & 63);
     if (bits[k >> 6] & bit) return;
     bits[k >> 6] |= bit;
     if (EMIT) sp_put(a, o + n, (int64_t)cy, (int64_t)cx, d.value);
     ++n;
   };
-  for (int64_t s = d.vbeg; s < d.vbeg + d.nv; ++s) sp_segment(a, J, d, s, emit);
+  for (int64_t s = d.vbeg; s < d.vbeg + d.nv; ++s) {
+    if (a.at) {
+      double x0, y0, x1, y1;
+      if (kept_segment(J, s - J.v_base, x0, y0, x1, y1) &&
+          !walk_all_touched(x0, y0, x1, y1, J.ncols, J.nrows, 0, J.ncols, 0, J.nrows,
+                            [&](int64_t x, int64_t y) { emit(x, y); }))
+        atomicOr(a.status, 64u);
+    } else {
+      sp_segment(a, J, d, s, emit);
+    }
+  }
   if (!EMIT) a.counts[u] = (uint32_t)n;
 }
 
@@ -177,15 +231,31 @@ __global__ __launch_bounds__(256) void k_sp_polys(SparseArgs a, uint64_t n_units
     const uint32_t p = sp_unit_pos(a.ubase, (uint32_t)a.n_items, u);
     const ItemDesc d = load_desc(as_const(a.desc) + a.perm[p]);
     if ((d.kind_layer & 3u) != 0u) continue;
-    const int32_t k = (int32_t)(u - a.ubase[p]);
+    const uint32_t n_out = sp_outline_units(a, d);
+    if (u - a.ubase[p] < (uint64_t)n_out) continue;  // an outline unit
+    const int32_t k = (int32_t)(u - a.ubase[p] - n_out);
     const int32_t nw = wide_row_words(d.c0, d.c1);
+    // all-touched sum / count: the fill skips pixels the outline's cache
+    // contains (FillWriter, writers.rs:38-60): inside the cache window the
+    // pixel's own bit, outside it the bit its wrapped index lands on
+    CacheWin cw{0, 0, 0, 0};
+    const bool filter = a.at && a.at_dedup && item_cache(a.item_win, (int64_t)d.last, cw) &&
+                        cw.cap <= ((uint64_t)1 << 34);
+    const uint64_t* cbits = filter ? a.bits + a.bit_base[p] : nullptr;
     const uint64_t* m = a.wide_masks + (uint64_t)d.wide * 16u + (uint64_t)k * 16u * (uint64_t)nw;
     const int32_t Rw = ((d.r0 >> 4) + k) * WAVE_ROWS, C = (d.c0 >> 6) * 64;
     uint64_t o = EMIT ? a.toff[u] : 0, n = 0;
     for (int32_t r = 0; r < WAVE_ROWS; ++r) {
       for (int32_t w0 = 0; w0 < nw; w0 += 64) {
         const int32_t w = w0 + lane;
-        const uint64_t word = w < nw ? m[(uint64_t)r * nw + (uint64_t)w] : 0ull;
+        uint64_t word = w < nw ? m[(uint64_t)r * nw + (uint64_t)w] : 0ull;
+        if (filter) {
+          for (uint64_t b = word; b; b &= b - 1) {
+            const int bit = __builtin_ctzll(b);
+            const uint64_t ci = cache_index(cw, (int64_t)(C + w * 64 + bit), (int64_t)(Rw + r));
+            if (ci < cw.cap && ((cbits[ci >> 6] >> (ci & 63)) & 1ull)) word &= ~(1ull << bit);
+          }
+        }
         uint32_t c = (uint32_t)__popcll(word), x = c;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -434,7 +504,11 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
   a.wide_masks = (const uint64_t*)c->wide_masks.p;
   a.esz = (int)rz_dtype_size(j.dtype);
   a.dedup = j.info.xres != j.info.yres;
+  a.at = j.all_touched ? 1 : 0;
+  a.at_dedup = j.all_touched && (j.fun == RZ_SUM || j.fun == RZ_COUNT);  // prelude.rs:116-118
+  a.item_win = (a.at_dedup && dg->has_poly) ? (const unsigned long long*)c->item_win.p : nullptr;
   a.status = (uint32_t*)c->misc.p;
+  const bool whole = a.dedup || a.at_dedup;  // items walked whole, with first-visit bits
   HIP_TRY(ensure(s, c->counts, (size_t)(I + 1) * 4));
   HIP_TRY(ensure(s, c->offsets, (size_t)(I + 1) * 8));
   HIP_TRY(ensure(s, c->band_base, (size_t)(I + 1) * 8));  // dedup bits per position
@@ -446,12 +520,12 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
   ia.clear((uint64_t*)c->band_base.p + I, 2);
   hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, ia);
   hipLaunchKernelGGL(k_sp_units, dim3(blocks_for(I)), dim3(256), 0, s, a, (uint32_t*)c->counts.p,
-                     a.dedup ? (uint64_t*)c->band_base.p : nullptr);
+                     whole ? (uint64_t*)c->band_base.p : nullptr);
   exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->counts.p, (uint64_t*)c->offsets.p, (uint64_t)I + 1,
                                      (uint64_t*)c->scan_tmp.p, s);
   a.ubase = (const uint64_t*)c->offsets.p;
   uint64_t n_bits_words = 0;
-  if (a.dedup) {
+  if (whole) {
     exclusive_scan<uint64_t, uint64_t>((const uint64_t*)c->band_base.p, (uint64_t*)c->heavy_base.p,
                                        (uint64_t)I + 1, (uint64_t*)c->scan_tmp.p, s);
     HIP_TRY(hipMemcpyAsync(&c->host_misc[1], (const uint64_t*)c->heavy_base.p + I, 8, hipMemcpyDeviceToHost, s));
@@ -459,14 +533,14 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
   HIP_TRY(hipMemcpyAsync(&c->host_misc[0], (const uint64_t*)c->offsets.p + I, 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   const uint64_t U = c->host_misc[0];
-  if (a.dedup) n_bits_words = c->host_misc[1];
+  if (whole) n_bits_words = c->host_misc[1];
   if (U >= ((uint64_t)1 << 32)) return rz::fail(RZ_CAPACITY_ERROR, "more than 2^32 sparse work units");
   // 3. triplets per unit, then their places
   HIP_TRY(ensure(s, c->cover_list, (size_t)(U + 1) * 4));   // counts per unit
   HIP_TRY(ensure(s, c->heavy_list, (size_t)(U + 1) * 8));   // their scan
   HIP_TRY(ensure(s, c->scan_tmp2, scan_blocks(U + 1) * 8));
   a.counts = (uint32_t*)c->cover_list.p;
-  if (a.dedup) {
+  if (whole) {
     HIP_TRY(ensure(s, c->wide_item, std::max<uint64_t>(2 * n_bits_words, 1) * 8));  // count + emit halves
     a.bits = (uint64_t*)c->wide_item.p;
     a.bit_base = (const uint64_t*)c->heavy_base.p;
@@ -474,13 +548,14 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
   InitArgs ib{};
   std::memset(&ib, 0, sizeof(ib));
   ib.clear(a.counts, U + 1);
-  if (a.dedup) ib.clear(a.bits, 4 * n_bits_words);  // (32-bit words)
+  if (whole) ib.clear(a.bits, 4 * n_bits_words);  // (32-bit words)
   hipLaunchKernelGGL(k_init, dim3(blocks_for((int64_t)std::max<uint64_t>(ib.words(), 1))), dim3(256), 0, s, ib);
+  // (the whole walks first: an all-touched outline's bits filter its fill)
   if (U > 0) {
+    if (whole) hipLaunchKernelGGL(k_sp_dedup<false>, dim3(blocks_for(I)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_sp_lines<false>, dim3(blocks_for((int64_t)U)), dim3(256), 0, s, a, U);
     hipLaunchKernelGGL(k_sp_polys<false>, dim3((unsigned)std::min<uint64_t>((U + 3) / 4, 4096)), dim3(256), 0, s,
                        a, U);
-    if (a.dedup) hipLaunchKernelGGL(k_sp_dedup<false>, dim3(blocks_for(I)), dim3(256), 0, s, a);
   }
   exclusive_scan<uint32_t, uint64_t>(a.counts, (uint64_t*)c->heavy_list.p, U + 1, (uint64_t*)c->scan_tmp2.p, s);
   a.toff = (const uint64_t*)c->heavy_list.p;
@@ -498,12 +573,12 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
   a.rows = sp->rows;
   a.cols = sp->cols;
   a.values = sp->values;
-  if (a.dedup) a.bits += n_bits_words;  // the second (clear) half
+  if (whole) a.bits += n_bits_words;  // the second (clear) half
   if (U > 0 && T > 0) {
+    if (whole) hipLaunchKernelGGL(k_sp_dedup<true>, dim3(blocks_for(I)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_sp_lines<true>, dim3(blocks_for((int64_t)U)), dim3(256), 0, s, a, U);
     hipLaunchKernelGGL(k_sp_polys<true>, dim3((unsigned)std::min<uint64_t>((U + 3) / 4, 4096)), dim3(256), 0, s,
                        a, U);
-    if (a.dedup) hipLaunchKernelGGL(k_sp_dedup<true>, dim3(blocks_for(I)), dim3(256), 0, s, a);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
@@ -511,14 +586,14 @@ static rz_status sparse_build(rz_context* c, const rz_job& j, rz_sparse* sp) {
 }
 
 extern "C" rz_status rz_rasterize_sparse(const rz_geometry* geoms, const rz_raster_info* info, rz_dtype dtype,
-                                         const void* field, const void* scalar, const uint32_t* layer_of_geom,
-                                         uint32_t n_layers, int all_touched, rz_sparse** out) {
+                                         rz_fun fun, const void* field, const void* scalar,
+                                         const uint32_t* layer_of_geom, uint32_t n_layers, int all_touched,
+                                         rz_sparse** out) {
   if (!geoms || !info || !out) return rz::fail(RZ_VALUE_ERROR, "NULL argument to rz_rasterize_sparse");
   *out = nullptr;
   if ((unsigned)dtype > 9) return rz::fail(RZ_TYPE_ERROR, "Invalid dtype or encoding provided.");
   if (!field && !scalar) return rz::fail(RZ_VALUE_ERROR, "no field and no scalar value");
-  if (all_touched)
-    return rz::fail(RZ_VALUE_ERROR, "all_touched=True is not implemented for the sparse encoding yet");
+  if ((unsigned)fun > 6) return rz::fail(RZ_VALUE_ERROR, "Unknown pixel function");
   if (!layer_of_geom) n_layers = 1;
   if (n_layers == 0) return rz::fail(RZ_VALUE_ERROR, "n_layers must be at least 1");
   for (int64_t i = 0; layer_of_geom && i < geoms->n_geoms; ++i)
@@ -552,7 +627,8 @@ extern "C" rz_status rz_rasterize_sparse(const rz_geometry* geoms, const rz_rast
     j.geometry = dg;
     j.info = *info;
     j.dtype = dtype;
-    j.fun = RZ_LAST;
+    j.fun = fun;  // (decides the all-touched dedup, prelude.rs:116-118)
+    j.all_touched = all_touched ? 1 : 0;
     j.field_dev = d_field;
     if (!field) std::memcpy(&j.scalar_bits, scalar, esz);
     j.layer_dev = (const uint32_t*)d_layer;

@@ -120,7 +120,8 @@ def rasterize_sparse(geom, info, dtype: str, fun: str, field, scalar, layer_of_g
     lay = np.ascontiguousarray(layer_of_geom, dtype=np.uint32) if layer_of_geom is not None else None
     h = c_void_p()
     st = _lib.lib().rz_rasterize_sparse(ctypes.byref(gs), ctypes.byref(info), _lib.dtype_code(dtype),
-                                        fld.ctypes.data if fld is not None else None, sc.ctypes.data,
+                                        _lib.fun_code(fun), fld.ctypes.data if fld is not None else None,
+                                        sc.ctypes.data,
                                         lay.ctypes.data_as(ctypes.POINTER(c_uint32)) if lay is not None else None,
                                         n_layers if lay is not None else 1, int(bool(all_touched)),
                                         ctypes.byref(h))

@@ -17,10 +17,10 @@ from tests.test_gpu_parity import assert_same, concat, field_for, line_path_geom
 pytestmark = pytest.mark.gpu
 
 
-def expected_triplets(geom, info, field, layers, n_layers):
+def expected_triplets(geom, info, field, layers, n_layers, fun="last", all_touched=False):
     """The oracle's writes in order, regrouped band after band (a stable
     sort by band keeps geometry order inside each band)."""
-    tr = oracle.trace(geom, oracle.copy_info(info))
+    tr = oracle.trace(geom, oracle.copy_info(info), fun, all_touched=all_touched)
     g = tr[:, 2].astype(np.int64)
     band = layers[g] if layers is not None else np.zeros(len(g), dtype=np.uint32)
     order = np.argsort(band, kind="stable")
@@ -29,11 +29,11 @@ def expected_triplets(geom, info, field, layers, n_layers):
     return tr[:, 0], tr[:, 1], field[tr[:, 2].astype(np.int64)], counts
 
 
-def check_sparse(geom, info, dtype, field, layers=None, n_layers=1, what=""):
-    sp = rasterize_sparse(geom, info, dtype, "last", field, None, layers, n_layers, 0,
-                          [f"b{i}" for i in range(n_layers if layers is not None else 1)])
+def check_sparse(geom, info, dtype, field, layers=None, n_layers=1, what="", fun="last", all_touched=False):
+    sp = rasterize_sparse(geom, info, dtype, fun, field, None, layers, n_layers, 0,
+                          [f"b{i}" for i in range(n_layers if layers is not None else 1)], all_touched=all_touched)
     rows, cols, vals = sp.triplets()
-    er, ec, ev, ecounts = expected_triplets(geom, info, field, layers, n_layers)
+    er, ec, ev, ecounts = expected_triplets(geom, info, field, layers, n_layers, fun, all_touched)
     assert sp.counts().tolist() == ecounts.tolist(), what
     np.testing.assert_array_equal(rows, er, err_msg=what)
     np.testing.assert_array_equal(cols, ec, err_msg=what)
@@ -98,3 +98,36 @@ def test_config1_sparse_at_full_size():
     dense = _lib.rasterize(g, info, "float64", "sum", field, None, None, 1, float("nan"))
     sp._fun, sp._bg = "sum", float("nan")
     assert_same(sp.to_numpy(), dense, "stars sum")
+
+
+@pytest.mark.parametrize("fun", ["last", "sum", "count"])
+@pytest.mark.parametrize("xres", [1.0, 0.75])
+def test_all_touched_triplets_match_the_write_trace(fun, xres):
+    # all-touched (burners.rs:94-247): lines through the GDAL walker, polygons
+    # their outline then their rows; sum / count write an outline pixel once
+    # and skip fill pixels the outline's PixelCache contains, also through
+    # wrapped cache indices (the alias polygons)
+    from tests.test_gpu_all_touched import ALIAS_WKT
+    g = concat([mixed_geometry(), line_path_geometry()])
+    info = _lib.raster_info(None, resolution=(xres, 1.0), extent=(0, 0, 300, 200))
+    for dtype in ("float64", "int32"):
+        field = field_for(dtype, g.n_geoms, seed=81)
+        check_sparse(g, info, dtype, field, what=f"all-touched {fun} {dtype} xres={xres}", fun=fun,
+                     all_touched=True)
+    layers = np.random.default_rng(8).integers(0, 3, g.n_geoms).astype(np.uint32)
+    field = field_for("float32", g.n_geoms, seed=82)
+    check_sparse(g, info, "float32", field, layers, 3, what=f"all-touched 3 bands {fun}", fun=fun, all_touched=True)
+    ga = Geometry.from_wkt(ALIAS_WKT)
+    info_a = _lib.raster_info(None, resolution=(xres, 1.0), extent=(0, 0, 100, 100))
+    check_sparse(ga, info_a, "int32", np.arange(1, ga.n_geoms + 1, dtype=np.int32), what=f"alias {fun}", fun=fun,
+                 all_touched=True)
+
+
+def test_all_touched_dense_equals_materialised_sparse():
+    g = concat([mixed_geometry(), line_path_geometry()])
+    info = _lib.raster_info(None, resolution=(1.0, 1.0), extent=(0, 0, 300, 200))
+    for fun in _lib.FUNS:
+        field = field_for("float64", g.n_geoms, seed=83)
+        sp = rasterize_sparse(g, info, "float64", fun, field, None, None, 1, float("nan"), ["a"], all_touched=True)
+        dense = _lib.rasterize(g, info, "float64", fun, field, None, None, 1, float("nan"), all_touched=True)
+        assert_same(sp.to_numpy(), dense, f"all-touched {fun}")

@@ -323,7 +323,7 @@ typedef struct rz_job {
   uint64_t background_bits;
   void* out_dev;             /* [n_layers][nrows][ncols] of dtype, disjoint per job */
   rz_sum_mode sum_mode;      /* fun = sum with a float dtype: ordered (exact) or unordered */
-  int all_touched;           /* AllTouched strategy (burners.rs:94-247; not in captured graphs) */
+  int all_touched;           /* AllTouched strategy (burners.rs:94-247) */
 } rz_job;
 
 /* Rasterises n_jobs independent jobs (any mix of sizes, dtypes and pixel

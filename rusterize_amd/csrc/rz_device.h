@@ -281,8 +281,9 @@ __device__ inline uint32_t xcd_unit_lo(uint32_t x, uint32_t n) {
 template <int NWORD>
 __device__ inline uint64_t strip_coverage(const PolyEdge* ebase, uint32_t n_e, int32_t Rw,
                                           int32_t strip_r1, int32_t C0, double ncols_f,
-                                          PolyEdge* stage, int lane) {
+                                          PolyEdge* stage, int lane, int wq = -1) {
   const int rl = lane >> 2, q = lane & 3;
+  if (wq < 0) wq = q;  // the window word this lane returns (window from C0, NWORD words)
   const int32_t r = Rw + rl;
   const bool row_ok = r < strip_r1;
   uint64_t tog[NWORD];
@@ -324,7 +325,7 @@ __device__ inline uint64_t strip_coverage(const PolyEdge* ebase, uint32_t n_e, i
   amax = quad_max_all(amax);
 #pragma unroll
   for (int w = 0; w < NWORD; ++w) tog[w] = quad_xor_all64(tog[w]);
-  return q < NWORD ? row_coverage_word<NWORD>(tog, lp, amax, C0, q) : 0ull;
+  return wq < NWORD ? row_coverage_word<NWORD>(tog, lp, amax, C0, wq) : 0ull;
 }
 
 // Largest value of v (0..16) over the lanes of the wave, from five ballots.

@@ -333,6 +333,7 @@ struct AtCacheArgs {
   const unsigned long long* item_win;
   const uint64_t* at_base;  // [n_items + 1] exclusive scan of at_words
   unsigned long long* at_bits;
+  uint64_t cap_words;       // words of at_bits (an item past them fails the call: bit 8)
   uint32_t* status;         // bit 64: a walk did not end
 };
 
@@ -460,6 +461,7 @@ struct TileArgs {
   const unsigned long long* item_win;
   const uint64_t* at_base;
   const unsigned long long* at_bits;
+  uint64_t at_cap;          // words of at_bits
   unsigned long long* burns;  // [kBurnSlots] counters, or null
   uint32_t* status;         // error flags
 };

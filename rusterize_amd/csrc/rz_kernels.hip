@@ -506,6 +506,10 @@ __global__ __launch_bounds__(256) void k_at_cache(AtCacheArgs a) {
   if (i >= a.n_items) return;
   const RZ_CONST ItemDesc& d = as_const(a.desc)[i];
   if (!(d.flags & kItemAtCache)) return;
+  if (a.at_base[i + 1] > a.cap_words) {  // a captured batch's cache capacity is exceeded
+    if (lane == 0) atomicOr(a.status, 8u);
+    return;
+  }
   const RZ_CONST JobDev& J = as_const(a.jobs)[d.job];
   CacheWin cw;
   if (!item_cache(a.item_win, i, cw)) return;

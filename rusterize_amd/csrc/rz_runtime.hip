@@ -107,7 +107,7 @@ struct rz_context {
   EventSet cur;
   double acc_ms[kStages] = {};
   uint64_t last_pairs = 0, last_tiles = 0, last_band_entries = 0, last_band_edges = 0;
-  uint64_t last_wide_units = 0, last_wide_words = 0;
+  uint64_t last_wide_units = 0, last_wide_words = 0, last_at_words = 0;
   bool burns_pending = false;
   std::vector<Buf*> all() {
     return {&edges,  &lseg,   &item_box, &item_last, &desc, &counts,    &offsets, &keys_a,
@@ -640,7 +640,7 @@ rz_status check_job_common(const rz_job& j, bool need_out) {
 // indices are global over the batch.
 // Fixed capacities of a captured batch (see rz_batch_graph_create).
 struct BatchCaps {
-  uint64_t pairs, band_entries, band_edges, wide_units, wide_words;
+  uint64_t pairs, band_entries, band_edges, wide_units, wide_words, at_words;
 };
 
 // The device record of job j, whose vertex / item / tile ranges start at
@@ -834,7 +834,6 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   }
   if (any_scan_compact) HIP_TRY(ensure(c->stream, c->scan_box, (size_t)std::max<int64_t>(I, 1) * 8));
   if (any_at_cache) {
-    if (capture) return rz::fail(RZ_VALUE_ERROR, "all_touched jobs cannot be captured in a graph");
     HIP_TRY(ensure(c->stream, c->item_win, (size_t)std::max<int64_t>(I, 1) * 64));
     HIP_TRY(ensure(c->stream, c->at_words, (size_t)(I + 1) * 4));
     HIP_TRY(ensure(c->stream, c->at_base, (size_t)(I + 1) * 8));
@@ -941,19 +940,27 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
   TileArgs ta{};
   ta.scan_box = any_scan_compact ? (const uint2*)c->scan_box.p : nullptr;
   // all-touched sum / count polygons whose fill reaches past their
-  // PixelCache window: their caches in global memory (one host round trip
-  // for the size)
+  // PixelCache window: their caches in global memory (an eager call reads
+  // their size back; a captured batch uses its fixed capacity, and items
+  // past it fail the replay)
   if (any_at_cache && I > 0) {
     HIP_TRY(ensure(c->stream, c->scan_tmp8, scan_blocks((uint64_t)I + 1) * 8));
     exclusive_scan<uint32_t, uint64_t>((const uint32_t*)c->at_words.p, (uint64_t*)c->at_base.p,
                                        (uint64_t)I + 1, (uint64_t*)c->scan_tmp8.p, s);
-    HIP_TRY(hipMemcpyAsync(&c->host_misc[6], (const uint64_t*)c->at_base.p + I, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const uint64_t words = c->host_misc[6];
-    if (words > 0) {
-      HIP_TRY(ensure(c->stream, c->at_bits, words * 8));
-      zero_words(c->at_bits.p, words * 2, s);
+    uint64_t words = 0;
+    if (capture) {
+      words = caps->at_words;
+    } else {
+      HIP_TRY(hipMemcpyAsync(&c->host_misc[6], (const uint64_t*)c->at_base.p + I, 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      words = c->host_misc[6];
+      c->last_at_words = words;
+    }
+    if (words > 0 || capture) {
+      HIP_TRY(ensure(c->stream, c->at_bits, std::max<uint64_t>(words, 1) * 8));
+      zero_words(c->at_bits.p, std::max<uint64_t>(words, 1) * 2, s);
       AtCacheArgs aa{};
+      aa.cap_words = words;
       aa.jobs = d_jobs;
       aa.desc = (const ItemDesc*)c->desc.p;
       aa.n_items = I;
@@ -965,7 +972,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
     }
     ta.item_win = (const unsigned long long*)c->item_win.p;
     ta.at_base = (const uint64_t*)c->at_base.p;
-    ta.at_bits = words > 0 ? (const unsigned long long*)c->at_bits.p : nullptr;
+    ta.at_bits = (words > 0 || capture) ? (const unsigned long long*)c->at_bits.p : nullptr;
+    ta.at_cap = words;
   }
   if (stage_ev) HIP_TRY(hipEventRecord(c->ev[1], s));
 
@@ -1342,8 +1350,8 @@ extern "C" rz_status rz_batch_graph_create(int device, const rz_job* jobs, int64
     return st;
   }
   auto grow = [&](uint64_t v) { return (uint64_t)((double)v * headroom) + (v ? 64 : 0); };
-  const BatchCaps caps{grow(c->last_pairs), grow(c->last_band_entries), grow(c->last_band_edges),
-                       grow(c->last_wide_units), grow(c->last_wide_words)};
+  const BatchCaps caps{grow(c->last_pairs),      grow(c->last_band_entries), grow(c->last_band_edges),
+                       grow(c->last_wide_units), grow(c->last_wide_words),   grow(c->last_at_words)};
   // 2. a pass at those capacities allocates every buffer, so the captured
   // pass allocates nothing; 3. capture
   st = run_batch(c, jobs, n_jobs, &caps);

@@ -147,7 +147,23 @@ __device__ inline uint64_t polygon_mask(const TileArgs& a, const ItemDesc& d, co
       ebase = a.band_edges + b0;
       n_e = (uint32_t)(as_const(a.band_off)[bi + 1u] - b0);
     }
-    cov = strip_coverage<NWORD>(ebase, n_e, u.Rw, u.strip_r1, u.C0, (double)u.ncols, stage, lane);
+    if constexpr (NWORD == 1) {
+      cov = strip_coverage<1>(ebase, n_e, u.Rw, u.strip_r1, u.C0, (double)u.ncols, stage, lane);
+    } else {
+      // only the tile words the item's columns [c0, c1) meet can be
+      // covered: the crossings are toggled and the parity scanned over
+      // those (1, 2 or all 4 words; crossings left of them flip the left
+      // parity, as left of a tile)
+      const int32_t wlo = max(0, (d.c0 - u.C0) >> 6), whi = min(NWORD - 1, (d.c1 - 1 - u.C0) >> 6);
+      const int32_t C0e = u.C0 + 64 * wlo;
+      const int wq = (lane & 3) - wlo;  // this lane's word in the narrowed window (negative: none)
+      if (whi == wlo)
+        cov = strip_coverage<1>(ebase, n_e, u.Rw, u.strip_r1, C0e, (double)u.ncols, stage, lane, wq == 0 ? 0 : 1);
+      else if (whi == wlo + 1)
+        cov = strip_coverage<2>(ebase, n_e, u.Rw, u.strip_r1, C0e, (double)u.ncols, stage, lane, wq < 0 ? 2 : wq);
+      else
+        cov = strip_coverage<NWORD>(ebase, n_e, u.Rw, u.strip_r1, u.C0, (double)u.ncols, stage, lane);
+    }
   }
   return cov;
 }
@@ -525,7 +541,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       wave_sync();
       if (q < NWORD) {
         uint64_t m = cov;
-        if (COUNTED && (d.flags & kItemAtCache) && m) {
+        if (COUNTED && (d.flags & kItemAtCache) && m && a.at_bits && a.at_base[d.last + 1] <= a.at_cap) {
           CacheWin cw;
           item_cache(a.item_win, (int64_t)d.last, cw);
           const uint64_t length = cw.cap / cw.width;

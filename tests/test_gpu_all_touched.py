@@ -121,3 +121,44 @@ def test_all_touched_star_polygons_at_full_size():
     for fun in ("sum", "last"):
         got, ref = run_at(g, info, "float64", fun, field, float("nan"))
         assert_same(got, ref, f"stars all_touched {fun}")
+
+
+def test_all_touched_sum_in_a_captured_graph():
+    # all-touched sum / count in a captured batch: the PixelCache words of
+    # the polygons whose fill leaves their cache window get a fixed capacity
+    # at capture; every replay equals the oracle
+    from rusterize_amd.device import BatchGraph, Context, DeviceBuffer, DeviceGeometry, Job, batch_rasterize, grid
+    ctx = Context(0)
+    ga, gm = Geometry.from_wkt(ALIAS_WKT), at_geometry()
+    da, dm = DeviceGeometry(ctx, ga), DeviceGeometry(ctx, gm)
+    fa = np.arange(1, ga.n_geoms + 1, dtype=np.int32)
+    fm = field_for("float64", gm.n_geoms, seed=44)
+    bufs = [DeviceBuffer.from_array(ctx, fa), DeviceBuffer.from_array(ctx, fm),
+            DeviceBuffer(ctx, 100 * 100 * 4), DeviceBuffer(ctx, 300 * 200 * 8)]
+    jobs = [Job(da, grid(100, 100, 0, 0), "int32", "count", bufs[2], field=bufs[0], all_touched=True),
+            Job(dm, grid(300, 200, 0, 0), "float64", "sum", bufs[3], field=bufs[1], background=float("nan"),
+                all_touched=True)]
+    refs = [oracle.rasterize(ga, oracle.copy_info(jobs[0].info), "int32", "count", field=fa, background=0,
+                             all_touched=True),
+            oracle.rasterize(gm, oracle.copy_info(jobs[1].info), "float64", "sum", field=fm,
+                             background=float("nan"), all_touched=True)]
+    g = None
+    try:
+        g = BatchGraph(jobs, headroom=1.0)
+        for rep in range(3):
+            for j in jobs:
+                j.out.upload(np.full(j.out_bytes, 0x5A, np.uint8))
+            g.launch()
+            g.synchronize()
+            for k, j in enumerate(jobs):
+                assert_same(j.result(), refs[k], f"replay {rep} job {k}")
+            batch_rasterize(ctx, jobs[::-1])  # eager work between replays
+            ctx.synchronize()
+    finally:
+        if g is not None:
+            g.close()
+        for b in bufs:
+            b.free()
+        da.free()
+        dm.free()
+        ctx.close()

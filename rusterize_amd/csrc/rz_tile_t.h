@@ -377,6 +377,16 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T>::value))) void k_tile(TileAr
   }
 }
 
+// One line item of a group walked together in k_tile_lp: its records, the
+// vertex range its end cell is tested against, kind | banded << 2.
+struct GroupItem {
+  const uint4* recs;
+  int64_t vbeg;
+  uint64_t last;
+  uint32_t kb;
+  uint32_t pad;
+};
+
 // Tile kernel of groups with lines or points (polygon items may be mixed
 // in), one launch per pixel-function group.  Same work queue, strips, item
 // order, background fill and write-once store as k_tile, but the strip's
@@ -386,12 +396,18 @@ __attribute__((amdgpu_waves_per_eu(TileOccupancy<T>::value))) void k_tile(TileAr
 //  * first / last / min / max / any give the same result for one or several
 //    writes of one value, so a line's cells set LDS mark bits (atomic OR,
 //    not waited for) and after the line each lane folds its marked pixels
-//    once;
+//    once.  A run of consecutive line items (up to 8 in a 64-wide strip)
+//    is walked together, its records spread over the lanes and each item
+//    marking its own array; one apply then folds each pixel's marks item by
+//    item, in item order (a line meets a strip in a few short segments, so
+//    one item alone leaves most lanes idle);
 //  * sum and count depend on the number of writes: a line's cells are
 //    counted in 16-bit LDS counters, and after the line each lane folds its
-//    pixels k times.  With xres != yres (LineWriter + PixelCache,
-//    burn_geometry.rs:179-185, 202-208) or all-touched a pixel is written
-//    once per line: a mark bit, folded once;
+//    pixels k times.  Two consecutive lines of at most 255 segments each
+//    in the strip are walked together, each counting in its own 8-bit
+//    counters (a segment writes a pixel once).  With xres != yres
+//    (LineWriter + PixelCache, burn_geometry.rs:179-185, 202-208) or
+//    all-touched a pixel is written once per line: a mark bit, folded once;
 //  * the unordered float sum adds every write with an LDS atomic;
 //  * runs of one-vertex point items are folded in geometry order by one
 //    lane each, from descriptors loaded 64 at a time (the point's cell is its
@@ -412,13 +428,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   constexpr int RPL = WAVE_ROWS / RSTEP;
   constexpr int NPIXS = WAVE_ROWS * TW;           // pixels of a strip
   constexpr bool COUNTED = FUN == 0 || FUN == 5;  // sum, count: the number of writes matters
-  constexpr int CNT_WORDS = COUNTED ? NPIXS / 2 : 1;   // u16 counters, 2 per word
   constexpr int PPL = NPIXS / 64;                      // pixels per lane in an apply (16 or 64)
+  constexpr int MW = NPIXS / 32;                       // words of one mark array
+  // line items walked together with marked writes (fewer in sum and count,
+  // which group only their dedup lines this way: there the registers of
+  // larger groups spill; 2, 4, 8 and 16 measured)
+  constexpr int KGRP = TW == 64 ? (COUNTED ? 4 : 8) : 2;
+  constexpr int CNT_WORDS = COUNTED ? NPIXS / 2 : 1;   // u16 counters of one item, 2 per word
+  // counted line items walked together: u8 counters of NPIXS / 4 words each
+  // in the counter words (more items, with the edge stage as counters too,
+  // measured slower: the kernel spills more registers)
+  constexpr int KCNT = COUNTED ? (CNT_WORDS * 4 / NPIXS < KGRP ? CNT_WORDS * 4 / NPIXS : KGRP) : 1;
   __shared__ PolyEdge s_edge[4][64];
   __shared__ uint64_t s_mask[4][2][WAVE_ROWS][NWORD];
   __shared__ __align__(16) T s_acc[4][NPIXS];
   __shared__ __align__(16) uint32_t s_cnt[4][CNT_WORDS];
-  __shared__ uint32_t s_mark[4][NPIXS / 32];
+  __shared__ uint32_t s_mark[4][KGRP * MW];
+  __shared__ GroupItem s_grp[4][KGRP];
   // unordered float sum: the strip holds plain sums and s_wr which pixels have any
   constexpr bool FSUM = FUN == 0 && (sizeof(T) == 4 || sizeof(T) == 8) && (T)0.5 != (T)0;
   __shared__ uint32_t s_wr[4][FSUM ? NPIXS / 32 : 1];
@@ -435,7 +461,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // (every apply leaves its counters and marks at zero)
   if (COUNTED)
     for (int w = lane; w < CNT_WORDS; w += 64) cnt[w] = 0u;
-  for (int w = lane; w < NPIXS / 32; w += 64) mark[w] = 0u;
+  for (int w = lane; w < KGRP * MW; w += 64) mark[w] = 0u;
   unsigned long long burns = 0;
   int mbuf = 0;
 
@@ -511,8 +537,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     };
 
     // all-touched: the cells of the kept segment from local vertex lv that
-    // fall in this strip, as marks (the walker stops past the strip)
-    auto walk_at = [&](int64_t lv) {
+    // fall in this strip, as marks in mk (the walker stops past the strip)
+    auto walk_at = [&](int64_t lv, uint32_t* mk) {
       double x0, y0, x1, y1;
       if (!kept_segment(J, lv, x0, y0, x1, y1)) return;
       // (cells lie within a pixel of the floors of the end points)
@@ -522,7 +548,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       const bool ok = walk_all_touched(x0, y0, x1, y1, J.ncols, J.nrows, u.C0, u.tile_c1, u.Rw, u.strip_r1,
                                        [&](int64_t x, int64_t y) {
                                          const uint32_t idx = (uint32_t)(y - u.Rw) * TW + (uint32_t)(x - u.C0);
-                                         atomicOr(&mark[idx >> 5], 1u << (idx & 31u));
+                                         atomicOr(&mk[idx >> 5], 1u << (idx & 31u));
                                        });
       if (!ok) atomicOr(a.status, 64u);
     };
@@ -536,7 +562,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // skipped when the cache bit its wrapped index lands on is set.
     auto polygon_at = [&](const ItemDesc& d, uint32_t pi) {
       const T v = from_bits<T>(d.value);
-      for (int64_t e = lane; e < d.nv; e += 64) walk_at(d.vbeg + e - J.v_base);
+      for (int64_t e = lane; e < d.nv; e += 64) walk_at(d.vbeg + e - J.v_base, mark);
       const uint64_t cov = polygon_mask<NWORD>(a, d, u, hb_ok, hbase, pb, pe, pi, strip, stage, lane);
       wave_sync();
       if (q < NWORD) {
@@ -635,7 +661,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             continue;
           }
           if (at_job) {  // the all-touched walker, from the world coordinates
-            walk_at(d.vbeg + (int64_t)e - J.v_base);
+            walk_at(d.vbeg + (int64_t)e - J.v_base, mark);
             continue;
           }
           if (r.x == kLsFar) {
@@ -714,6 +740,187 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       return (uint32_t)lane < n_r ? recs[lane] : make_uint4(kLsNone, 0u, 0u, 0u);
     };
 
+    // The line items of lanes G (two or more, consecutive in item order):
+    // each lane's own item is dl with records [rp, rp + rn).  The lanes take
+    // the items' records in turn, item k writing mark array k (cmode false)
+    // or 8-bit counter array k (cmode: sum and count without dedup, at most
+    // 255 records per item, and a record writes a pixel at most once); then
+    // each lane folds its pixels' writes item by item.
+    auto line_group = [&](uint64_t G, const ItemDesc& dl, const uint4* rp, uint32_t rn, bool bnd, bool cmode) {
+      GroupItem* gi = s_grp[wave];
+      const int ng = __popcll(G);
+      if ((G >> lane) & 1ull) {
+        GroupItem it;
+        it.recs = rp;
+        it.vbeg = dl.vbeg;
+        it.last = dl.last;
+        it.kb = (dl.kind_layer & 3u) | (bnd ? 4u : 0u);
+        it.pad = 0u;
+        gi[__popcll(G & ((1ull << lane) - 1ull))] = it;
+      }
+      // record offsets and values of the items (uniform)
+      uint32_t pre[KGRP];
+      T vk[KGRP];
+      uint32_t tot = 0;
+      uint64_t g = G;
+#pragma unroll
+      for (int k = 0; k < KGRP; ++k) {
+        pre[k] = tot;
+        vk[k] = bg;
+        if (g) {
+          const int pk = __builtin_ctzll(g);
+          g &= g - 1ull;
+          tot += (uint32_t)__builtin_amdgcn_readlane((int)rn, pk);
+          const uint64_t vb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(dl.value >> 32), pk) << 32) |
+                              (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)dl.value, pk);
+          vk[k] = from_bits<T>(vb);
+        }
+      }
+      // counter array of item k
+      auto carr = [&](int k) { return cnt + k * (NPIXS / 4); };
+      wave_sync();
+      // record e of the group: its item k and index ei in that item
+      auto locate = [&](uint32_t e, int& k, uint32_t& ei) {
+        uint32_t b = 0;
+        k = 0;
+#pragma unroll
+        for (int j = 1; j < KGRP; ++j)
+          if (e >= pre[j]) {
+            k = j;
+            b = pre[j];
+          }
+        ei = e - b;
+      };
+      bool far = false;
+      for (uint32_t e = (uint32_t)lane; e < tot; e += 64u) {
+        int k;
+        uint32_t ei;
+        locate(e, k, ei);
+        const GroupItem it = gi[k];
+        uint32_t* mk = mark + k * MW;
+        uint32_t* ck = carr(k);
+        auto emit = [&](auto cx, auto cy) {
+          if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+          const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
+          if (COUNTED && cmode) atomicAdd(&ck[idx >> 2], 1u << ((idx & 3u) * 8u));
+          else atomicOr(&mk[idx >> 5], 1u << (idx & 31u));
+        };
+        const uint4 r = it.recs[ei];
+        if (r.x == kLsNone) continue;
+        if ((it.kb & 3u) == 2u) {  // a vertex of a multi-point: its cell
+          emit((int32_t)r.x, (int32_t)r.y);
+          continue;
+        }
+        if (at_job) {
+          walk_at(it.vbeg + (int64_t)ei - J.v_base, mk);
+          continue;
+        }
+        if (r.x == kLsFar) {
+          far = true;
+          continue;
+        }
+        const int32_t ix0 = (int32_t)((int64_t)r.x - kLsBias), iy0 = (int32_t)((int64_t)r.y - kLsBias);
+        const int32_t ix1 = (int32_t)((int64_t)r.z - kLsBias);
+        const int32_t iy1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
+        walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit);
+        const bool endc = (it.kb & 4u) ? (r.w & kLsFlag) != 0
+                                       : ((uint64_t)(it.vbeg + (int64_t)ei + 1) == it.last && !(r.w & kLsFlag));
+        if (endc) emit(ix1, iy1);
+      }
+      if (__any(far)) {
+        for (uint32_t e = (uint32_t)lane; e < tot; e += 64u) {
+          int k;
+          uint32_t ei;
+          locate(e, k, ei);
+          const GroupItem it = gi[k];
+          const uint4 r = it.recs[ei];
+          if (r.x != kLsFar) continue;
+          uint32_t* mk = mark + k * MW;
+          uint32_t* ck = carr(k);
+          auto emit = [&](auto cx, auto cy) {
+            if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+            const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
+            if (COUNTED && cmode) atomicAdd(&ck[idx >> 2], 1u << ((idx & 3u) * 8u));
+            else atomicOr(&mk[idx >> 5], 1u << (idx & 31u));
+          };
+          const int64_t lv = it.vbeg + (int64_t)ei - J.v_base;
+          auto px_x = [&](double X) { return isnan(X) ? X : (X - J.xmin) / J.xres; };
+          auto px_y = [&](double Y) { return isnan(Y) ? Y : (J.ymax - Y) / J.yres; };
+          const int64_t ix0 = sat_isize(floor(px_x(J.xy[2 * lv]))), iy0 = sat_isize(floor(px_y(J.xy[2 * lv + 1])));
+          const int64_t ix1 = sat_isize(floor(px_x(J.xy[2 * lv + 2]))), iy1 = sat_isize(floor(px_y(J.xy[2 * lv + 3])));
+          const int64_t lim = (int64_t)1 << 60;
+          if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim && iy1 > -lim &&
+              iy1 < lim)
+            walk_window<__int128>(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit);
+          else
+            atomicOr(a.status, 1u);  // pixel coordinates beyond +-2^60: unsupported
+          const bool endc = (it.kb & 4u) ? (r.w & kLsFlag) != 0
+                                         : ((uint64_t)(it.vbeg + (int64_t)ei + 1) == it.last && !(r.w & kLsFlag));
+          if (endc) emit(ix1, iy1);
+        }
+      }
+      wave_sync();
+      if (COUNTED && cmode) {
+        // each lane folds its pixels k times per item, item by item, and
+        // clears their counters (its own words)
+#pragma unroll
+        for (int c0 = 0; c0 < PPL; c0 += 16) {
+          const int base = lane * PPL + c0;
+          T px[16];
+          bool ld = false;
+#pragma unroll
+          for (int k = 0; k < KCNT; ++k) {
+            if (k >= ng) break;
+            uint32_t w[4];
+            __builtin_memcpy(w, carr(k) + base / 4, 16);
+            if (w[0] | w[1] | w[2] | w[3]) {
+              if (!ld) __builtin_memcpy(px, acc + base, sizeof(px));
+              ld = true;
+#pragma unroll
+              for (int j = 0; j < 16; ++j) {
+                const uint32_t c = (w[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+                burns += c;
+                fold<T, FUN>(px[j], vk[k], bg, c);
+              }
+              const uint32_t z[4] = {0u, 0u, 0u, 0u};
+              __builtin_memcpy(carr(k) + base / 4, z, 16);
+            }
+          }
+          if (ld) __builtin_memcpy(acc + base, px, sizeof(px));
+        }
+        wave_sync();
+        return;
+      }
+      // each lane folds its pixels' marks, item by item, and clears them
+#pragma unroll
+      for (int c0 = 0; c0 < PPL; c0 += 16) {
+        const int base = lane * PPL + c0;
+        uint32_t bk[KGRP];
+        uint32_t any = 0u;
+#pragma unroll
+        for (int k = 0; k < KGRP; ++k) {
+          bk[k] = k < ng ? (mark[k * MW + (base >> 5)] >> (base & 31)) & 0xFFFFu : 0u;
+          any |= bk[k];
+        }
+        if (any) {
+          T px[16];
+          __builtin_memcpy(px, acc + base, sizeof(px));
+#pragma unroll
+          for (int k = 0; k < KGRP; ++k) {
+            if (bk[k]) {
+              burns += (unsigned long long)__popc(bk[k]);
+#pragma unroll
+              for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], vk[k], bg, (bk[k] >> j) & 1u);
+            }
+          }
+          __builtin_memcpy(acc + base, px, sizeof(px));
+        }
+      }
+      wave_sync();
+      for (int w = lane; w < ng * MW; w += 64) mark[w] = 0u;
+      wave_sync();
+    };
+
     auto process_item = [&](const ItemDesc& d, uint32_t pi) {
       if (d.r1 <= u.Rw || d.r0 >= u.strip_r1 || d.c1 <= u.C0 || d.c0 >= u.tile_c1) return;
       if ((d.kind_layer & 3u) == 0u) {
@@ -737,7 +944,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       // records are loaded while the current one is walked.
       for (uint32_t i0 = pb; i0 < pe; i0 += 64u) {
         const uint32_t nb = min(64u, pe - i0);
-        bool pt = false, ln = false, po = false, bnd = false;
+        bool pt = false, ln = false, po = false, bnd = false, grp = false;
         uint32_t pidx = 0, rn = 0;
         uint64_t pv = 0;
         const uint4* rp = nullptr;
@@ -755,8 +962,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             pv = dl.value;
           }
           if (ln) bnd = records(dl, rp, rn);
+          // lines walked in groups: those whose writes are marks, or counted
+          // ones (sum, count without dedup) of at most 255 records here
+          grp = ln && (!COUNTED ? true : (dedup_job || at_job) ? kind == 1u : (!unord && rn <= 255u));
         }
-        const uint64_t ptm = __ballot(pt), lnm = __ballot(ln);
+        const uint64_t ptm = __ballot(pt), lnm = __ballot(ln), grm = __ballot(grp);
         uint64_t todo = ptm | lnm | __ballot(po);
         // the first records of line item q (a uniform lane index)
         auto prefetch = [&](int q) {
@@ -795,6 +1005,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             }
             todo &= ~run;
             continue;
+          }
+          if ((grm >> p) & 1ull) {
+            // the run of groupable lines up to the next other item, at most KGRP
+            const uint64_t other = todo & ~grm & (~0ull << p);
+            uint64_t run = todo & grm & (other ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull);
+            const bool cmode = COUNTED && !dedup_job && !at_job;
+            uint64_t G = 0;
+#pragma unroll
+            for (int k = 0; k < KGRP; ++k) {
+              if (!cmode || k < KCNT) {
+                G |= run & (0ull - run);
+                run &= run - 1ull;
+              }
+            }
+            if (G & (G - 1ull)) {
+              line_group(G, dl, rp, rn, bnd, cmode);
+              todo &= ~G;
+              lrest &= ~G;
+              if (lrest) pre = prefetch(__builtin_ctzll(lrest));
+              continue;
+            }
           }
           todo &= todo - 1;
           const ItemDesc d = bcast_desc(dl, p);  // lane p's copy, no reload

@@ -260,6 +260,58 @@ def test_line_point_kernel_paths(binning, monkeypatch):
             assert_same(got, ref, f"line paths {binning} {dtype} {fun} xres={xres}")
 
 
+
+GROUP_CACHE = {}
+
+
+def line_group_geometry():
+    """Runs of short line items that meet one strip together (walked as one
+    group, each item writing its own marks or counters), overlapping so the
+    item order decides first / last / sum; in between a line of more than
+    255 segments in one strip (walked alone under sum and count), two-vertex
+    lines, multi-points with repeated cells, one-vertex points and a polygon,
+    each ending a run."""
+    if "g" not in GROUP_CACHE:
+        rng = np.random.default_rng(41)
+        wkts = []
+        for run in range(12):
+            x0, y0 = rng.uniform(0, 180, 2)
+            for _ in range(int(rng.integers(2, 14))):
+                wkts.append(zigzag(x0, y0, 20.0, 10.0, int(rng.integers(2, 9)), int(rng.integers(1 << 30))))
+            if run == 3:
+                wkts.append(zigzag(x0, y0 + 2.0, 60.0, 12.0, 400, 7))   # > 255 segments in one strip
+            if run == 5:
+                wkts.append(f"MULTIPOINT (({x0 + 1.5} {y0 + 1.5}), ({x0 + 1.5} {y0 + 1.5}), ({x0 + 2.5} {y0 + 1.5}))")
+            if run == 7:
+                wkts.append(f"POINT ({x0 + 3.5} {y0 + 2.5})")
+            if run == 9:
+                wkts.append(sq(x0, y0, x0 + 12, y0 + 7))
+            if run == 10:
+                wkts.append(f"LINESTRING ({x0} {y0}, {x0 + 40.5} {y0 + 3.25})")
+        GROUP_CACHE["g"] = Geometry.from_wkt(wkts)
+    return GROUP_CACHE["g"]
+
+
+@pytest.mark.parametrize("binning", [None, "always", "never"])
+def test_line_item_groups(binning, monkeypatch):
+    if binning:
+        monkeypatch.setenv("RUSTERIZE_BINNING", binning)
+    g = line_group_geometry()
+    for dtype, fun, bg in [("float32", "sum", float("nan")), ("float64", "sum", 0), ("int32", "count", 0),
+                           ("uint8", "sum", 0), ("uint16", "count", 0), ("float32", "first", float("nan")),
+                           ("int32", "last", 0), ("float64", "max", float("nan")), ("uint8", "min", 0),
+                           ("int16", "any", 0)]:
+        field = field_for(dtype, g.n_geoms, seed=43)
+        for xres in (1.0, 0.75):  # 0.75: non-square pixels, one write per pixel per line
+            info = _lib.raster_info(None, resolution=(xres, 1.0), extent=(0, 0, 220, 210))
+            got, ref = run_both(g, info, dtype, fun, field=field, bg=bg)
+            assert_same(got, ref, f"line groups {binning} {dtype} {fun} xres={xres}")
+            if xres == 1.0 and fun in ("sum", "last"):
+                got = _lib.rasterize(g, info, dtype, fun, field, None, None, 1, bg, all_touched=True)
+                ref = oracle.rasterize(g, oracle.copy_info(info), dtype, fun, field=field, background=bg,
+                                       all_touched=True)
+                assert_same(got, ref, f"line groups all_touched {binning} {dtype} {fun}")
+
 def sum_error_bound(g, info, dtype, field):
     """8 n eps sum|v| per pixel: the rounding of n additions in any order,
     from the oracle's count of writes and its sum of |v|."""

@@ -660,6 +660,13 @@ __host__ __device__ inline void walk_window(int64_t ix0, int64_t iy0, int64_t ix
 // set up in 64 bits and the loop stepped in 32 (its positions stay inside
 // the window; the remainder test rem + 2m >= 2n is rem >= 2n - 2m, so the
 // remainder, below 2n < 2^32, never overflows).
+// floor(a / b) for a >= 0, b > 0: a 32-bit division when both fit (short
+// segments), else div_floor_pos.
+__host__ __device__ inline int64_t div_pos32(int64_t a, int64_t b) {
+  if ((((uint64_t)a | (uint64_t)b) >> 32) == 0) return (int64_t)((uint32_t)a / (uint32_t)b);
+  return div_floor_pos<int64_t>(a, b);
+}
+
 template <typename F>
 __host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t ix1, int32_t iy1,
                                               int32_t X0, int32_t X1, int32_t Y0, int32_t Y1, F&& emit) {
@@ -692,18 +699,23 @@ __host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t 
     if (jlo > 0) return;
   } else {
     if (jlo > 0) {
-      const int64_t kmin = div_floor_pos<int64_t>(two_n * jlo - n + two_m - 1, two_m);
+      const int64_t kmin = div_pos32(two_n * jlo - n + two_m - 1, two_m);
       if (kmin > klo) klo = kmin;
     }
     if (jhi < m) {
-      const int64_t kmax = div_floor_pos<int64_t>(two_n * (jhi + 1) - n + two_m - 1, two_m) - 1;
+      const int64_t kmax = div_pos32(two_n * (jhi + 1) - n + two_m - 1, two_m) - 1;
       if (kmax < khi) khi = kmax;
     }
   }
   if (klo > khi) return;
-  const int64_t numer = two_m * klo + n;
-  const int64_t j = div_floor_pos<int64_t>(numer, two_n);
-  uint32_t rem = (uint32_t)(numer - j * two_n);
+  // (from the first cell: j = floor(n / 2n) = 0, remainder n)
+  int64_t j = 0;
+  uint32_t rem = (uint32_t)n;
+  if (klo > 0) {
+    const int64_t numer = two_m * klo + n;
+    j = div_pos32(numer, two_n);
+    rem = (uint32_t)(numer - j * two_n);
+  }
   const uint32_t tm = (uint32_t)two_m, gap = (uint32_t)(two_n - two_m);
   int32_t u = u0 + su * (int32_t)klo, w = w0 + sw * (int32_t)j;
   const int32_t cnt = (int32_t)(khi - klo + 1);

@@ -633,8 +633,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       // line's first write per pixel with xres != yres or all-touched) or a
       // count; both are LDS atomics whose result is not waited for
       const bool marked = !COUNTED || dedup;
-      auto emit = [&](auto cx, auto cy) {
-        if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+      // a write at a cell of the strip (emit_in: the clipped walk's cells,
+      // inside by construction)
+      auto emit_in = [&](int32_t cx, int32_t cy) {
         const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
         if (FSUM && unord) {
           ++burns;
@@ -644,6 +645,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         } else {
           atomicAdd(&cnt[idx >> 1], 1u << ((idx & 1u) * 16u));
         }
+      };
+      auto emit = [&](auto cx, auto cy) {
+        if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+        emit_in((int32_t)cx, (int32_t)cy);
       };
       // end cell of the last kept edge of an open line (burners.rs:86-89)
       auto end_cell = [&](const uint4& r, uint32_t e) {
@@ -671,7 +676,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           const int32_t ix0 = (int32_t)((int64_t)r.x - kLsBias), iy0 = (int32_t)((int64_t)r.y - kLsBias);
           const int32_t ix1 = (int32_t)((int64_t)r.z - kLsBias);
           const int32_t iy1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
-          walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit);
+          walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit_in);
           if (end_cell(r, e)) emit(ix1, iy1);
         }
         if (__any(far)) {
@@ -799,11 +804,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const GroupItem it = gi[k];
         uint32_t* mk = mark + k * MW;
         uint32_t* ck = carr(k);
-        auto emit = [&](auto cx, auto cy) {
-          if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+        auto emit_in = [&](int32_t cx, int32_t cy) {
           const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
           if (COUNTED && cmode) atomicAdd(&ck[idx >> 2], 1u << ((idx & 3u) * 8u));
           else atomicOr(&mk[idx >> 5], 1u << (idx & 31u));
+        };
+        auto emit = [&](auto cx, auto cy) {
+          if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
+          emit_in((int32_t)cx, (int32_t)cy);
         };
         const uint4 r = it.recs[ei];
         if (r.x == kLsNone) continue;
@@ -822,7 +830,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int32_t ix0 = (int32_t)((int64_t)r.x - kLsBias), iy0 = (int32_t)((int64_t)r.y - kLsBias);
         const int32_t ix1 = (int32_t)((int64_t)r.z - kLsBias);
         const int32_t iy1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
-        walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit);
+        walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit_in);
         const bool endc = (it.kb & 4u) ? (r.w & kLsFlag) != 0
                                        : ((uint64_t)(it.vbeg + (int64_t)ei + 1) == it.last && !(r.w & kLsFlag));
         if (endc) emit(ix1, iy1);

@@ -77,6 +77,28 @@ def test_bresenham_far_endpoints_128bit():
             reference_cells(ix0, iy0, ix1, iy1, X0, X0 + 64, Y0, Y0 + 16)
 
 
+
+def test_bresenham_long_segments_wide_numerators():
+    # segments of 1e5-3e5 major steps cut by 64 x 16 windows along them: the
+    # window set-up's numerators (2n * j) pass 2^32, beyond its 32-bit
+    # divisions, and must still give the literal loop's cells
+    rng = np.random.default_rng(11)
+    for _ in range(12):
+        n = int(rng.integers(100_000, 300_000))
+        m = int(rng.integers(0, n + 1))
+        du, dw = (n, m) if rng.integers(2) else (m, n)
+        ix0, iy0 = (int(v) for v in rng.integers(-1000, 1000, 2))
+        ix1 = ix0 + du * (1 if rng.integers(2) else -1)
+        iy1 = iy0 + dw * (1 if rng.integers(2) else -1)
+        xs, ys = oracle.bresenham_cells(ix0, iy0, ix1, iy1)
+        for c in rng.integers(0, len(xs), 8):
+            X0 = int(xs[c]) - int(rng.integers(0, 64))
+            Y0 = int(ys[c]) - int(rng.integers(0, 16))
+            X1, Y1 = X0 + 64, Y0 + 16
+            inside = (xs >= X0) & (xs < X1) & (ys >= Y0) & (ys < Y1)
+            assert walk(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1) == \
+                list(zip(xs[inside].tolist(), ys[inside].tolist()))
+
 # ---------------------------------------------------------- row coverage
 
 def total_key(x):

@@ -425,7 +425,8 @@ rz_status rz_context_last_counters(rz_context* ctx, uint64_t* burns, uint64_t* p
  * rz_debug_walk_cells: the cells Standard::burn_line
  * (rust/src/rasterization/burners.rs:53-84) visits between integer cells
  * (ix0,iy0) -> (ix1,iy1), end cell excluded, restricted to [X0,X1) x [Y0,Y1);
- * returns the count and stores up to cap cells.
+ * returns the count and stores up to cap cells (-2: the kernel's
+ * index-stepped form of the walk gave other cells).
  * rz_debug_row_coverage: which of the columns [C0, C0+width) the spans of
  * burn_polygon (burners.rs:302-316) cover for the crossings xs[0..n) of one
  * row; width is 64, 128 or 256.

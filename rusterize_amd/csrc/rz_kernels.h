@@ -655,11 +655,6 @@ __host__ __device__ inline void walk_window(int64_t ix0, int64_t iy0, int64_t ix
   }
 }
 
-// walk_window for end cells within +-2^30 and a window inside the int32
-// range (every tile strip): the same cells in the same order, the k range
-// set up in 64 bits and the loop stepped in 32 (its positions stay inside
-// the window; the remainder test rem + 2m >= 2n is rem >= 2n - 2m, so the
-// remainder, below 2n < 2^32, never overflows).
 // floor(a / b) for a >= 0, b > 0: a 32-bit division when both fit (short
 // segments), else div_floor_pos.
 __host__ __device__ inline int64_t div_pos32(int64_t a, int64_t b) {
@@ -667,16 +662,24 @@ __host__ __device__ inline int64_t div_pos32(int64_t a, int64_t b) {
   return div_floor_pos<int64_t>(a, b);
 }
 
-template <typename F>
-__host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t ix1, int32_t iy1,
-                                              int32_t X0, int32_t X1, int32_t Y0, int32_t Y1, F&& emit) {
+// The set-up of walk_window32: the first cell in the window as (u, w) along
+// the major / minor axes, their steps, the remainder state and the number
+// of cells; false when no cell falls in the window.
+struct Walk32 {
+  int32_t u, w, su, sw, cnt;
+  uint32_t rem, tm, gap;
+  bool xmaj;
+};
+
+__host__ __device__ inline bool walk32_setup(int32_t ix0, int32_t iy0, int32_t ix1, int32_t iy1, int32_t X0,
+                                             int32_t X1, int32_t Y0, int32_t Y1, Walk32& s) {
   const int32_t dxs = ix1 - ix0, dys = iy1 - iy0;
   const int64_t dx = dxs < 0 ? -(int64_t)dxs : dxs, D = dys < 0 ? -(int64_t)dys : dys;
   const int32_t sx = ix0 < ix1 ? 1 : -1, sy = iy0 < iy1 ? 1 : -1;
   const bool xmaj = dx >= D;
   const int64_t n = xmaj ? dx : D;  // iterations (major steps)
   const int64_t m = xmaj ? D : dx;  // minor steps in total
-  if (n == 0) return;
+  if (n == 0) return false;
   const int32_t u0 = xmaj ? ix0 : iy0, w0 = xmaj ? iy0 : ix0;
   const int32_t su = xmaj ? sx : sy, sw = xmaj ? sy : sx;
   const int64_t U0 = xmaj ? X0 : Y0, U1 = xmaj ? X1 : Y1;
@@ -686,17 +689,17 @@ __host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t 
     int64_t lo, hi;  // k in [lo, hi] from the major-axis window
     if (su > 0) { lo = U0 - u0; hi = U1 - 1 - u0; }
     else { lo = u0 - (U1 - 1); hi = u0 - U0; }
-    if (hi < 0 || lo >= n || lo > hi) return;
+    if (hi < 0 || lo >= n || lo > hi) return false;
     klo = lo < 0 ? 0 : lo;
     khi = hi >= n ? n - 1 : hi;
   }
   int64_t jlo, jhi;  // minor index window
   if (sw > 0) { jlo = W0 - w0; jhi = W1 - 1 - w0; }
   else { jlo = w0 - (W1 - 1); jhi = w0 - W0; }
-  if (jhi < 0 || jlo > m || jlo > jhi) return;
+  if (jhi < 0 || jlo > m || jlo > jhi) return false;
   const int64_t two_n = 2 * n, two_m = 2 * m;
   if (m == 0) {
-    if (jlo > 0) return;
+    if (jlo > 0) return false;
   } else {
     if (jlo > 0) {
       const int64_t kmin = div_pos32(two_n * jlo - n + two_m - 1, two_m);
@@ -707,7 +710,7 @@ __host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t 
       if (kmax < khi) khi = kmax;
     }
   }
-  if (klo > khi) return;
+  if (klo > khi) return false;
   // (from the first cell: j = floor(n / 2n) = 0, remainder n)
   int64_t j = 0;
   uint32_t rem = (uint32_t)n;
@@ -716,18 +719,63 @@ __host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t 
     j = div_pos32(numer, two_n);
     rem = (uint32_t)(numer - j * two_n);
   }
-  const uint32_t tm = (uint32_t)two_m, gap = (uint32_t)(two_n - two_m);
-  int32_t u = u0 + su * (int32_t)klo, w = w0 + sw * (int32_t)j;
-  const int32_t cnt = (int32_t)(khi - klo + 1);
-  for (int32_t i = 0; i < cnt; ++i) {
-    if (xmaj) emit(u, w);
+  s.u = u0 + su * (int32_t)klo;
+  s.w = w0 + sw * (int32_t)j;
+  s.su = su;
+  s.sw = sw;
+  s.cnt = (int32_t)(khi - klo + 1);
+  s.rem = rem;
+  s.tm = (uint32_t)two_m;
+  s.gap = (uint32_t)(two_n - two_m);
+  s.xmaj = xmaj;
+  return true;
+}
+
+// walk_window for end cells within +-2^30 and a window inside the int32
+// range (every tile strip): the same cells in the same order, the k range
+// set up in 64 bits and the loop stepped in 32 (its positions stay inside
+// the window; the remainder test rem + 2m >= 2n is rem >= 2n - 2m, so the
+// remainder, below 2n < 2^32, never overflows).
+template <typename F>
+__host__ __device__ inline void walk_window32(int32_t ix0, int32_t iy0, int32_t ix1, int32_t iy1,
+                                              int32_t X0, int32_t X1, int32_t Y0, int32_t Y1, F&& emit) {
+  Walk32 s;
+  if (!walk32_setup(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, s)) return;
+  int32_t u = s.u, w = s.w;
+  uint32_t rem = s.rem;
+  for (int32_t i = 0; i < s.cnt; ++i) {
+    if (s.xmaj) emit(u, w);
     else emit(w, u);
-    u += su;
-    if (rem >= gap) {
-      rem -= gap;
-      w += sw;
+    u += s.su;
+    if (rem >= s.gap) {
+      rem -= s.gap;
+      w += s.sw;
     } else {
-      rem += tm;
+      rem += s.tm;
+    }
+  }
+}
+
+// walk_window32 giving each cell as its index (y - Y0) * stride + (x - X0)
+// in the window (stride >= X1 - X0), stepped along the walk.
+template <typename F>
+__host__ __device__ inline void walk_window32_idx(int32_t ix0, int32_t iy0, int32_t ix1, int32_t iy1, int32_t X0,
+                                                  int32_t X1, int32_t Y0, int32_t Y1, int32_t stride, F&& emit) {
+  Walk32 s;
+  if (!walk32_setup(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, s)) return;
+  const int32_t x = s.xmaj ? s.u : s.w, y = s.xmaj ? s.w : s.u;
+  uint32_t idx = (uint32_t)(y - Y0) * (uint32_t)stride + (uint32_t)(x - X0);
+  const uint32_t du = s.xmaj ? (uint32_t)s.su : (uint32_t)(s.su * stride);
+  const uint32_t dw = s.xmaj ? (uint32_t)(s.sw * stride) : (uint32_t)s.sw;
+  uint32_t rem = s.rem;
+  for (int32_t i = 0; i < s.cnt; ++i) {
+    emit(idx);
+    idx += du;
+    if (rem >= s.gap) {
+      rem -= s.gap;
+      idx += dw;
+    } else {
+      rem += s.tm;
     }
   }
 }

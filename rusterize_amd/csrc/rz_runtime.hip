@@ -1725,7 +1725,9 @@ extern "C" rz_status rz_rasterize(const rz_geometry* geoms, const rz_raster_info
 // ------------------------------------------------------------ test hooks --
 
 // Host evaluation of the Bresenham closed form used by the tile kernel
-// (walk_window in rz_kernels.h), for brute-force checks on the CPU.
+// (walk_window in rz_kernels.h), for brute-force checks on the CPU.  In the
+// 32-bit domain the index-stepped form (walk_window32_idx) must give the
+// same cells: -2 when it does not.
 extern "C" int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, int64_t iy1,
                                        int64_t X0, int64_t X1, int64_t Y0, int64_t Y1,
                                        int64_t* xs, int64_t* ys, int64_t cap) {
@@ -1739,10 +1741,23 @@ extern "C" int64_t rz_debug_walk_cells(int64_t ix0, int64_t iy0, int64_t ix1, in
   };
   const int64_t lim = (int64_t)1 << 30, wl = (int64_t)1 << 31;
   if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
-      iy1 > -lim && iy1 < lim && X0 > -wl && X1 < wl && Y0 > -wl && Y1 < wl && X0 <= X1 && Y0 <= Y1)
+      iy1 > -lim && iy1 < lim && X0 > -wl && X1 < wl && Y0 > -wl && Y1 < wl && X0 <= X1 && Y0 <= Y1) {
     walk_window32((int32_t)ix0, (int32_t)iy0, (int32_t)ix1, (int32_t)iy1, (int32_t)X0, (int32_t)X1,
                   (int32_t)Y0, (int32_t)Y1, emit);
-  else if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
+    const int64_t W = X1 - X0;
+    if (W > 0 && W * (Y1 - Y0) < ((int64_t)1 << 32)) {
+      int64_t k = 0;
+      bool same = true;
+      walk_window32_idx((int32_t)ix0, (int32_t)iy0, (int32_t)ix1, (int32_t)iy1, (int32_t)X0, (int32_t)X1,
+                        (int32_t)Y0, (int32_t)Y1, (int32_t)W, [&](uint32_t idx) {
+                          if (k < n && k < cap && (xs[k] != X0 + (int64_t)(idx % (uint64_t)W) ||
+                                                   ys[k] != Y0 + (int64_t)(idx / (uint64_t)W)))
+                            same = false;
+                          ++k;
+                        });
+      if (!same || k != n) return -2;
+    }
+  } else if (ix0 > -lim && ix0 < lim && ix1 > -lim && ix1 < lim && iy0 > -lim && iy0 < lim &&
            iy1 > -lim && iy1 < lim)
     walk_window<int64_t>(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, emit);
   else

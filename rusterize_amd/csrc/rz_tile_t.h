@@ -633,10 +633,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       // line's first write per pixel with xres != yres or all-touched) or a
       // count; both are LDS atomics whose result is not waited for
       const bool marked = !COUNTED || dedup;
-      // a write at a cell of the strip (emit_in: the clipped walk's cells,
-      // inside by construction)
-      auto emit_in = [&](int32_t cx, int32_t cy) {
-        const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
+      // a write at strip pixel idx (emit_idx: the clipped walk's cells,
+      // inside by construction) or at a cell
+      auto emit_idx = [&](uint32_t idx) {
         if (FSUM && unord) {
           ++burns;
           add1(idx, v);
@@ -648,7 +647,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       };
       auto emit = [&](auto cx, auto cy) {
         if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
-        emit_in((int32_t)cx, (int32_t)cy);
+        emit_idx((uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0));
       };
       // end cell of the last kept edge of an open line (burners.rs:86-89)
       auto end_cell = [&](const uint4& r, uint32_t e) {
@@ -676,7 +675,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           const int32_t ix0 = (int32_t)((int64_t)r.x - kLsBias), iy0 = (int32_t)((int64_t)r.y - kLsBias);
           const int32_t ix1 = (int32_t)((int64_t)r.z - kLsBias);
           const int32_t iy1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
-          walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit_in);
+          walk_window32_idx(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, TW, emit_idx);
           if (end_cell(r, e)) emit(ix1, iy1);
         }
         if (__any(far)) {
@@ -804,14 +803,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const GroupItem it = gi[k];
         uint32_t* mk = mark + k * MW;
         uint32_t* ck = carr(k);
-        auto emit_in = [&](int32_t cx, int32_t cy) {
-          const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
+        auto emit_idx = [&](uint32_t idx) {
           if (COUNTED && cmode) atomicAdd(&ck[idx >> 2], 1u << ((idx & 3u) * 8u));
           else atomicOr(&mk[idx >> 5], 1u << (idx & 31u));
         };
         auto emit = [&](auto cx, auto cy) {
           if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
-          emit_in((int32_t)cx, (int32_t)cy);
+          emit_idx((uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0));
         };
         const uint4 r = it.recs[ei];
         if (r.x == kLsNone) continue;
@@ -830,7 +828,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int32_t ix0 = (int32_t)((int64_t)r.x - kLsBias), iy0 = (int32_t)((int64_t)r.y - kLsBias);
         const int32_t ix1 = (int32_t)((int64_t)r.z - kLsBias);
         const int32_t iy1 = (int32_t)((int64_t)(r.w & ~kLsFlag) - kLsBias);
-        walk_window32(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, emit_in);
+        walk_window32_idx(ix0, iy0, ix1, iy1, u.C0, u.tile_c1, u.Rw, u.strip_r1, TW, emit_idx);
         const bool endc = (it.kb & 4u) ? (r.w & kLsFlag) != 0
                                        : ((uint64_t)(it.vbeg + (int64_t)ei + 1) == it.last && !(r.w & kLsFlag));
         if (endc) emit(ix1, iy1);

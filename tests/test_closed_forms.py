@@ -28,7 +28,7 @@ def walk(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1, cap=1 << 16):
     n = _lib.lib().rz_debug_walk_cells(ix0, iy0, ix1, iy1, X0, X1, Y0, Y1,
                                        xs.ctypes.data_as(POINTER(ctypes.c_int64)),
                                        ys.ctypes.data_as(POINTER(ctypes.c_int64)), cap)
-    assert n <= cap
+    assert 0 <= n <= cap  # (-2: the index-stepped walk disagrees)
     return list(zip(xs[:n].tolist(), ys[:n].tolist()))
 
 

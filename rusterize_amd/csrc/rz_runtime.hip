@@ -1229,7 +1229,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       HIP_TRY(ensure(c->stream, c->masks, (size_t)n_pairs * 4 * slot_words * 8));
       uint32_t* cover_count = (uint32_t*)c->cover_list.p + n_pairs;
       zero_words(cover_count, 1, s);
-      hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs)), dim3(256), 0, s, d_jobs,
+      hipLaunchKernelGGL(k_heavy_pairs, dim3(blocks_for((int64_t)n_pairs, kHeavyPairsBlock)),
+                         dim3(kHeavyPairsBlock), 0, s, d_jobs,
                          sorted_keys, sorted_vals, (const ItemDesc*)c->pair_desc.p, d_total, n_pairs,
                          (const uint32_t*)c->tile_off.p, (const uint64_t*)c->heavy_base.p,
                          (const uint64_t*)c->offsets.p, (uint32_t*)c->wide_j.p, (uint32_t*)c->cover_list.p,

@@ -38,12 +38,13 @@ __global__ void k_heavy_list(const uint32_t* heavy_flag, const uint32_t* heavy_r
 
 // The compact list of the pairs of heavy tiles that hold polygon items other
 // than wide ones (the only ones k_cover writes; k_wide_cover writes the wide
-// ones), appended with one atomic per wave.  For a wide item's pair it also
+// ones), appended with one atomic per block.  For a wide item's pair it also
 // records the item's place in the tile's pair list, at the pair's emission
 // index (k_bin_emit's order over the item's tile rectangle), where
 // k_wide_cover finds it with one load.
-__global__ void k_heavy_pairs(const JobDev* jobs, const uint32_t* pair_tile, const uint32_t* pair_item,
-                              const ItemDesc* pair_desc, const uint64_t* n_pairs, uint64_t cap,
+constexpr int kHeavyPairsBlock = 1024;
+__global__ __launch_bounds__(kHeavyPairsBlock) void k_heavy_pairs(const JobDev* jobs, const uint32_t* pair_tile,
+                              const uint32_t* pair_item, const ItemDesc* pair_desc, const uint64_t* n_pairs, uint64_t cap,
                               const uint32_t* tile_off, const uint64_t* heavy_base, const uint64_t* pair_off,
                               uint32_t* wide_j, uint32_t* cover_list, uint32_t* cover_count, uint32_t n_tiles) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,12 +67,20 @@ __global__ void k_heavy_pairs(const JobDev* jobs, const uint32_t* pair_tile, con
       }
     }
   }
+  // one atomic per block (the list's order does not matter)
+  __shared__ uint32_t s_n[kHeavyPairsBlock / 64], s_base;
+  const int wave = threadIdx.x >> 6;
   const uint64_t m = __ballot(poly);
-  if (m == 0) return;
-  uint32_t base = 0;
-  const int leader = __builtin_ctzll(m);
-  if (lane == leader) base = atomicAdd(cover_count, (uint32_t)__popcll(m));
-  base = (uint32_t)__shfl((int)base, leader, 64);
+  if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = 0;
+    for (int w = 0; w < kHeavyPairsBlock / 64; ++w) n += s_n[w];
+    s_base = n ? atomicAdd(cover_count, n) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = s_base;
+  for (int w = 0; w < wave; ++w) base += s_n[w];
   if (poly) cover_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
 }
 

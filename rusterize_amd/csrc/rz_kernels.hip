@@ -464,18 +464,29 @@ __global__ void k_item_desc(DescArgs a) {
 // Items whose band entries would pass the capacity stay unbanded (their
 // strips then scan the whole edge list: slower, same result).
 // Assigned items are also appended to the list the band passes walk (any
-// order: every item owns its entries), one atomic per wave.
-__global__ void k_band_assign(ItemDesc* desc, const uint32_t* band_n, const uint64_t* band_base,
-                              int64_t n_items, uint64_t cap_entries, uint32_t* items, uint32_t* n_list) {
+// order: every item owns its entries), one atomic per block (one per wave
+// contended measurably on millions of items).
+constexpr int kBandAssignBlock = 1024;
+__global__ __launch_bounds__(kBandAssignBlock) void k_band_assign(ItemDesc* desc, const uint32_t* band_n,
+                                                                  const uint64_t* band_base, int64_t n_items,
+                                                                  uint64_t cap_entries, uint32_t* items,
+                                                                  uint32_t* n_list) {
+  __shared__ uint32_t s_n[kBandAssignBlock / 64], s_base;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = i < n_items && band_n[i] && band_base[i] + band_n[i] <= cap_entries;
   if (ok) desc[i].band = (uint32_t)band_base[i] + 1u;
   const uint64_t m = __ballot(ok);
-  if (m == 0) return;
-  const int lane = threadIdx.x & 63, leader = __builtin_ctzll(m);
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(n_list, (uint32_t)__popcll(m));
-  base = (uint32_t)__shfl((int)base, leader, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = 0;
+    for (int w = 0; w < kBandAssignBlock / 64; ++w) n += s_n[w];
+    s_base = n ? atomicAdd(n_list, n) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = s_base;
+  for (int w = 0; w < wave; ++w) base += s_n[w];
   if (ok) items[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
 }
 

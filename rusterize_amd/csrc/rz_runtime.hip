@@ -1061,7 +1061,8 @@ static rz_status run_batch(rz_context* c, const rz_job* jobs_in, int64_t n_in,
       HIP_TRY(ensure(c->stream, c->band_items, ((size_t)I + 1) * 4));
       uint32_t* n_list = (uint32_t*)c->band_items.p + I;
       zero_words(n_list, 1, s);
-      hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I)), dim3(256), 0, s, (ItemDesc*)c->desc.p,
+      hipLaunchKernelGGL(k_band_assign, dim3(blocks_for(I, kBandAssignBlock)), dim3(kBandAssignBlock), 0, s,
+                         (ItemDesc*)c->desc.p,
                          (const uint32_t*)c->band_n.p, (const uint64_t*)c->band_base.p, I, nb,
                          (uint32_t*)c->band_items.p, n_list);
       BandArgs bd{};

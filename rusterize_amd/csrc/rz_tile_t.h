@@ -435,10 +435,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // larger groups spill; 2, 4, 8 and 16 measured)
   constexpr int KGRP = TW == 64 ? (COUNTED ? 4 : 8) : 2;
   constexpr int CNT_WORDS = COUNTED ? NPIXS / 2 : 1;   // u16 counters of one item, 2 per word
-  // counted line items walked together: u8 counters of NPIXS / 4 words each
-  // in the counter words (more items, with the edge stage as counters too,
-  // measured slower: the kernel spills more registers)
-  constexpr int KCNT = COUNTED ? (CNT_WORDS * 4 / NPIXS < KGRP ? CNT_WORDS * 4 / NPIXS : KGRP) : 1;
+  // counted line items walked together: CB-bit counters (NPIXS * CB / 32
+  // words per item) in the counter words, for items of at most 2^CB - 1
+  // records in the strip (a record writes a pixel at most once).  More
+  // items measured slower, whether with more LDS, with the edge stage as
+  // counters or with 4-bit counters (4 items): the kernel spills more.
+  constexpr int CB = 8;
+  constexpr int CW = NPIXS * CB / 32;                  // words of one item's counters
+  constexpr int KCNT = COUNTED ? (CNT_WORDS / CW < KGRP ? CNT_WORDS / CW : KGRP) : 1;
   __shared__ PolyEdge s_edge[4][64];
   __shared__ uint64_t s_mask[4][2][WAVE_ROWS][NWORD];
   __shared__ __align__(16) T s_acc[4][NPIXS];
@@ -781,7 +785,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
       }
       // counter array of item k
-      auto carr = [&](int k) { return cnt + k * (NPIXS / 4); };
+      auto carr = [&](int k) { return cnt + k * CW; };
       wave_sync();
       // record e of the group: its item k and index ei in that item
       auto locate = [&](uint32_t e, int& k, uint32_t& ei) {
@@ -804,7 +808,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         uint32_t* mk = mark + k * MW;
         uint32_t* ck = carr(k);
         auto emit_idx = [&](uint32_t idx) {
-          if (COUNTED && cmode) atomicAdd(&ck[idx >> 2], 1u << ((idx & 3u) * 8u));
+          if (COUNTED && cmode) atomicAdd(&ck[idx / (32 / CB)], 1u << ((idx % (32 / CB)) * CB));
           else atomicOr(&mk[idx >> 5], 1u << (idx & 31u));
         };
         auto emit = [&](auto cx, auto cy) {
@@ -846,7 +850,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           auto emit = [&](auto cx, auto cy) {
             if (cx < u.C0 || cx >= u.tile_c1 || cy < u.Rw || cy >= u.strip_r1) return;
             const uint32_t idx = (uint32_t)(cy - u.Rw) * TW + (uint32_t)(cx - u.C0);
-            if (COUNTED && cmode) atomicAdd(&ck[idx >> 2], 1u << ((idx & 3u) * 8u));
+            if (COUNTED && cmode) atomicAdd(&ck[idx / (32 / CB)], 1u << ((idx % (32 / CB)) * CB));
             else atomicOr(&mk[idx >> 5], 1u << (idx & 31u));
           };
           const int64_t lv = it.vbeg + (int64_t)ei - J.v_base;
@@ -877,19 +881,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
           for (int k = 0; k < KCNT; ++k) {
             if (k >= ng) break;
-            uint32_t w[4];
-            __builtin_memcpy(w, carr(k) + base / 4, 16);
-            if (w[0] | w[1] | w[2] | w[3]) {
+            constexpr int NW = 16 * CB / 32;  // counter words of 16 pixels
+            uint32_t w[NW];
+            __builtin_memcpy(w, carr(k) + base / (32 / CB), sizeof(w));
+            uint32_t any = 0u;
+#pragma unroll
+            for (int x = 0; x < NW; ++x) any |= w[x];
+            if (any) {
               if (!ld) __builtin_memcpy(px, acc + base, sizeof(px));
               ld = true;
 #pragma unroll
               for (int j = 0; j < 16; ++j) {
-                const uint32_t c = (w[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+                const uint32_t c = (w[j / (32 / CB)] >> ((j % (32 / CB)) * CB)) & ((1u << CB) - 1u);
                 burns += c;
                 fold<T, FUN>(px[j], vk[k], bg, c);
               }
-              const uint32_t z[4] = {0u, 0u, 0u, 0u};
-              __builtin_memcpy(carr(k) + base / 4, z, 16);
+              const uint32_t z[NW] = {};
+              __builtin_memcpy(carr(k) + base / (32 / CB), z, sizeof(z));
             }
           }
           if (ld) __builtin_memcpy(acc + base, px, sizeof(px));
@@ -970,7 +978,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (ln) bnd = records(dl, rp, rn);
           // lines walked in groups: those whose writes are marks, or counted
           // ones (sum, count without dedup) of at most 255 records here
-          grp = ln && (!COUNTED ? true : (dedup_job || at_job) ? kind == 1u : (!unord && rn <= 255u));
+          grp = ln && (!COUNTED ? true : (dedup_job || at_job) ? kind == 1u : (!unord && rn < (1u << CB)));
         }
         const uint64_t ptm = __ballot(pt), lnm = __ballot(ln), grm = __ballot(grp);
         uint64_t todo = ptm | lnm | __ballot(po);

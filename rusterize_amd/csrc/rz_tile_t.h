@@ -910,22 +910,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int c0 = 0; c0 < PPL; c0 += 16) {
         const int base = lane * PPL + c0;
         uint32_t bk[KGRP];
-        uint32_t any = 0u;
+        uint32_t kmask = 0u;  // the items with marks in this lane's pixels
 #pragma unroll
         for (int k = 0; k < KGRP; ++k) {
           bk[k] = k < ng ? (mark[k * MW + (base >> 5)] >> (base & 31)) & 0xFFFFu : 0u;
-          any |= bk[k];
+          kmask |= bk[k] ? 1u << k : 0u;
         }
-        if (any) {
+        if (kmask) {
           T px[16];
           __builtin_memcpy(px, acc + base, sizeof(px));
+          // each lane takes its own items in order (a wave-wide loop over
+          // the items would fold every item in every lane)
+          while (kmask) {
+            const int k = __builtin_ctz(kmask);
+            kmask &= kmask - 1u;
+            uint32_t b = 0u;
+            T v = bg;
 #pragma unroll
-          for (int k = 0; k < KGRP; ++k) {
-            if (bk[k]) {
-              burns += (unsigned long long)__popc(bk[k]);
+            for (int j = 0; j < KGRP; ++j)
+              if (j == k) {
+                b = bk[j];
+                v = vk[j];
+              }
+            burns += (unsigned long long)__popc(b);
 #pragma unroll
-              for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], vk[k], bg, (bk[k] >> j) & 1u);
-            }
+            for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (b >> j) & 1u);
           }
           __builtin_memcpy(acc + base, px, sizeof(px));
         }

@@ -748,12 +748,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       return (uint32_t)lane < n_r ? recs[lane] : make_uint4(kLsNone, 0u, 0u, 0u);
     };
 
-    // The line items of lanes G (two or more, consecutive in item order):
-    // each lane's own item is dl with records [rp, rp + rn).  The lanes take
-    // the items' records in turn, item k writing mark array k (cmode false)
-    // or 8-bit counter array k (cmode: sum and count without dedup, at most
-    // 255 records per item, and a record writes a pixel at most once); then
-    // each lane folds its pixels' writes item by item.
+    // The line items of lanes G (two or more, consecutive in item order, at
+    // most 65,535 records each): each lane's own item is dl with records
+    // [rp, rp + rn).  The lanes take the items' records in turn, item k
+    // writing mark array k (cmode false) or CB-bit counter array k (cmode:
+    // sum and count without dedup, at most 2^CB - 1 records per item, and a
+    // record writes a pixel at most once); then each lane folds its pixels'
+    // writes item by item.
     auto line_group = [&](uint64_t G, const ItemDesc& dl, const uint4* rp, uint32_t rn, bool bnd, bool cmode) {
       GroupItem* gi = s_grp[wave];
       const int ng = __popcll(G);
@@ -986,8 +987,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           }
           if (ln) bnd = records(dl, rp, rn);
           // lines walked in groups: those whose writes are marks, or counted
-          // ones (sum, count without dedup) of at most 255 records here
-          grp = ln && (!COUNTED ? true : (dedup_job || at_job) ? kind == 1u : (!unord && rn < (1u << CB)));
+          // ones (sum, count without dedup) of at most 2^CB - 1 records here
+          grp = ln && rn <= 0xFFFFu &&
+                (!COUNTED ? true : (dedup_job || at_job) ? kind == 1u : (!unord && rn < (1u << CB)));
         }
         const uint64_t ptm = __ballot(pt), lnm = __ballot(ln), grm = __ballot(grp);
         uint64_t todo = ptm | lnm | __ballot(po);

@@ -748,30 +748,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       return (uint32_t)lane < n_r ? recs[lane] : make_uint4(kLsNone, 0u, 0u, 0u);
     };
 
-    // The line items of lanes G (two or more, consecutive in item order, at
-    // most 65,535 records each): each lane's own item is dl with records
-    // [rp, rp + rn).  The lanes take the items' records in turn, item k
-    // writing mark array k (cmode false) or CB-bit counter array k (cmode:
-    // sum and count without dedup, at most 2^CB - 1 records per item, and a
-    // record writes a pixel at most once); then each lane folds its pixels'
-    // writes item by item.
-    auto line_group = [&](uint64_t G, const ItemDesc& dl, const uint4* rp, uint32_t rn, bool bnd, bool cmode) {
+    // The items of lanes G, consecutive in item order: the line items L (two
+    // or more, at most 65,535 records each) and the one-vertex points
+    // between them (cell pidx, value bits pv in their lanes).  Each lane's
+    // own item is dl with records [rp, rp + rn).  The lanes take the lines'
+    // records in turn, line k writing mark array k (cmode false) or CB-bit
+    // counter array k (cmode: sum and count without dedup, at most
+    // 2^CB - 1 records per line, and a record writes a pixel at most once);
+    // then each lane folds its pixels' writes line by line, and the points
+    // are folded in order after them.  When a point's cell is also written
+    // by a line of the group, the members are folded one after the other in
+    // item order instead.
+    auto line_group = [&](uint64_t G, uint64_t L, const ItemDesc& dl, const uint4* rp, uint32_t rn, bool bnd,
+                          bool cmode, uint32_t pidx, uint64_t pv) {
       GroupItem* gi = s_grp[wave];
-      const int ng = __popcll(G);
-      if ((G >> lane) & 1ull) {
+      const int ng = __popcll(L);
+      const uint64_t P = COUNTED ? 0ull : G & ~L;  // (sum and count: no points, measured slower)
+      // the points, one lane at a time in item order
+      auto fold_points = [&]() {
+        for (uint64_t m = P; m; m &= m - 1ull) {
+          const int b = __builtin_ctzll(m);
+          const uint32_t idx = (uint32_t)__builtin_amdgcn_readlane((int)pidx, b);
+          const uint64_t vb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), b) << 32) |
+                              (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pv, b);
+          if (lane == 0) {
+            ++burns;
+            fold1(idx, from_bits<T>(vb));
+          }
+        }
+      };
+      if ((L >> lane) & 1ull) {
         GroupItem it;
         it.recs = rp;
         it.vbeg = dl.vbeg;
         it.last = dl.last;
         it.kb = (dl.kind_layer & 3u) | (bnd ? 4u : 0u);
         it.pad = 0u;
-        gi[__popcll(G & ((1ull << lane) - 1ull))] = it;
+        gi[__popcll(L & ((1ull << lane) - 1ull))] = it;
       }
-      // record offsets and values of the items (uniform)
+      // record offsets and values of the lines (uniform)
       uint32_t pre[KGRP];
       T vk[KGRP];
       uint32_t tot = 0;
-      uint64_t g = G;
+      uint64_t g = L;
 #pragma unroll
       for (int k = 0; k < KGRP; ++k) {
         pre[k] = tot;
@@ -871,6 +890,83 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
       }
       wave_sync();
+      // the writes of line k (uniform) at this lane's pixels, folded
+      auto apply_one = [&](int k) {
+        T v = bg;
+#pragma unroll
+        for (int j = 0; j < KGRP; ++j)
+          if (j == k) v = vk[j];
+#pragma unroll
+        for (int c0 = 0; c0 < PPL; c0 += 16) {
+          const int base = lane * PPL + c0;
+          if (COUNTED && cmode) {
+            constexpr int NW = 16 * CB / 32;
+            uint32_t w[NW];
+            __builtin_memcpy(w, carr(k) + base / (32 / CB), sizeof(w));
+            uint32_t any = 0u;
+#pragma unroll
+            for (int x = 0; x < NW; ++x) any |= w[x];
+            if (any) {
+              T px[16];
+              __builtin_memcpy(px, acc + base, sizeof(px));
+#pragma unroll
+              for (int j = 0; j < 16; ++j) {
+                const uint32_t c = (w[j / (32 / CB)] >> ((j % (32 / CB)) * CB)) & ((1u << CB) - 1u);
+                burns += c;
+                fold<T, FUN>(px[j], v, bg, c);
+              }
+              __builtin_memcpy(acc + base, px, sizeof(px));
+            }
+          } else {
+            const uint32_t b = (mark[k * MW + (base >> 5)] >> (base & 31)) & 0xFFFFu;
+            if (b) {
+              T px[16];
+              __builtin_memcpy(px, acc + base, sizeof(px));
+              burns += (unsigned long long)__popc(b);
+#pragma unroll
+              for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (b >> j) & 1u);
+              __builtin_memcpy(acc + base, px, sizeof(px));
+            }
+          }
+        }
+      };
+      // a point whose cell some line of the group writes: members in order
+      bool hit = false;
+      if ((P >> lane) & 1ull) {
+#pragma unroll
+        for (int k = 0; k < KGRP; ++k) {
+          if (k >= ng) break;
+          if (COUNTED && cmode)
+            hit |= ((carr(k)[pidx / (32 / CB)] >> ((pidx % (32 / CB)) * CB)) & ((1u << CB) - 1u)) != 0u;
+          else
+            hit |= ((mark[k * MW + (pidx >> 5)] >> (pidx & 31u)) & 1u) != 0u;
+        }
+      }
+      if (__ballot(hit)) {
+        int k = 0;
+        for (uint64_t m = G; m; m &= m - 1ull) {
+          const int b = __builtin_ctzll(m);
+          if ((L >> b) & 1ull) {
+            apply_one(k);
+            ++k;
+          } else {
+            const uint32_t idx = (uint32_t)__builtin_amdgcn_readlane((int)pidx, b);
+            const uint64_t vb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), b) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pv, b);
+            if (lane == 0) {
+              ++burns;
+              fold1(idx, from_bits<T>(vb));
+            }
+          }
+          wave_sync();
+        }
+        if (COUNTED && cmode)
+          for (int w = lane; w < ng * CW; w += 64) cnt[w] = 0u;
+        else
+          for (int w = lane; w < ng * MW; w += 64) mark[w] = 0u;
+        wave_sync();
+        return;
+      }
       if (COUNTED && cmode) {
         // each lane folds its pixels k times per item, item by item, and
         // clears their counters (its own words)
@@ -904,6 +1000,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (ld) __builtin_memcpy(acc + base, px, sizeof(px));
         }
         wave_sync();
+        fold_points();
         return;
       }
       // each lane folds its pixels' marks, item by item, and clears them
@@ -943,6 +1040,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       wave_sync();
       for (int w = lane; w < ng * MW; w += 64) mark[w] = 0u;
       wave_sync();
+      fold_points();
     };
 
     auto process_item = [&](const ItemDesc& d, uint32_t pi) {
@@ -1032,20 +1130,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             continue;
           }
           if ((grm >> p) & 1ull) {
-            // the run of groupable lines up to the next other item, at most KGRP
-            const uint64_t other = todo & ~grm & (~0ull << p);
-            uint64_t run = todo & grm & (other ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull);
+            // the run of groupable lines (and one-vertex points, except under
+            // sum and count, where the larger group's registers spill) up to
+            // the next other item, to its KGRP-th line
+            const uint64_t mem = COUNTED ? grm : (grm | ptm);
+            const uint64_t other = todo & ~mem & (~0ull << p);
+            const uint64_t run = todo & mem & (other ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull);
             const bool cmode = COUNTED && !dedup_job && !at_job;
-            uint64_t G = 0;
+            uint64_t lr = run & grm, L = 0;
 #pragma unroll
             for (int k = 0; k < KGRP; ++k) {
               if (!cmode || k < KCNT) {
-                G |= run & (0ull - run);
-                run &= run - 1ull;
+                L |= lr & (0ull - lr);
+                lr &= lr - 1ull;
               }
             }
-            if (G & (G - 1ull)) {
-              line_group(G, dl, rp, rn, bnd, cmode);
+            if (L & (L - 1ull)) {
+              const int hi = 63 - __builtin_clzll(L);
+              const uint64_t G = run & (hi == 63 ? ~0ull : ((2ull << hi) - 1ull));
+              line_group(G, L, dl, rp, rn, bnd, cmode, pidx, pv);
               todo &= ~G;
               lrest &= ~G;
               if (lrest) pre = prefetch(__builtin_ctzll(lrest));

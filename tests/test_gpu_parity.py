@@ -267,17 +267,31 @@ GROUP_CACHE = {}
 def line_group_geometry():
     """Runs of short line items that meet one strip together (walked as one
     group, each item writing its own marks or counters), overlapping so the
-    item order decides first / last / sum; in between a line of more than
-    255 segments in one strip (walked alone under sum and count), two-vertex
-    lines, multi-points with repeated cells, one-vertex points and a polygon,
-    each ending a run."""
+    item order decides first / last / sum, with one-vertex points between
+    them (on a cell of a line of the group, repeated, or elsewhere); in
+    between a line of more than 255 segments in one strip (walked alone
+    under sum and count), two-vertex lines, multi-points with repeated
+    cells, one-vertex points and a polygon, each ending a run."""
     if "g" not in GROUP_CACHE:
         rng = np.random.default_rng(41)
         wkts = []
         for run in range(12):
             x0, y0 = rng.uniform(0, 180, 2)
             for _ in range(int(rng.integers(2, 14))):
-                wkts.append(zigzag(x0, y0, 20.0, 10.0, int(rng.integers(2, 9)), int(rng.integers(1 << 30))))
+                z = zigzag(x0, y0, 20.0, 10.0, int(rng.integers(2, 9)), int(rng.integers(1 << 30)))
+                wkts.append(z)
+                # one-vertex points between the lines: on a vertex of the
+                # line just drawn (its cell is written by a line of the
+                # group), repeated, or elsewhere in the run's area
+                r = rng.random()
+                if r < 0.3:
+                    vx, vy = z[len("LINESTRING ("):].split(",")[0].split()
+                    wkts.append(f"POINT ({vx} {vy})")
+                    if r < 0.1:
+                        wkts.append(f"POINT ({vx} {vy})")
+                elif r < 0.6:
+                    px, py = rng.uniform(0, 20, 2)
+                    wkts.append(f"POINT ({x0 + px} {y0 + py / 2})")
             if run == 3:
                 wkts.append(zigzag(x0, y0 + 2.0, 60.0, 12.0, 400, 7))   # > 255 segments in one strip
             if run == 5:

@@ -396,11 +396,13 @@ struct GroupItem {
 //  * first / last / min / max / any give the same result for one or several
 //    writes of one value, so a line's cells set LDS mark bits (atomic OR,
 //    not waited for) and after the line each lane folds its marked pixels
-//    once.  A run of consecutive line items (up to 8 in a 64-wide strip)
-//    is walked together, its records spread over the lanes and each item
-//    marking its own array; one apply then folds each pixel's marks item by
-//    item, in item order (a line meets a strip in a few short segments, so
-//    one item alone leaves most lanes idle);
+//    once.  A run of consecutive line items (up to 8 in a 64-wide strip,
+//    with the one-vertex points between them) is walked together, its
+//    records spread over the lanes and each line marking its own array; one
+//    apply then folds each pixel's marks line by line, in item order (a line
+//    meets a strip in a few short segments, so one item alone leaves most
+//    lanes idle), and the points follow, or the members fold one by one in
+//    item order when a point's cell is also a line's;
 //  * sum and count depend on the number of writes: a line's cells are
 //    counted in 16-bit LDS counters, and after the line each lane folds its
 //    pixels k times.  Two consecutive lines of at most 255 segments each

@@ -1158,7 +1158,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             }
           }
           todo &= todo - 1;
-          const ItemDesc d = bcast_desc(dl, p);  // lane p's copy, no reload
+          // (reloaded with one uniform load: broadcasting lane p's copy kept
+          // every descriptor word live across the loop, measured slower)
+          const ItemDesc d = load_desc(pdesc_c + i0 + (uint32_t)p);
           if ((lnm >> p) & 1ull) {
             const uint4 cur = pre;
             lrest &= lrest - 1;

@@ -242,17 +242,6 @@ __device__ inline ItemDesc load_desc(const RZ_CONST ItemDesc* p) {
   return d;
 }
 
-// Lane p's descriptor in every lane (p wave-uniform).
-__device__ inline ItemDesc bcast_desc(const ItemDesc& dl, int p) {
-  uint32_t w[16];
-  __builtin_memcpy(w, &dl, sizeof(w));
-#pragma unroll
-  for (int k = 0; k < 16; ++k) w[k] = (uint32_t)__builtin_amdgcn_readlane((int)w[k], p);
-  ItemDesc d;
-  __builtin_memcpy(&d, w, sizeof(d));
-  return d;
-}
-
 // The box (r0, r1, c0, c1) and kind | band word of a descriptor: its first
 // 32 bytes (two 16-byte loads), all a box test needs.
 struct BoxLayer {

@@ -666,6 +666,7 @@ def serve_bench(args) -> int:
         """Arrivals, and the fresh stream jobs they need (generated here)."""
         rng = np.random.default_rng(seed)
         t, nxt, sets, reqs = 0.0, 0, [], []
+        fresh_bytes = [0.0]
         for _ in range(args.requests):
             t += rng.exponential(1.0 / rate)
             if sets and rng.random() < 0.3:  # cached geometry, new extent / field / function
@@ -679,6 +680,12 @@ def serve_bench(args) -> int:
                 k = (offset + nxt) % 4096
                 nxt += 1
                 sp = _lib.job_spec(BATCH_SEED, k)
+                # (every fresh set stays on the device for the run, beside the
+                X
+                fresh_bytes[0] += 32 * sp["n_vertices"] + 16 * sp["n_poly"]
+                if fresh_bytes[0] > 80e9:
+                    raise SystemExit(f"serve: --requests {args.requests} needs more than 80 GB of fresh geometry "
+                                     f"on the device beside the 200 GB budget; use fewer requests per rate")
                 dg = DeviceGeometry.synth_job(ctx, BATCH_SEED, k)
                 f0 = _lib.job_field(BATCH_SEED, k)
                 alt = np.random.default_rng(k).random(f0.size)

@@ -680,11 +680,12 @@ def serve_bench(args) -> int:
                 k = (offset + nxt) % 4096
                 nxt += 1
                 sp = _lib.job_spec(BATCH_SEED, k)
-                # (every fresh set stays on the device for the run, beside the
-                X
+                # (every fresh set stays on the device for the rate's run,
+                # beside the servers' 200 GB budget: at most about 100 GB at
+                # 32 B per vertex; the default 1,000 requests need 64-83 GB)
                 fresh_bytes[0] += 32 * sp["n_vertices"] + 16 * sp["n_poly"]
-                if fresh_bytes[0] > 80e9:
-                    raise SystemExit(f"serve: --requests {args.requests} needs more than 80 GB of fresh geometry "
+                if fresh_bytes[0] > 100e9:
+                    raise SystemExit(f"serve: --requests {args.requests} needs more than 100 GB of fresh geometry "
                                      f"on the device beside the 200 GB budget; use fewer requests per rate")
                 dg = DeviceGeometry.synth_job(ctx, BATCH_SEED, k)
                 f0 = _lib.job_field(BATCH_SEED, k)

@@ -768,6 +768,7 @@ __host__ __device__ inline void walk_window32_idx(int32_t ix0, int32_t iy0, int3
   const uint32_t du = s.xmaj ? (uint32_t)s.su : (uint32_t)(s.su * stride);
   const uint32_t dw = s.xmaj ? (uint32_t)(s.sw * stride) : (uint32_t)s.sw;
   uint32_t rem = s.rem;
+#pragma unroll 2  // (2 % faster on long line runs than no unrolling; 4 measured the same as 2)
   for (int32_t i = 0; i < s.cnt; ++i) {
     emit(idx);
     idx += du;

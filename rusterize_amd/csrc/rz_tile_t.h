@@ -760,12 +760,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // then each lane folds its pixels' writes line by line, and the points
     // are folded in order after them.  When a point's cell is also written
     // by a line of the group, the members are folded one after the other in
-    // item order instead.
+    // item order instead.  A counted group (two lines) always folds line 0,
+    // the points between them, then line 1.
     auto line_group = [&](uint64_t G, uint64_t L, const ItemDesc& dl, const uint4* rp, uint32_t rn, bool bnd,
                           bool cmode, uint32_t pidx, uint64_t pv) {
       GroupItem* gi = s_grp[wave];
       const int ng = __popcll(L);
-      const uint64_t P = COUNTED ? 0ull : G & ~L;  // (sum and count: no points, measured slower)
+      const uint64_t P = G & ~L;
       // the points, one lane at a time in item order
       auto fold_points = [&]() {
         for (uint64_t m = P; m; m &= m - 1ull) {
@@ -932,9 +933,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           }
         }
       };
+      if (COUNTED && cmode) {
+        // two lines with the points between them: line 0, the points, line 1
+        apply_one(0);
+        wave_sync();
+        fold_points();
+        wave_sync();
+        apply_one(1);
+        wave_sync();
+        for (int w = lane; w < ng * CW; w += 64) cnt[w] = 0u;
+        wave_sync();
+        return;
+      }
       // a point whose cell some line of the group writes: members in order
       bool hit = false;
-      if ((P >> lane) & 1ull) {
+      if (!(COUNTED && cmode) && ((P >> lane) & 1ull)) {
 #pragma unroll
         for (int k = 0; k < KGRP; ++k) {
           if (k >= ng) break;
@@ -967,42 +980,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         else
           for (int w = lane; w < ng * MW; w += 64) mark[w] = 0u;
         wave_sync();
-        return;
-      }
-      if (COUNTED && cmode) {
-        // each lane folds its pixels k times per item, item by item, and
-        // clears their counters (its own words)
-#pragma unroll
-        for (int c0 = 0; c0 < PPL; c0 += 16) {
-          const int base = lane * PPL + c0;
-          T px[16];
-          bool ld = false;
-#pragma unroll
-          for (int k = 0; k < KCNT; ++k) {
-            if (k >= ng) break;
-            constexpr int NW = 16 * CB / 32;  // counter words of 16 pixels
-            uint32_t w[NW];
-            __builtin_memcpy(w, carr(k) + base / (32 / CB), sizeof(w));
-            uint32_t any = 0u;
-#pragma unroll
-            for (int x = 0; x < NW; ++x) any |= w[x];
-            if (any) {
-              if (!ld) __builtin_memcpy(px, acc + base, sizeof(px));
-              ld = true;
-#pragma unroll
-              for (int j = 0; j < 16; ++j) {
-                const uint32_t c = (w[j / (32 / CB)] >> ((j % (32 / CB)) * CB)) & ((1u << CB) - 1u);
-                burns += c;
-                fold<T, FUN>(px[j], vk[k], bg, c);
-              }
-              const uint32_t z[NW] = {};
-              __builtin_memcpy(carr(k) + base / (32 / CB), z, sizeof(z));
-            }
-          }
-          if (ld) __builtin_memcpy(acc + base, px, sizeof(px));
-        }
-        wave_sync();
-        fold_points();
         return;
       }
       // each lane folds its pixels' marks, item by item, and clears them
@@ -1132,10 +1109,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             continue;
           }
           if ((grm >> p) & 1ull) {
-            // the run of groupable lines (and one-vertex points, except under
-            // sum and count, where the larger group's registers spill) up to
-            // the next other item, to its KGRP-th line
-            const uint64_t mem = COUNTED ? grm : (grm | ptm);
+            // the run of groupable lines and one-vertex points (not in dedup
+            // groups of sum and count) up to the next other item, to its
+            // KGRP-th line
+            const uint64_t mem = (COUNTED && (dedup_job || at_job)) ? grm : (grm | ptm);
             const uint64_t other = todo & ~mem & (~0ull << p);
             const uint64_t run = todo & mem & (other ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull);
             const bool cmode = COUNTED && !dedup_job && !at_job;

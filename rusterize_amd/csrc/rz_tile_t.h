@@ -405,9 +405,10 @@ struct GroupItem {
 //    item order when a point's cell is also a line's;
 //  * sum and count depend on the number of writes: a line's cells are
 //    counted in 16-bit LDS counters, and after the line each lane folds its
-//    pixels k times.  Two consecutive lines of at most 255 segments each
-//    in the strip are walked together, each counting in its own 8-bit
-//    counters (a segment writes a pixel once).  With xres != yres
+//    pixels k times.  Two lines of at most 255 segments each in the strip
+//    (with the one-vertex points between them) are walked together, each
+//    counting in its own 8-bit counters (a segment writes a pixel once),
+//    and fold line 0, the points, then line 1.  With xres != yres
 //    (LineWriter + PixelCache, burn_geometry.rs:179-185, 202-208) or
 //    all-touched a pixel is written once per line: a mark bit, folded once;
 //  * the unordered float sum adds every write with an LDS atomic;

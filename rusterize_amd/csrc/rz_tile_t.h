@@ -759,18 +759,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // counter array k (cmode: sum and count without dedup, at most
     // 2^CB - 1 records per line, and a record writes a pixel at most once);
     // then each lane folds its pixels' writes line by line, and the points
-    // are folded in order after them.  When a point's cell is also written
-    // by a line of the group, the members are folded one after the other in
-    // item order instead.  A counted group (two lines) always folds line 0,
-    // the points between them, then line 1.
+    // are folded in order after them, except that a point whose cell a line
+    // of the group also writes is folded between the lines before it and
+    // those after it.  A counted group (two lines) always folds line 0, the
+    // points between them, then line 1.
     auto line_group = [&](uint64_t G, uint64_t L, const ItemDesc& dl, const uint4* rp, uint32_t rn, bool bnd,
                           bool cmode, uint32_t pidx, uint64_t pv) {
       GroupItem* gi = s_grp[wave];
       const int ng = __popcll(L);
       const uint64_t P = G & ~L;
-      // the points, one lane at a time in item order
-      auto fold_points = [&]() {
-        for (uint64_t m = P; m; m &= m - 1ull) {
+      // the points of lanes pm, one at a time in item order
+      auto fold_points = [&](uint64_t pm) {
+        for (uint64_t m = pm; m; m &= m - 1ull) {
           const int b = __builtin_ctzll(m);
           const uint32_t idx = (uint32_t)__builtin_amdgcn_readlane((int)pidx, b);
           const uint64_t vb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), b) << 32) |
@@ -938,7 +938,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         // two lines with the points between them: line 0, the points, line 1
         apply_one(0);
         wave_sync();
-        fold_points();
+        fold_points(P);
         wave_sync();
         apply_one(1);
         wave_sync();
@@ -946,81 +946,70 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         wave_sync();
         return;
       }
-      // a point whose cell some line of the group writes: members in order
+      // the points whose cells a line of the group also writes (C): each is
+      // folded after the lines before it and before those after it; the
+      // others share no cell with a line and are folded last
       bool hit = false;
-      if (!(COUNTED && cmode) && ((P >> lane) & 1ull)) {
+      if ((P >> lane) & 1ull) {
 #pragma unroll
         for (int k = 0; k < KGRP; ++k) {
           if (k >= ng) break;
-          if (COUNTED && cmode)
-            hit |= ((carr(k)[pidx / (32 / CB)] >> ((pidx % (32 / CB)) * CB)) & ((1u << CB) - 1u)) != 0u;
-          else
-            hit |= ((mark[k * MW + (pidx >> 5)] >> (pidx & 31u)) & 1u) != 0u;
+          hit |= ((mark[k * MW + (pidx >> 5)] >> (pidx & 31u)) & 1u) != 0u;
         }
       }
-      if (__ballot(hit)) {
-        int k = 0;
-        for (uint64_t m = G; m; m &= m - 1ull) {
-          const int b = __builtin_ctzll(m);
-          if ((L >> b) & 1ull) {
-            apply_one(k);
-            ++k;
-          } else {
-            const uint32_t idx = (uint32_t)__builtin_amdgcn_readlane((int)pidx, b);
-            const uint64_t vb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), b) << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pv, b);
-            if (lane == 0) {
-              ++burns;
-              fold1(idx, from_bits<T>(vb));
+      const uint64_t C = __ballot(hit);
+      // each lane folds its pixels' marks of lines [lo, hi), line by line (a
+      // wave-wide loop over the lines would fold every line in every lane)
+      auto apply_range = [&](int lo, int hi) {
+#pragma unroll
+        for (int c0 = 0; c0 < PPL; c0 += 16) {
+          const int base = lane * PPL + c0;
+          uint32_t bk[KGRP];
+          uint32_t kmask = 0u;  // the lines with marks in this lane's pixels
+#pragma unroll
+          for (int k = 0; k < KGRP; ++k) {
+            bk[k] = (k >= lo && k < hi) ? (mark[k * MW + (base >> 5)] >> (base & 31)) & 0xFFFFu : 0u;
+            kmask |= bk[k] ? 1u << k : 0u;
+          }
+          if (kmask) {
+            T px[16];
+            __builtin_memcpy(px, acc + base, sizeof(px));
+            while (kmask) {
+              const int k = __builtin_ctz(kmask);
+              kmask &= kmask - 1u;
+              uint32_t b = 0u;
+              T v = bg;
+#pragma unroll
+              for (int j = 0; j < KGRP; ++j)
+                if (j == k) {
+                  b = bk[j];
+                  v = vk[j];
+                }
+              burns += (unsigned long long)__popc(b);
+#pragma unroll
+              for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (b >> j) & 1u);
             }
+            __builtin_memcpy(acc + base, px, sizeof(px));
           }
+        }
+      };
+      int lo = 0;
+      for (uint64_t m = C; m; m &= m - 1ull) {
+        const int b = __builtin_ctzll(m);
+        const int r = __popcll(L & ((1ull << b) - 1ull));  // the lines before the point
+        if (r > lo) {
+          apply_range(lo, r);
           wave_sync();
+          lo = r;
         }
-        if (COUNTED && cmode)
-          for (int w = lane; w < ng * CW; w += 64) cnt[w] = 0u;
-        else
-          for (int w = lane; w < ng * MW; w += 64) mark[w] = 0u;
+        fold_points(1ull << b);
         wave_sync();
-        return;
       }
-      // each lane folds its pixels' marks, item by item, and clears them
-#pragma unroll
-      for (int c0 = 0; c0 < PPL; c0 += 16) {
-        const int base = lane * PPL + c0;
-        uint32_t bk[KGRP];
-        uint32_t kmask = 0u;  // the items with marks in this lane's pixels
-#pragma unroll
-        for (int k = 0; k < KGRP; ++k) {
-          bk[k] = k < ng ? (mark[k * MW + (base >> 5)] >> (base & 31)) & 0xFFFFu : 0u;
-          kmask |= bk[k] ? 1u << k : 0u;
-        }
-        if (kmask) {
-          T px[16];
-          __builtin_memcpy(px, acc + base, sizeof(px));
-          // each lane takes its own items in order (a wave-wide loop over
-          // the items would fold every item in every lane)
-          while (kmask) {
-            const int k = __builtin_ctz(kmask);
-            kmask &= kmask - 1u;
-            uint32_t b = 0u;
-            T v = bg;
-#pragma unroll
-            for (int j = 0; j < KGRP; ++j)
-              if (j == k) {
-                b = bk[j];
-                v = vk[j];
-              }
-            burns += (unsigned long long)__popc(b);
-#pragma unroll
-            for (int j = 0; j < 16; ++j) px[j] = fold_select<T, FUN>(px[j], v, bg, (b >> j) & 1u);
-          }
-          __builtin_memcpy(acc + base, px, sizeof(px));
-        }
-      }
+      apply_range(lo, ng);
       wave_sync();
       for (int w = lane; w < ng * MW; w += 64) mark[w] = 0u;
       wave_sync();
-      fold_points();
+      fold_points(P & ~C);
     };
 
     auto process_item = [&](const ItemDesc& d, uint32_t pi) {

@@ -401,8 +401,8 @@ struct GroupItem {
 //    records spread over the lanes and each line marking its own array; one
 //    apply then folds each pixel's marks line by line, in item order (a line
 //    meets a strip in a few short segments, so one item alone leaves most
-//    lanes idle), and the points follow, or the members fold one by one in
-//    item order when a point's cell is also a line's;
+//    lanes idle), and the points follow (a point whose cell a line of the
+//    group also writes folds between the lines before and after it);
 //  * sum and count depend on the number of writes: a line's cells are
 //    counted in 16-bit LDS counters, and after the line each lane folds its
 //    pixels k times.  Two lines of at most 255 segments each in the strip
